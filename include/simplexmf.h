@@ -1,0 +1,147 @@
+/* simplexmf — matrix-free tetrahedral finite-element operators on MI355X.
+ *
+ * C ABI.  Every entry point returns 0 on success and a negative code on
+ * failure; mf_last_error() then returns a thread-local message.  Vectors are
+ * device pointers (FP64 or FP32 as chosen at creation), streams are
+ * hipStream_t values passed as void* (NULL = default stream).  Calls are
+ * stream-ordered; only the create/setup calls synchronise.  dst and src must
+ * not alias.
+ *
+ * Reference interface each entry replaces (paths relative to the
+ * simplex-benchmarks repository):
+ *   Operator::reinit                CG/cells_batched/application.cc:1056-1128
+ *                                   DG/cells_batched/application.cc:1170-1277
+ *   Operator::vmult_masked_gather   CG/cells_batched/application.cc:1165-1176
+ *   Operator::vmult_manual          DG/cells_batched/application.cc:1318-1332
+ *   Operator::Tvmult                CG/cells_batched/application.cc:1192-1196
+ *   Operator::m                     CG/cells_batched/application.cc:1130-1138
+ *   Operator::compute_inverse_diagonal CG/cells_batched/application_hp.cc:1386-1409
+ */
+#ifndef SIMPLEXMF_H
+#define SIMPLEXMF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mf_operator* mf_op_t;
+
+enum mf_kind {
+  MF_CG_POISSON = 0,   /* scalar Poisson, continuous P_k, homogeneous Dirichlet rows = identity */
+  MF_DG_SIPG = 1,      /* scalar Poisson, discontinuous P_k, symmetric interior penalty */
+  MF_CG_VECTOR = 2,    /* 3-component Laplacian, continuous P_k (components batched) */
+  MF_DG_VECTOR = 3,    /* 3-component SIPG Laplacian (components batched) */
+  MF_DG_HELMHOLTZ = 4  /* 3-component mass*u + laplace*SIPG (components batched) */
+};
+
+enum mf_precision { MF_F64 = 0, MF_F32 = 1 };
+
+typedef struct mf_mesh_desc {
+  int64_t n_vertices;
+  const double* vertices; /* 3*n_vertices, x y z per vertex */
+  int64_t n_cells;
+  const int32_t* cells;   /* 4*n_cells vertex ids per tetrahedron */
+  int32_t kuhn_n;         /* >0: mesh came from mf_mesh_kuhn_cube(kuhn_n, ...) (enables h-transfers) */
+} mf_mesh_desc;
+
+typedef struct mf_options {
+  int32_t reorder;          /* 1: hierarchical cell reordering (default); 0: keep input order */
+  int32_t patch_cells;      /* cells per patch (0: default for the degree) */
+  double mass_coefficient;  /* Helmholtz mass coefficient (default 1) */
+  double laplace_coefficient; /* Helmholtz Laplace coefficient (default 1) */
+  int32_t penalty_kind;     /* 0: (p+1)(p+3)/3 max(tau-,tau+) (reference); 1: (p+1)(p+2)/h_face */
+  int32_t host_only;        /* 1: build the host plan only (no device memory); device entry points fail */
+} mf_options;
+
+typedef struct mf_info {
+  int64_t n_dofs;          /* == mf_m() */
+  int64_t n_cells;
+  int64_t n_constrained;   /* Dirichlet rows (CG) */
+  int32_t degree, dofs_per_cell, n_q_points, n_components;
+  int64_t n_patches;
+  int32_t n_patch_colors, max_cell_colors, max_local_dofs, precision;
+  int64_t n_interior_faces, n_boundary_faces;
+  double algorithmic_bytes; /* per vmult, reference data model */
+  double flops;             /* per vmult */
+} mf_info;
+
+/* ---- meshes (host) ------------------------------------------------------ */
+
+/* Unit cube split into n^3 hexes x 6 Kuhn tetrahedra; interior vertices
+ * displaced per coordinate by perturb*h*U[-1,1) from splitmix64(seed).
+ * vertices: 3*(n+1)^3 doubles, cells: 4*6*n^3 int32 (caller-allocated). */
+int mf_mesh_kuhn_cube(int32_t n, double perturb, uint64_t seed, double* vertices,
+                      int32_t* cells);
+
+/* ---- operator lifetime -------------------------------------------------- */
+
+void mf_default_options(mf_options* opt);
+
+/* Replaces Operator::reinit: builds DoF numbering, hierarchical patches,
+ * colourings, geometry and device tables.  Synchronises the device. */
+int mf_create(const mf_mesh_desc* mesh, int32_t degree, int32_t kind, int32_t precision,
+              const mf_options* opt, mf_op_t* out);
+int mf_destroy(mf_op_t op);
+
+/* ---- the operator (device) ---------------------------------------------- */
+
+/* dst = A src (overwrites dst).  CG: Dirichlet rows are identity rows,
+ * dst_c = src_c (CG/cells_batched/application.cc:1173-1175). */
+int mf_vmult(mf_op_t op, void* dst, const void* src, void* stream);
+/* A is symmetric: Tvmult == vmult (CG/cells_batched/application.cc:1192-1196). */
+int mf_tvmult(mf_op_t op, void* dst, const void* src, void* stream);
+/* diag_i = e_i^T A e_i (Dirichlet rows: 1). */
+int mf_compute_diagonal(mf_op_t op, void* diag, void* stream);
+/* inv_i = 1/diag_i if |diag_i| > 1e-10 else 1 (application_hp.cc:1404-1407). */
+int mf_compute_inverse_diagonal(mf_op_t op, void* inv, void* stream);
+
+/* ---- queries (host) ----------------------------------------------------- */
+
+int64_t mf_m(mf_op_t op);   /* number of rows (all components) */
+int mf_get_info(mf_op_t op, mf_info* info);
+/* internal cell i is input cell perm[i] (n_cells int64) */
+int mf_get_cell_order(mf_op_t op, int64_t* perm);
+/* per internal cell, the global scalar DoF of each local node (n_cells*dofs_per_cell int64);
+ * for n_components > 1, component c of scalar DoF d is row c*n_scalar + d (CG)
+ * or, for DG, rows are cell-contiguous: see mf_get_info. */
+int mf_get_cell_dofs(mf_op_t op, int64_t* dofs);
+/* support point of every scalar DoF (3*n_scalar_dofs doubles) */
+int mf_get_dof_points(mf_op_t op, double* xyz);
+/* 1 for Dirichlet-constrained scalar DoFs (n_scalar_dofs bytes) */
+int mf_get_constrained(mf_op_t op, uint8_t* mask);
+
+const char* mf_last_error(void);
+
+/* Runs the device algorithm's patch/colour schedule on the host, in the
+ * operator's precision, from the same tables the kernels read (mode 0: vmult,
+ * 1: diagonal).  Host vectors.  A self-check of the setup for machines without
+ * a GPU; not an operator path (mf_vmult never calls it). */
+int mf_debug_host_apply(mf_op_t op, int32_t mode, void* dst, const void* src);
+
+/* ---- device plumbing (host) ----------------------------------------------
+ * Thin wrappers over the HIP runtime this library is linked against, so that
+ * callers without their own HIP bindings (tests, benchmarks) share one runtime
+ * with the operators.  No reference counterpart (the reference is CPU-only). */
+int mf_device_count(int32_t* n);
+int mf_device_name(char* buf, int32_t len);
+int mf_malloc(void** ptr, uint64_t bytes);
+int mf_free(void* ptr);
+int mf_memcpy_htod(void* dst, const void* src, uint64_t bytes);
+int mf_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+int mf_memcpy_dtod(void* dst, const void* src, uint64_t bytes, void* stream);
+int mf_memset(void* ptr, int32_t value, uint64_t bytes, void* stream);
+int mf_synchronize(void);
+int mf_stream_create(void** stream);
+int mf_stream_destroy(void* stream);
+int mf_stream_synchronize(void* stream);
+int mf_event_create(void** event);
+int mf_event_destroy(void* event);
+int mf_event_record(void* event, void* stream);
+int mf_event_elapsed_ms(void* start, void* stop, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,61 @@
+/* CPU oracle for the simplexmf operators.
+ *
+ * TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg call this library as the checker; the product never links
+ * or loads it.
+ *
+ * It restates the reference algorithm in plain C, independently of the
+ * product's tables: its own nodal basis (Vandermonde inversion over monomials
+ * from the node-position convention), its own quadrature (collapsed
+ * Gauss-Legendre product rules of high degree), its own geometry.
+ */
+#ifndef SMF_ORACLE_H
+#define SMF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_mesh {
+  int64_t n_vertices;
+  const double* vertices; /* 3*n_vertices */
+  int64_t n_cells;
+  const int32_t* cells;   /* 4*n_cells, in the order of the cell_dofs table */
+} orc_mesh;
+
+/* Nodal basis of degree p on the unit tetrahedron at x: values[n_loc], grads[3*n_loc]. */
+int orc_basis(int p, const double* x, double* values, double* grads);
+/* Collapsed Gauss-Legendre rule with n points per direction (n^3 points):
+ * pts[3*n^3], w[n^3]. Exact for polynomials of degree <= 2n-3. */
+int orc_tet_rule(int n, double* pts, double* w);
+/* Gauss-Legendre on [0,1] */
+int orc_gauss01(int n, double* x, double* w);
+
+/* CG Poisson vmult (CG/cells_batched/application.cc:1236-1416 + 1173-1175):
+ * dst = A src with identity rows on constrained DoFs. */
+int orc_cg_vmult(const orc_mesh* m, int p, const int64_t* cell_dofs, const uint8_t* constrained,
+                 int64_t n_dofs, const double* src, double* dst);
+/* same, OpenMP over cell colours (CPU baseline); colors[n_cells] from orc_cg_colors */
+int orc_cg_vmult_omp(const orc_mesh* m, int p, const int64_t* cell_dofs, const uint8_t* constrained,
+                     int64_t n_dofs, const int32_t* color_ptr, const int64_t* color_cells,
+                     int n_colors, const double* src, double* dst);
+/* greedy colouring of cells sharing a DoF; returns number of colours, fills
+ * color_ptr[n_colors+1] (caller gives room for 256+1) and color_cells[n_cells] */
+int orc_cg_colors(const orc_mesh* m, int p, const int64_t* cell_dofs, int64_t n_dofs,
+                  int32_t* color_ptr, int64_t* color_cells);
+/* diagonal of A (constrained rows: 1) (application_hp.cc:1386-1409) */
+int orc_cg_diagonal(const orc_mesh* m, int p, const int64_t* cell_dofs, const uint8_t* constrained,
+                    int64_t n_dofs, double* diag);
+/* independent matrix-based check: assemble A as CSR from element matrices and
+ * multiply (the reference's own matrix comparison, application.cc:1944-1988) */
+int orc_cg_matrix_vmult(const orc_mesh* m, int p, const int64_t* cell_dofs,
+                        const uint8_t* constrained, int64_t n_dofs, const double* src, double* dst);
+/* dense A for tiny meshes (n_dofs^2 doubles, row-major) */
+int orc_cg_dense(const orc_mesh* m, int p, const int64_t* cell_dofs, const uint8_t* constrained,
+                 int64_t n_dofs, double* A);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

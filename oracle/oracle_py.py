@@ -1,0 +1,125 @@
+"""ctypes loader of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+
+
+class OrcMesh(C.Structure):
+    _fields_ = [("n_vertices", C.c_int64), ("vertices", C.c_void_p),
+                ("n_cells", C.c_int64), ("cells", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+        _lib = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        for name in ["orc_cg_vmult", "orc_cg_diagonal", "orc_cg_matrix_vmult", "orc_cg_dense"]:
+            getattr(_lib, name).restype = C.c_int
+        _lib.orc_basis.argtypes = [C.c_int, vp, vp, vp]
+        _lib.orc_tet_rule.argtypes = [C.c_int, vp, vp]
+        _lib.orc_gauss01.argtypes = [C.c_int, vp, vp]
+    return _lib
+
+
+class OracleProblem:
+    """A mesh in the operator's internal cell order plus its DoF table."""
+
+    def __init__(self, vertices, cells, cell_dofs, constrained, degree):
+        self.vertices = np.ascontiguousarray(vertices, np.float64)
+        self.cells = np.ascontiguousarray(cells, np.int32)
+        self.cell_dofs = np.ascontiguousarray(cell_dofs, np.int64)
+        self.constrained = np.ascontiguousarray(constrained, np.uint8)
+        self.degree = int(degree)
+        self.n_dofs = int(self.constrained.size)
+        self._m = OrcMesh(self.vertices.shape[0], self.vertices.ctypes.data,
+                          self.cells.shape[0], self.cells.ctypes.data)
+
+    @staticmethod
+    def from_operator(op):
+        perm = op.cell_order()
+        return OracleProblem(op.mesh.vertices, op.mesh.cells[perm], op.cell_dofs(),
+                             op.constrained(), op.info["degree"])
+
+    def _call(self, fn, *args):
+        rc = fn(C.byref(self._m), self.degree, C.c_void_p(self.cell_dofs.ctypes.data),
+                C.c_void_p(self.constrained.ctypes.data), C.c_int64(self.n_dofs), *args)
+        if rc < 0:
+            raise RuntimeError(f"oracle call failed ({rc})")
+        return rc
+
+    def vmult(self, src):
+        src = np.ascontiguousarray(src, np.float64)
+        dst = np.empty(self.n_dofs)
+        self._call(lib().orc_cg_vmult, C.c_void_p(src.ctypes.data), C.c_void_p(dst.ctypes.data))
+        return dst
+
+    def matrix_vmult(self, src):
+        src = np.ascontiguousarray(src, np.float64)
+        dst = np.empty(self.n_dofs)
+        self._call(lib().orc_cg_matrix_vmult, C.c_void_p(src.ctypes.data), C.c_void_p(dst.ctypes.data))
+        return dst
+
+    def diagonal(self):
+        d = np.empty(self.n_dofs)
+        self._call(lib().orc_cg_diagonal, C.c_void_p(d.ctypes.data))
+        return d
+
+    def dense(self):
+        A = np.empty((self.n_dofs, self.n_dofs))
+        self._call(lib().orc_cg_dense, C.c_void_p(A.ctypes.data))
+        return A
+
+    def colors(self):
+        cptr = np.zeros(258, np.int32)
+        ccells = np.empty(self.cells.shape[0], np.int64)
+        L = lib()
+        n = L.orc_cg_colors(C.byref(self._m), self.degree, C.c_void_p(self.cell_dofs.ctypes.data),
+                            C.c_int64(self.n_dofs), C.c_void_p(cptr.ctypes.data),
+                            C.c_void_p(ccells.ctypes.data))
+        if n < 0:
+            raise RuntimeError("oracle colouring failed")
+        return cptr[: n + 1].copy(), ccells, n
+
+    def vmult_omp(self, src, colors=None, dst=None):
+        if colors is None:
+            colors = self.colors()
+        cptr, ccells, n = colors
+        src = np.ascontiguousarray(src, np.float64)
+        if dst is None:
+            dst = np.empty(self.n_dofs)
+        self._call(lib().orc_cg_vmult_omp, C.c_void_p(cptr.ctypes.data),
+                   C.c_void_p(ccells.ctypes.data), C.c_int(n), C.c_void_p(src.ctypes.data),
+                   C.c_void_p(dst.ctypes.data))
+        return dst
+
+
+def basis(p, x):
+    nl = (p + 1) * (p + 2) * (p + 3) // 6
+    x = np.ascontiguousarray(x, np.float64)
+    v = np.empty(nl)
+    g = np.empty((nl, 3))
+    if lib().orc_basis(p, x.ctypes.data, v.ctypes.data, g.ctypes.data):
+        raise RuntimeError("orc_basis failed")
+    return v, g
+
+
+def tet_rule(n):
+    pts = np.empty((n ** 3, 3))
+    w = np.empty(n ** 3)
+    if lib().orc_tet_rule(n, pts.ctypes.data, w.ctypes.data):
+        raise RuntimeError("orc_tet_rule failed")
+    return pts, w

@@ -1,0 +1,314 @@
+// C ABI of the operator library (declared in include/simplexmf.h).
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "op.h"
+
+namespace smf {
+
+int cg_quadrature_degree(int p);
+void launch_invert_diagonal(int precision, int64_t n, void* d, hipStream_t s);
+
+thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+template <typename T>
+static std::vector<T> convert(const std::vector<double>& v) {
+  return std::vector<T>(v.begin(), v.end());
+}
+
+void make_cg_shape(CGPlan& pl) {
+  // reference shape gradients at the quadrature points, then the weighted copy
+  const Lagrange fe(pl.p);
+  const Quadrature q = tet_rule(cg_quadrature_degree(pl.p));
+  pl.n_q = q.size();
+  pl.shape.assign(2 * pl.n_loc * pl.n_q * 3, 0.0);
+  std::vector<double> grads(pl.n_loc * 3);
+  for (int k = 0; k < pl.n_q; ++k) {
+    fe.eval(q.pts[k].data(), nullptr, grads.data());
+    for (int i = 0; i < pl.n_loc; ++i)
+      for (int d = 0; d < 3; ++d) {
+        pl.shape[(i * pl.n_q + k) * 3 + d] = grads[i * 3 + d];
+        pl.shape[pl.n_loc * pl.n_q * 3 + (i * pl.n_q + k) * 3 + d] = q.w[k] * grads[i * 3 + d];
+      }
+  }
+}
+
+void upload_cg(Operator& op) {
+  CGPlan& pl = op.cg;
+  op.d_cell_start.upload(pl.cell_start);
+  op.d_own_start.upload(pl.own_start);
+  op.d_ghost_start.upload(pl.ghost_start);
+  op.d_ghost_idx.upload(pl.ghost_idx);
+  op.d_icol_ptr.upload(pl.icol_ptr);
+  op.d_icol_off.upload(pl.icol_off);
+  op.d_color_patches.upload(pl.color_patches);
+  op.d_lidx.upload(pl.lidx);
+  if (op.precision == MF_F64) {
+    op.d_metric.upload(pl.metric);
+    op.d_shape.upload(pl.shape);
+  } else {
+    op.d_metric.upload(convert<float>(pl.metric));
+    op.d_shape.upload(convert<float>(pl.shape));
+  }
+  op.on_device = true;
+}
+
+namespace {
+// Same schedule as cg_patch_kernel, executed serially on the host.
+template <typename T>
+void host_apply_cg_t(const Operator& op, int mode, T* dst, const T* src) {
+  const CGPlan& pl = op.cg;
+  const int nl = pl.n_loc, nq = pl.n_q;
+  std::vector<T> G(pl.shape.begin(), pl.shape.end());
+  const T* GW = G.data() + nl * nq * 3;
+  std::vector<T> su, sy;
+  for (int c = 0; c < pl.n_colors; ++c)
+    for (int32_t k = pl.color_ptr[c]; k < pl.color_ptr[c + 1]; ++k) {
+      const int patch = pl.color_patches[k];
+      const int c0 = pl.cell_start[patch], nc = pl.cell_start[patch + 1] - c0;
+      const int o0 = pl.own_start[patch], no = pl.own_start[patch + 1] - o0;
+      const int g0 = pl.ghost_start[patch], ng = pl.ghost_start[patch + 1] - g0;
+      su.assign(no + ng, T(0));
+      sy.assign(no + ng, T(0));
+      if (mode == 0)
+        for (int j = 0; j < no + ng; ++j) su[j] = j < no ? src[o0 + j] : src[pl.ghost_idx[g0 + j - no]];
+      std::vector<T> y((size_t)nc * nl, T(0));
+      for (int lc = 0; lc < nc; ++lc) {
+        const int64_t cell = c0 + lc;
+        T m[6];
+        for (int t = 0; t < 6; ++t) m[t] = (T)pl.metric[(size_t)t * pl.n_cells + cell];
+        T u[20];
+        for (int i = 0; i < nl; ++i) {
+          const uint16_t li = pl.lidx[(size_t)i * pl.n_cells + cell];
+          u[i] = (mode == 0 && li != 0xFFFF) ? su[li] : T(0);
+        }
+        T* yc = &y[(size_t)lc * nl];
+        for (int q = 0; q < nq; ++q) {
+          if (mode == 0) {
+            T g[3] = {0, 0, 0};
+            for (int i = 0; i < nl; ++i)
+              for (int d = 0; d < 3; ++d) g[d] += G[(i * nq + q) * 3 + d] * u[i];
+            const T tx = m[0] * g[0] + m[1] * g[1] + m[2] * g[2];
+            const T ty = m[1] * g[0] + m[3] * g[1] + m[4] * g[2];
+            const T tz = m[2] * g[0] + m[4] * g[1] + m[5] * g[2];
+            for (int i = 0; i < nl; ++i)
+              yc[i] += GW[(i * nq + q) * 3] * tx + GW[(i * nq + q) * 3 + 1] * ty + GW[(i * nq + q) * 3 + 2] * tz;
+          } else {
+            for (int i = 0; i < nl; ++i) {
+              const T* g = &G[(i * nq + q) * 3];
+              const T tx = m[0] * g[0] + m[1] * g[1] + m[2] * g[2];
+              const T ty = m[1] * g[0] + m[3] * g[1] + m[4] * g[2];
+              const T tz = m[2] * g[0] + m[4] * g[1] + m[5] * g[2];
+              yc[i] += GW[(i * nq + q) * 3] * tx + GW[(i * nq + q) * 3 + 1] * ty + GW[(i * nq + q) * 3 + 2] * tz;
+            }
+          }
+        }
+      }
+      const int cb = pl.icol_ptr[patch], ncol = pl.icol_ptr[patch + 1] - cb - 1;
+      for (int col = 0; col < ncol; ++col)
+        for (int lc = pl.icol_off[cb + col]; lc < pl.icol_off[cb + col + 1]; ++lc)
+          for (int i = 0; i < nl; ++i) {
+            const uint16_t li = pl.lidx[(size_t)i * pl.n_cells + c0 + lc];
+            if (li != 0xFFFF) sy[li] += y[(size_t)lc * nl + i];
+          }
+      for (int j = 0; j < no + ng; ++j) {
+        if (j < no) dst[o0 + j] = sy[j];
+        else dst[pl.ghost_idx[g0 + j - no]] += sy[j];
+      }
+    }
+  for (int64_t i = pl.n_free; i < pl.n_dofs; ++i) dst[i] = mode == 0 ? src[i] : T(1);
+}
+}  // namespace
+
+void host_apply_cg(const Operator& op, int mode, void* dst, const void* src) {
+  if (op.precision == MF_F64)
+    host_apply_cg_t<double>(op, mode, static_cast<double*>(dst), static_cast<const double*>(src));
+  else
+    host_apply_cg_t<float>(op, mode, static_cast<float*>(dst), static_cast<const float*>(src));
+}
+
+}  // namespace smf
+
+using namespace smf;
+
+struct mf_operator {
+  Operator op;
+};
+
+#define MF_TRY(body)                      \
+  try {                                   \
+    body;                                 \
+    return 0;                             \
+  } catch (const std::exception& e) {     \
+    set_error(e.what());                  \
+    return -1;                            \
+  } catch (...) {                         \
+    set_error("unknown error");           \
+    return -1;                            \
+  }
+
+static void require_device(const Operator& op) {
+  if (!op.on_device) throw std::runtime_error("operator was created with host_only=1: no device data");
+}
+
+extern "C" {
+
+int mf_debug_host_apply(mf_op_t h, int32_t mode, void* dst, const void* src) {
+  MF_TRY({
+    if (!h || !dst || (mode == 0 && !src)) throw std::runtime_error("null argument");
+    if (mode != 0 && mode != 1) throw std::runtime_error("mode must be 0 or 1");
+    host_apply_cg(h->op, mode, dst, src);
+  });
+}
+
+const char* mf_last_error(void) { return g_last_error.c_str(); }
+
+void mf_default_options(mf_options* opt) {
+  std::memset(opt, 0, sizeof(*opt));
+  opt->reorder = 1;
+  opt->patch_cells = 0;
+  opt->mass_coefficient = 1.0;
+  opt->laplace_coefficient = 1.0;
+  opt->penalty_kind = 0;
+  opt->host_only = 0;
+}
+
+int mf_mesh_kuhn_cube(int32_t n, double perturb, uint64_t seed, double* vertices, int32_t* cells) {
+  MF_TRY({
+    if (!vertices || !cells) throw std::runtime_error("null output array");
+    Mesh m = kuhn_cube(n, perturb, seed);
+    std::memcpy(vertices, m.x.data(), m.x.size() * sizeof(double));
+    std::memcpy(cells, m.cells.data(), m.cells.size() * sizeof(int32_t));
+  });
+}
+
+int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t precision,
+              const mf_options* opt, mf_op_t* out) {
+  MF_TRY({
+    if (!desc || !out) throw std::runtime_error("null argument");
+    *out = nullptr;
+    if (degree < 1 || degree > 3) throw std::runtime_error("degree must be 1, 2 or 3");
+    if (precision != MF_F64 && precision != MF_F32) throw std::runtime_error("precision must be MF_F64 or MF_F32");
+    if (kind != MF_CG_POISSON) throw std::runtime_error("operator kind not available in this build");
+    if (desc->n_vertices <= 0 || desc->n_cells < 0 || !desc->vertices || (!desc->cells && desc->n_cells))
+      throw std::runtime_error("invalid mesh description");
+    auto h = std::make_unique<mf_operator>();
+    Operator& op = h->op;
+    op.kind = kind;
+    op.precision = precision;
+    op.degree = degree;
+    if (opt) op.opt = *opt; else mf_default_options(&op.opt);
+    op.mesh.nv = desc->n_vertices;
+    op.mesh.nc = desc->n_cells;
+    op.mesh.x.assign(desc->vertices, desc->vertices + 3 * desc->n_vertices);
+    op.mesh.cells.assign(desc->cells, desc->cells + 4 * desc->n_cells);
+    for (int32_t v : op.mesh.cells)
+      if (v < 0 || v >= desc->n_vertices) throw std::runtime_error("cell references a missing vertex");
+    op.mesh.kuhn_n = desc->kuhn_n;
+    build_cg_plan(op.mesh, degree, op.opt, op.cg);
+    make_cg_shape(op.cg);
+    if (!op.opt.host_only) {
+      upload_cg(op);
+      SMF_HIP(hipDeviceSynchronize());
+    }
+    *out = h.release();
+  });
+}
+
+int mf_destroy(mf_op_t op) {
+  MF_TRY({ delete op; });
+}
+
+int mf_vmult(mf_op_t h, void* dst, const void* src, void* stream) {
+  MF_TRY({
+    if (!h || !dst || !src) throw std::runtime_error("null argument");
+    if (dst == src) throw std::runtime_error("dst and src must not alias");
+    require_device(h->op);
+    launch_cg(h->op, 0, dst, src, static_cast<hipStream_t>(stream));
+  });
+}
+
+int mf_tvmult(mf_op_t h, void* dst, const void* src, void* stream) {
+  return mf_vmult(h, dst, src, stream);
+}
+
+int mf_compute_diagonal(mf_op_t h, void* diag, void* stream) {
+  MF_TRY({
+    if (!h || !diag) throw std::runtime_error("null argument");
+    require_device(h->op);
+    launch_cg(h->op, 1, diag, nullptr, static_cast<hipStream_t>(stream));
+  });
+}
+
+int mf_compute_inverse_diagonal(mf_op_t h, void* inv, void* stream) {
+  MF_TRY({
+    if (!h || !inv) throw std::runtime_error("null argument");
+    require_device(h->op);
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    launch_cg(h->op, 1, inv, nullptr, s);
+    launch_invert_diagonal(h->op.precision, h->op.m(), inv, s);
+  });
+}
+
+int64_t mf_m(mf_op_t h) { return h ? h->op.m() : -1; }
+
+int mf_get_info(mf_op_t h, mf_info* info) {
+  MF_TRY({
+    if (!h || !info) throw std::runtime_error("null argument");
+    const Operator& op = h->op;
+    const CGPlan& pl = op.cg;
+    std::memset(info, 0, sizeof(*info));
+    info->n_dofs = op.m();
+    info->n_cells = pl.n_cells;
+    info->n_constrained = pl.n_dofs - pl.n_free;
+    info->degree = op.degree;
+    info->dofs_per_cell = pl.n_loc;
+    info->n_q_points = pl.n_q;
+    info->n_components = op.ncomp;
+    info->n_patches = pl.n_patches;
+    info->n_patch_colors = pl.n_colors;
+    info->max_cell_colors = pl.max_icolors;
+    info->max_local_dofs = pl.max_local;
+    info->precision = op.precision;
+    const double w = (double)op.wbytes();
+    // reference data model: src + dst, 32-bit index per cell DoF, J^-T + det J per cell
+    info->algorithmic_bytes = 2.0 * w * pl.n_dofs + 4.0 * pl.n_loc * pl.n_cells + 10.0 * w * pl.n_cells;
+    info->flops = (12.0 * pl.n_loc * pl.n_q + 18.0 * pl.n_q) * pl.n_cells;
+  });
+}
+
+int mf_get_cell_order(mf_op_t h, int64_t* perm) {
+  MF_TRY({
+    if (!h || !perm) throw std::runtime_error("null argument");
+    std::memcpy(perm, h->op.cg.cell_perm.data(), h->op.cg.cell_perm.size() * sizeof(int64_t));
+  });
+}
+
+int mf_get_cell_dofs(mf_op_t h, int64_t* dofs) {
+  MF_TRY({
+    if (!h || !dofs) throw std::runtime_error("null argument");
+    std::memcpy(dofs, h->op.cg.cell_dofs.data(), h->op.cg.cell_dofs.size() * sizeof(int64_t));
+  });
+}
+
+int mf_get_dof_points(mf_op_t h, double* xyz) {
+  MF_TRY({
+    if (!h || !xyz) throw std::runtime_error("null argument");
+    std::memcpy(xyz, h->op.cg.dof_points.data(), h->op.cg.dof_points.size() * sizeof(double));
+  });
+}
+
+int mf_get_constrained(mf_op_t h, uint8_t* mask) {
+  MF_TRY({
+    if (!h || !mask) throw std::runtime_error("null argument");
+    const CGPlan& pl = h->op.cg;
+    for (int64_t i = 0; i < pl.n_dofs; ++i) mask[i] = i >= pl.n_free ? 1 : 0;
+  });
+}
+
+}  // extern "C"

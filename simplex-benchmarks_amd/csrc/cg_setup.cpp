@@ -1,0 +1,276 @@
+// Continuous-Galerkin setup: DoF numbering, Dirichlet constraints, patch and
+// cell colourings, patch-local index tables and per-cell geometry.
+//
+// Counterpart of Operator::reinit (CG/cells_batched/application.cc:1056-1128),
+// which builds manual_dof_indices (global index per lane, invalid for
+// constrained DoFs) on top of deal.II's DoFHandler + MatrixFree.  Here the
+// table is patch-local: each patch of cells (from the hierarchical reordering)
+// stages its DoFs in LDS, so a cell's index is a 16-bit offset into its
+// patch's DoF list and constrained DoFs carry the invalid marker 0xFFFF, as
+// in the reference's masked gather (application.cc:1287-1292).
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+#include "op.h"
+
+namespace smf {
+
+namespace {
+
+// local edge index of the pair of local vertices (a,b)
+int local_edge(int a, int b) {
+  if (a > b) std::swap(a, b);
+  for (int e = 0; e < 6; ++e)
+    if (kEdgeVerts[e][0] == a && kEdgeVerts[e][1] == b) return e;
+  return -1;
+}
+
+}  // namespace
+
+void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan) {
+  const Lagrange fe(p);
+  const int nl = fe.n_loc;
+  plan.p = p;
+  plan.n_loc = nl;
+  plan.n_cells = mesh.nc;
+  const int64_t nc = mesh.nc, nv = mesh.nv;
+
+  int patch = opt.patch_cells > 0 ? opt.patch_cells : (p == 1 ? 2048 : (p == 2 ? 1024 : 512));
+  const int patch_cap = p == 1 ? 2048 : (p == 2 ? 1024 : 512);
+  if (patch > patch_cap) throw std::runtime_error("patch_cells exceeds the kernel capacity for this degree");
+  plan.max_patch_cells = 0;
+
+  // ---- entities and raw DoFs (entity-based numbering) ----
+  FaceInfo fi = build_faces(mesh);
+  EdgeInfo ei;
+  if (p >= 2) ei = build_edges(mesh);
+  const int64_t n_raw = nv + ei.n_edges * (p - 1) + (p == 3 ? fi.n_faces : 0);
+  std::vector<uint8_t> raw_con(n_raw, 0);
+  for (int64_t f = 0; f < fi.n_faces; ++f) {
+    if (fi.face_cell[f][1] >= 0) continue;
+    const int64_t c = fi.face_cell[f][0];
+    const int lf = fi.face_lf[f][0];
+    for (int k = 0; k < 3; ++k) raw_con[mesh.cells[4 * c + kFaceVerts[lf][k]]] = 1;
+    if (p >= 2)
+      for (int k = 0; k < 3; ++k) {
+        const int e = local_edge(kFaceVerts[lf][k], kFaceVerts[lf][(k + 1) % 3]);
+        const int64_t ge = ei.cell_edge[6 * c + e];
+        for (int s = 0; s < p - 1; ++s) raw_con[nv + ge * (p - 1) + s] = 1;
+      }
+    if (p == 3) raw_con[nv + ei.n_edges * 2 + f] = 1;
+  }
+  // raw DoF of every (input cell, local node)
+  auto raw_of = [&](int64_t c, int i) -> int64_t {
+    const LocalNode& n = fe.nodes[i];
+    if (n.kind == 0) return mesh.cells[4 * c + n.entity];
+    if (n.kind == 1) {
+      const int32_t ga = mesh.cells[4 * c + kEdgeVerts[n.entity][0]];
+      const int32_t gb = mesh.cells[4 * c + kEdgeVerts[n.entity][1]];
+      const int s = ga < gb ? n.sub : p - 2 - n.sub;
+      return nv + ei.cell_edge[6 * c + n.entity] * (p - 1) + s;
+    }
+    return nv + ei.n_edges * (p - 1) + fi.cell_face[4 * c + n.entity];
+  };
+
+  // ---- hierarchical patches ----
+  std::vector<int64_t> perm, patch_ptr;
+  reorder_cells(mesh, opt.reorder != 0, patch, perm, patch_ptr);
+  const int64_t np = (int64_t)patch_ptr.size() - 1;
+  plan.n_patches = np;
+
+  // ---- patch colouring: patches sharing a vertex conflict ----
+  std::vector<int32_t> pcolor(np, -1);
+  {
+    std::vector<std::pair<int32_t, int32_t>> vp;  // (vertex, patch)
+    vp.reserve(4 * nc);
+    for (int64_t q = 0; q < np; ++q)
+      for (int64_t k = patch_ptr[q]; k < patch_ptr[q + 1]; ++k)
+        for (int t = 0; t < 4; ++t) vp.push_back({mesh.cells[4 * perm[k] + t], (int32_t)q});
+    std::sort(vp.begin(), vp.end());
+    vp.erase(std::unique(vp.begin(), vp.end()), vp.end());
+    std::vector<int64_t> vptr(nv + 1, 0);
+    for (auto& e : vp) vptr[e.first + 1]++;
+    for (int64_t v = 0; v < nv; ++v) vptr[v + 1] += vptr[v];
+    // per patch, list of its vertices
+    std::vector<std::vector<int32_t>> pverts(np);
+    for (auto& e : vp) pverts[e.second].push_back(e.first);
+    std::vector<int64_t> seen(256, -1);
+    for (int64_t q = 0; q < np; ++q) {
+      std::vector<char> used(256, 0);
+      for (int32_t v : pverts[q])
+        for (int64_t k = vptr[v]; k < vptr[v + 1]; ++k) {
+          const int32_t o = vp[k].second;
+          if (pcolor[o] >= 0) used[pcolor[o]] = 1;
+        }
+      int col = 0;
+      while (col < 256 && used[col]) ++col;
+      if (col == 256) throw std::runtime_error("patch colouring needs more than 256 colours");
+      pcolor[q] = col;
+      plan.n_colors = std::max(plan.n_colors, col + 1);
+    }
+  }
+
+  // ---- intra-patch cell colouring (cells sharing a vertex conflict), cells
+  //      of a patch sorted by colour ----
+  plan.icol_ptr.assign(1, 0);
+  {
+    std::vector<uint64_t> vmask(2 * nv, 0);
+    std::vector<int64_t> newperm(nc);
+    for (int64_t q = 0; q < np; ++q) {
+      const int64_t b = patch_ptr[q], e = patch_ptr[q + 1];
+      std::vector<int> col(e - b);
+      int ncol = 0;
+      for (int64_t k = b; k < e; ++k) {
+        const int32_t* v = &mesh.cells[4 * perm[k]];
+        uint64_t u0 = 0, u1 = 0;
+        for (int t = 0; t < 4; ++t) {
+          u0 |= vmask[2 * v[t]];
+          u1 |= vmask[2 * v[t] + 1];
+        }
+        int c;
+        if (~u0) c = __builtin_ctzll(~u0);
+        else if (~u1) c = 64 + __builtin_ctzll(~u1);
+        else throw std::runtime_error("cell colouring needs more than 128 colours");
+        col[k - b] = c;
+        ncol = std::max(ncol, c + 1);
+        for (int t = 0; t < 4; ++t) vmask[2 * v[t] + (c >> 6)] |= 1ull << (c & 63);
+      }
+      for (int64_t k = b; k < e; ++k)
+        for (int t = 0; t < 4; ++t) {
+          const int32_t v = mesh.cells[4 * perm[k] + t];
+          vmask[2 * v] = vmask[2 * v + 1] = 0;
+        }
+      // stable counting sort by colour
+      std::vector<int> cnt(ncol + 1, 0);
+      for (int c : col) cnt[c + 1]++;
+      for (int c = 0; c < ncol; ++c) cnt[c + 1] += cnt[c];
+      for (int c = 0; c <= ncol; ++c) plan.icol_off.push_back((uint16_t)cnt[c]);
+      plan.icol_ptr.push_back((int32_t)plan.icol_off.size());
+      std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+      for (int64_t k = b; k < e; ++k) newperm[b + pos[col[k - b]]++] = perm[k];
+      plan.max_icolors = std::max(plan.max_icolors, ncol);
+      plan.max_patch_cells = std::max<int>(plan.max_patch_cells, (int)(e - b));
+    }
+    perm.swap(newperm);
+  }
+  plan.cell_perm = perm;
+
+  // ---- owners: the first toucher in processing order (colour, patch) ----
+  std::vector<int64_t> owner_key(n_raw, INT64_MAX);
+  std::vector<int32_t> patch_of(nc);
+  for (int64_t q = 0; q < np; ++q)
+    for (int64_t k = patch_ptr[q]; k < patch_ptr[q + 1]; ++k) patch_of[k] = (int32_t)q;
+  for (int64_t k = 0; k < nc; ++k) {
+    const int64_t key = (int64_t)pcolor[patch_of[k]] * np + patch_of[k];
+    for (int i = 0; i < nl; ++i) {
+      const int64_t r = raw_of(perm[k], i);
+      if (!raw_con[r]) owner_key[r] = std::min(owner_key[r], key);
+    }
+  }
+
+  // ---- numbering: free DoFs patch by patch (owned ranges contiguous), then
+  //      constrained DoFs ----
+  std::vector<int64_t> number(n_raw, -1);
+  int64_t next = 0;
+  plan.own_start.assign(1, 0);
+  for (int64_t q = 0; q < np; ++q) {
+    const int64_t key = (int64_t)pcolor[q] * np + q;
+    for (int64_t k = patch_ptr[q]; k < patch_ptr[q + 1]; ++k)
+      for (int i = 0; i < nl; ++i) {
+        const int64_t r = raw_of(perm[k], i);
+        if (!raw_con[r] && owner_key[r] == key && number[r] < 0) number[r] = next++;
+      }
+    plan.own_start.push_back((int32_t)next);
+  }
+  plan.n_free = next;
+  for (int64_t k = 0; k < nc; ++k)
+    for (int i = 0; i < nl; ++i) {
+      const int64_t r = raw_of(perm[k], i);
+      if (number[r] < 0) number[r] = next++;
+    }
+  plan.n_dofs = next;
+  if (next != n_raw) throw std::runtime_error("DoF numbering left unreached entities");
+  if (plan.n_dofs > INT32_MAX) throw std::runtime_error("too many DoFs for 32-bit indices");
+
+  // ---- global cell table, support points ----
+  plan.cell_dofs.resize(nc * nl);
+  plan.dof_points.assign(3 * plan.n_dofs, 0.0);
+  for (int64_t k = 0; k < nc; ++k) {
+    const int64_t c = perm[k];
+    const CellGeom g = cell_geometry(mesh, c);
+    const double* x0 = &mesh.x[3 * mesh.cells[4 * c]];
+    for (int i = 0; i < nl; ++i) {
+      const int64_t d = number[raw_of(c, i)];
+      plan.cell_dofs[k * nl + i] = d;
+      const auto xr = fe.node_point(i);
+      for (int a = 0; a < 3; ++a) {
+        double s = x0[a];
+        for (int b = 0; b < 3; ++b) s += g.J[a][b] * xr[b];
+        plan.dof_points[3 * d + a] = s;
+      }
+    }
+  }
+
+  // ---- patch-local tables ----
+  plan.cell_start.resize(np + 1);
+  for (int64_t q = 0; q <= np; ++q) plan.cell_start[q] = (int32_t)patch_ptr[q];
+  plan.ghost_start.assign(1, 0);
+  plan.lidx.assign((size_t)nl * nc, 0xFFFF);
+  std::vector<int32_t> stamp(plan.n_dofs, -1), loc(plan.n_dofs, 0);
+  for (int64_t q = 0; q < np; ++q) {
+    const int32_t o0 = plan.own_start[q], o1 = plan.own_start[q + 1];
+    int32_t nloc_p = o1 - o0;
+    for (int64_t k = patch_ptr[q]; k < patch_ptr[q + 1]; ++k)
+      for (int i = 0; i < nl; ++i) {
+        const int64_t d = plan.cell_dofs[k * nl + i];
+        if (d >= plan.n_free) continue;  // constrained: 0xFFFF
+        int32_t li;
+        if (d >= o0 && d < o1) {
+          li = (int32_t)(d - o0);
+        } else if (stamp[d] == q) {
+          li = loc[d];
+        } else {
+          stamp[d] = (int32_t)q;
+          li = loc[d] = nloc_p++;
+          plan.ghost_idx.push_back((int32_t)d);
+        }
+        if (li >= 0xFFFF) throw std::runtime_error("patch has too many local DoFs");
+        plan.lidx[(size_t)i * nc + k] = (uint16_t)li;
+      }
+    plan.ghost_start.push_back((int32_t)plan.ghost_idx.size());
+    plan.max_local = std::max(plan.max_local, nloc_p);
+  }
+  plan.n_ghost_total = (int64_t)plan.ghost_idx.size();
+
+  // ---- patches grouped by colour ----
+  plan.color_ptr.assign(plan.n_colors + 1, 0);
+  for (int64_t q = 0; q < np; ++q) plan.color_ptr[pcolor[q] + 1]++;
+  for (int c = 0; c < plan.n_colors; ++c) plan.color_ptr[c + 1] += plan.color_ptr[c];
+  plan.color_patches.resize(np);
+  {
+    std::vector<int32_t> pos(plan.color_ptr.begin(), plan.color_ptr.end() - 1);
+    for (int64_t q = 0; q < np; ++q) plan.color_patches[pos[pcolor[q]]++] = (int32_t)q;
+  }
+
+  // ---- geometry ----
+  plan.metric.resize(6 * nc);
+  plan.jit.resize(9 * nc);
+  plan.detj.resize(nc);
+  for (int64_t k = 0; k < nc; ++k) {
+    const CellGeom g = cell_geometry(mesh, perm[k]);
+    if (!(g.det > 0)) throw std::runtime_error("cell with non-positive Jacobian determinant");
+    // M = |det J| J^-1 J^-T ; (J^-1 J^-T)_{ab} = sum_e Jinv[a][e] Jinv[b][e]
+    const int ab[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int t = 0; t < 6; ++t) {
+      double s = 0;
+      for (int e = 0; e < 3; ++e) s += g.Jinv[ab[t][0]][e] * g.Jinv[ab[t][1]][e];
+      plan.metric[(size_t)t * nc + k] = g.det * s;
+    }
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) plan.jit[(size_t)(3 * a + b) * nc + k] = g.Jinv[b][a];
+    plan.detj[k] = g.det;
+  }
+}
+
+}  // namespace smf

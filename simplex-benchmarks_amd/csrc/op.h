@@ -1,0 +1,101 @@
+// Internal operator state shared by the setup code (C++) and the HIP launchers.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/simplexmf.h"
+#include "fe.h"
+#include "mesh.h"
+
+namespace smf {
+
+// Thread-local error message returned by mf_last_error().
+void set_error(const std::string& msg);
+
+#define SMF_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      throw std::runtime_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Device allocation owned by an operator (freed in its destructor).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+  template <typename T>
+  void upload(const std::vector<T>& v) {
+    release();
+    bytes = v.size() * sizeof(T);
+    if (bytes == 0) return;
+    SMF_HIP(hipMalloc(&p, bytes));
+    SMF_HIP(hipMemcpy(p, v.data(), bytes, hipMemcpyHostToDevice));
+  }
+  void alloc(size_t b) {
+    release();
+    bytes = b;
+    if (b) SMF_HIP(hipMalloc(&p, b));
+  }
+};
+
+// Host-side plan of a continuous-Galerkin operator: DoF numbering, patches,
+// colourings, local index tables and geometry (all in internal cell order).
+struct CGPlan {
+  int p = 0, n_loc = 0, n_q = 0, ncomp = 1;
+  int64_t n_cells = 0, n_dofs = 0, n_free = 0;  // scalar DoFs (per component)
+  std::vector<int64_t> cell_perm;   // internal cell -> input cell
+  std::vector<int64_t> cell_dofs;   // n_cells*n_loc global DoF per local node
+  std::vector<double> dof_points;   // 3*n_dofs support points
+  // patches: cells [cell_start[p], cell_start[p+1]); owned DoFs
+  // [own_start[p], own_start[p+1]); ghost DoFs ghost_idx[ghost_start[p]..)
+  int64_t n_patches = 0;
+  int n_colors = 0, max_icolors = 0, max_local = 0, max_patch_cells = 0;
+  std::vector<int32_t> cell_start, own_start, ghost_start, ghost_idx;
+  std::vector<int32_t> icol_ptr;     // per patch offsets into icol_off
+  std::vector<uint16_t> icol_off;    // per patch: colour boundaries (cells sorted by colour)
+  std::vector<int32_t> color_patches, color_ptr;  // patches grouped by patch colour
+  std::vector<uint16_t> lidx;        // n_loc x n_cells patch-local index (0xFFFF = constrained)
+  std::vector<double> metric;        // 6 x n_cells: |det J| J^-1 J^-T (xx xy xz yy yz zz)
+  std::vector<double> jit;           // 9 x n_cells: J^-T row-major
+  std::vector<double> detj;          // n_cells
+  int64_t n_ghost_total = 0;
+  std::vector<double> shape;         // G (n_loc x n_q x 3) then w_q G
+};
+
+struct Operator {
+  int kind = MF_CG_POISSON, precision = MF_F64, degree = 1, ncomp = 1;
+  mf_options opt{};
+  Mesh mesh;            // kept for transfers and RHS assembly
+  CGPlan cg;
+  // device data
+  DevBuf d_cell_start, d_own_start, d_ghost_start, d_ghost_idx, d_icol_ptr, d_icol_off,
+      d_color_patches, d_lidx, d_metric, d_shape;
+  bool on_device = false;
+  int64_t m() const { return cg.n_dofs * ncomp; }
+  size_t wbytes() const { return precision == MF_F64 ? 8 : 4; }
+};
+
+void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan);
+void make_cg_shape(CGPlan& plan);
+// host execution of the kernels' schedule (self-check of the tables)
+void host_apply_cg(const Operator& op, int mode, void* dst, const void* src);
+void upload_cg(Operator& op);
+// mode 0: dst = A src; mode 1: dst = diag(A)
+void launch_cg(const Operator& op, int mode, void* dst, const void* src, hipStream_t s);
+
+}  // namespace smf

@@ -1,0 +1,358 @@
+"""Python host mirror of the simplexmf C ABI (ctypes).
+
+Mirrors the reference operator's interface -- ``reinit`` (construction),
+``vmult``, ``Tvmult``, ``m``, ``initialize_dof_vector``,
+``compute_inverse_diagonal`` (CG/cells_batched/application.cc:1038-1196,
+application_hp.cc:1386-1409) -- over the HIP library
+``lib/libsimplexmf.so``.  Device vectors are :class:`DeviceVector` objects
+allocated through the library's own HIP runtime.
+
+The library is required: if it is missing every entry point raises; there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsimplexmf.so")
+
+MF_CG_POISSON, MF_DG_SIPG, MF_CG_VECTOR, MF_DG_VECTOR, MF_DG_HELMHOLTZ = range(5)
+MF_F64, MF_F32 = 0, 1
+KINDS = {"cg": MF_CG_POISSON, "dg": MF_DG_SIPG, "cg_vector": MF_CG_VECTOR,
+         "dg_vector": MF_DG_VECTOR, "dg_helmholtz": MF_DG_HELMHOLTZ}
+PRECISIONS = {"f64": MF_F64, "f32": MF_F32}
+NP_DTYPES = {MF_F64: np.float64, MF_F32: np.float32}
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("n_vertices", C.c_int64), ("vertices", C.POINTER(C.c_double)),
+                ("n_cells", C.c_int64), ("cells", C.POINTER(C.c_int32)),
+                ("kuhn_n", C.c_int32)]
+
+
+class Options(C.Structure):
+    _fields_ = [("reorder", C.c_int32), ("patch_cells", C.c_int32),
+                ("mass_coefficient", C.c_double), ("laplace_coefficient", C.c_double),
+                ("penalty_kind", C.c_int32), ("host_only", C.c_int32)]
+
+
+class Info(C.Structure):
+    _fields_ = [("n_dofs", C.c_int64), ("n_cells", C.c_int64), ("n_constrained", C.c_int64),
+                ("degree", C.c_int32), ("dofs_per_cell", C.c_int32), ("n_q_points", C.c_int32),
+                ("n_components", C.c_int32), ("n_patches", C.c_int64),
+                ("n_patch_colors", C.c_int32), ("max_cell_colors", C.c_int32),
+                ("max_local_dofs", C.c_int32), ("precision", C.c_int32),
+                ("n_interior_faces", C.c_int64), ("n_boundary_faces", C.c_int64),
+                ("algorithmic_bytes", C.c_double), ("flops", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"simplexmf HIP library not built: {LIB_PATH} missing "
+                           "(run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64
+    sig = {
+        "mf_mesh_kuhn_cube": (C.c_int, [i32, C.c_double, u64, vp, vp]),
+        "mf_default_options": (None, [C.POINTER(Options)]),
+        "mf_create": (C.c_int, [C.POINTER(MeshDesc), i32, i32, i32, C.POINTER(Options), C.POINTER(vp)]),
+        "mf_destroy": (C.c_int, [vp]),
+        "mf_vmult": (C.c_int, [vp, vp, vp, vp]),
+        "mf_tvmult": (C.c_int, [vp, vp, vp, vp]),
+        "mf_compute_diagonal": (C.c_int, [vp, vp, vp]),
+        "mf_compute_inverse_diagonal": (C.c_int, [vp, vp, vp]),
+        "mf_m": (i64, [vp]),
+        "mf_get_info": (C.c_int, [vp, C.POINTER(Info)]),
+        "mf_get_cell_order": (C.c_int, [vp, vp]),
+        "mf_get_cell_dofs": (C.c_int, [vp, vp]),
+        "mf_get_dof_points": (C.c_int, [vp, vp]),
+        "mf_get_constrained": (C.c_int, [vp, vp]),
+        "mf_debug_host_apply": (C.c_int, [vp, i32, vp, vp]),
+        "mf_last_error": (C.c_char_p, []),
+        "mf_device_count": (C.c_int, [C.POINTER(i32)]),
+        "mf_device_name": (C.c_int, [C.c_char_p, i32]),
+        "mf_malloc": (C.c_int, [C.POINTER(vp), u64]),
+        "mf_free": (C.c_int, [vp]),
+        "mf_memcpy_htod": (C.c_int, [vp, vp, u64]),
+        "mf_memcpy_dtoh": (C.c_int, [vp, vp, u64]),
+        "mf_memcpy_dtod": (C.c_int, [vp, vp, u64, vp]),
+        "mf_memset": (C.c_int, [vp, i32, u64, vp]),
+        "mf_synchronize": (C.c_int, []),
+        "mf_stream_create": (C.c_int, [C.POINTER(vp)]),
+        "mf_stream_destroy": (C.c_int, [vp]),
+        "mf_stream_synchronize": (C.c_int, [vp]),
+        "mf_event_create": (C.c_int, [C.POINTER(vp)]),
+        "mf_event_destroy": (C.c_int, [vp]),
+        "mf_event_record": (C.c_int, [vp, vp]),
+        "mf_event_elapsed_ms": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+class MFError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise MFError(lib().mf_last_error().decode())
+
+
+def exported_symbols():
+    """Names every caller may bind (for the ABI completeness test)."""
+    lib()
+    return sorted(_SIGNATURE_NAMES)
+
+
+_SIGNATURE_NAMES = [
+    "mf_mesh_kuhn_cube", "mf_default_options", "mf_create", "mf_destroy", "mf_vmult",
+    "mf_tvmult", "mf_compute_diagonal", "mf_compute_inverse_diagonal", "mf_m", "mf_get_info",
+    "mf_get_cell_order", "mf_get_cell_dofs", "mf_get_dof_points", "mf_get_constrained",
+    "mf_debug_host_apply", "mf_last_error", "mf_device_count", "mf_device_name", "mf_malloc",
+    "mf_free", "mf_memcpy_htod", "mf_memcpy_dtoh", "mf_memcpy_dtod", "mf_memset",
+    "mf_synchronize", "mf_stream_create", "mf_stream_destroy", "mf_stream_synchronize",
+    "mf_event_create", "mf_event_destroy", "mf_event_record", "mf_event_elapsed_ms",
+]
+
+
+# --------------------------------------------------------------------------
+# meshes
+
+class Mesh:
+    """Tetrahedral mesh on the host (vertices (nv,3) float64, cells (nc,4) int32)."""
+
+    def __init__(self, vertices, cells, kuhn_n=0):
+        self.vertices = np.ascontiguousarray(vertices, dtype=np.float64)
+        self.cells = np.ascontiguousarray(cells, dtype=np.int32)
+        self.kuhn_n = int(kuhn_n)
+
+    @staticmethod
+    def kuhn_cube(n, perturb=0.2, seed=0):
+        """Unit cube, n^3 hexes x 6 Kuhn tets, interior vertices perturbed by
+        perturb*h*U[-1,1) per coordinate (splitmix64 seeded)."""
+        v = np.empty(((n + 1) ** 3, 3), np.float64)
+        c = np.empty((6 * n ** 3, 4), np.int32)
+        check(lib().mf_mesh_kuhn_cube(n, perturb, seed, v.ctypes.data, c.ctypes.data))
+        return Mesh(v, c, kuhn_n=n)
+
+    def desc(self):
+        d = MeshDesc()
+        d.n_vertices = self.vertices.shape[0]
+        d.vertices = self.vertices.ctypes.data_as(C.POINTER(C.c_double))
+        d.n_cells = self.cells.shape[0]
+        d.cells = self.cells.ctypes.data_as(C.POINTER(C.c_int32))
+        d.kuhn_n = self.kuhn_n
+        return d
+
+
+# --------------------------------------------------------------------------
+# device memory
+
+class Stream:
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().mf_stream_create(C.byref(h)))
+        self.handle = h.value
+
+    def synchronize(self):
+        check(lib().mf_stream_synchronize(self.handle))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.mf_stream_destroy(self.handle)
+            self.handle = None
+
+
+class Event:
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().mf_event_create(C.byref(h)))
+        self.handle = h.value
+
+    def record(self, stream=None):
+        check(lib().mf_event_record(self.handle, stream.handle if stream else None))
+
+    def elapsed_ms(self, stop):
+        ms = C.c_float()
+        check(lib().mf_event_elapsed_ms(self.handle, stop.handle, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.mf_event_destroy(self.handle)
+            self.handle = None
+
+
+class DeviceVector:
+    """A device array of n elements of dtype (float64/float32/...)."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().mf_malloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @staticmethod
+    def from_numpy(a):
+        a = np.ascontiguousarray(a)
+        v = DeviceVector(a.size, a.dtype)
+        check(lib().mf_memcpy_htod(v.ptr, a.ctypes.data, v.nbytes))
+        return v
+
+    def copy_from(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(lib().mf_memcpy_htod(self.ptr, a.ctypes.data, self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.n, self.dtype)
+        check(lib().mf_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def zero(self, stream=None):
+        check(lib().mf_memset(self.ptr, 0, self.nbytes, stream.handle if stream else None))
+
+    def free(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.mf_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def synchronize():
+    check(lib().mf_synchronize())
+
+
+def device_count():
+    n = C.c_int32()
+    if lib().mf_device_count(C.byref(n)) != 0:
+        return 0
+    return n.value
+
+
+def device_name():
+    buf = C.create_string_buffer(256)
+    check(lib().mf_device_name(buf, 256))
+    return buf.value.decode()
+
+
+# --------------------------------------------------------------------------
+# operator
+
+class Operator:
+    """Matrix-free operator; construction plays the role of ``reinit``."""
+
+    def __init__(self, mesh: Mesh, degree: int, kind="cg", precision="f64", reorder=True,
+                 patch_cells=0, mass_coefficient=1.0, laplace_coefficient=1.0,
+                 penalty_kind=0, host_only=False):
+        L = lib()
+        opt = Options()
+        L.mf_default_options(C.byref(opt))
+        opt.reorder = 1 if reorder else 0
+        opt.patch_cells = patch_cells
+        opt.mass_coefficient = mass_coefficient
+        opt.laplace_coefficient = laplace_coefficient
+        opt.penalty_kind = penalty_kind
+        opt.host_only = 1 if host_only else 0
+        self.mesh = mesh
+        self.kind = KINDS[kind] if isinstance(kind, str) else int(kind)
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self.dtype = NP_DTYPES[self.precision]
+        h = C.c_void_p()
+        d = mesh.desc()
+        check(L.mf_create(C.byref(d), degree, self.kind, self.precision, C.byref(opt), C.byref(h)))
+        self.handle = h.value
+        inf = Info()
+        check(L.mf_get_info(self.handle, C.byref(inf)))
+        self.info = inf.as_dict()
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.mf_destroy(self.handle)
+            self.handle = None
+
+    # -- reference surface --
+    def m(self):
+        return lib().mf_m(self.handle)
+
+    def initialize_dof_vector(self):
+        v = DeviceVector(self.m(), self.dtype)
+        v.zero()
+        return v
+
+    def vmult(self, dst: DeviceVector, src: DeviceVector, stream: Stream | None = None):
+        self._check_vec(dst)
+        self._check_vec(src)
+        check(lib().mf_vmult(self.handle, dst.ptr, src.ptr, stream.handle if stream else None))
+
+    def Tvmult(self, dst, src, stream=None):
+        self._check_vec(dst)
+        self._check_vec(src)
+        check(lib().mf_tvmult(self.handle, dst.ptr, src.ptr, stream.handle if stream else None))
+
+    def compute_diagonal(self, diag: DeviceVector, stream=None):
+        self._check_vec(diag)
+        check(lib().mf_compute_diagonal(self.handle, diag.ptr, stream.handle if stream else None))
+
+    def compute_inverse_diagonal(self, inv: DeviceVector, stream=None):
+        self._check_vec(inv)
+        check(lib().mf_compute_inverse_diagonal(self.handle, inv.ptr, stream.handle if stream else None))
+
+    def _check_vec(self, v):
+        if v.n != self.m() or v.dtype != self.dtype:
+            raise ValueError(f"vector of {v.n} {v.dtype} does not match operator ({self.m()} {self.dtype})")
+
+    # -- numbering queries (host) --
+    def cell_order(self):
+        out = np.empty(self.info["n_cells"], np.int64)
+        check(lib().mf_get_cell_order(self.handle, out.ctypes.data))
+        return out
+
+    def cell_dofs(self):
+        out = np.empty((self.info["n_cells"], self.info["dofs_per_cell"]), np.int64)
+        check(lib().mf_get_cell_dofs(self.handle, out.ctypes.data))
+        return out
+
+    def n_scalar_dofs(self):
+        return self.info["n_dofs"] // max(1, self.info["n_components"])
+
+    def dof_points(self):
+        out = np.empty((self.n_scalar_dofs(), 3), np.float64)
+        check(lib().mf_get_dof_points(self.handle, out.ctypes.data))
+        return out
+
+    def constrained(self):
+        out = np.empty(self.n_scalar_dofs(), np.uint8)
+        check(lib().mf_get_constrained(self.handle, out.ctypes.data))
+        return out
+
+    def host_apply(self, src=None, mode=0):
+        """Host execution of the kernels' schedule (table self-check, no GPU)."""
+        dst = np.zeros(self.m(), self.dtype)
+        s = None if src is None else np.ascontiguousarray(src, dtype=self.dtype)
+        check(lib().mf_debug_host_apply(self.handle, mode, dst.ctypes.data,
+                                         None if s is None else s.ctypes.data))
+        return dst
