@@ -22,19 +22,17 @@ static std::vector<T> convert(const std::vector<double>& v) {
 }
 
 void make_cg_shape(CGPlan& pl) {
-  // reference shape gradients at the quadrature points, then the weighted copy
+  // reference shape gradients at the quadrature points [q][i][d], then w[q]
   const Lagrange fe(pl.p);
   const Quadrature q = tet_rule(cg_quadrature_degree(pl.p));
   pl.n_q = q.size();
-  pl.shape.assign(2 * pl.n_loc * pl.n_q * 3, 0.0);
+  pl.shape.assign(pl.n_loc * pl.n_q * 3 + pl.n_q, 0.0);
   std::vector<double> grads(pl.n_loc * 3);
   for (int k = 0; k < pl.n_q; ++k) {
     fe.eval(q.pts[k].data(), nullptr, grads.data());
     for (int i = 0; i < pl.n_loc; ++i)
-      for (int d = 0; d < 3; ++d) {
-        pl.shape[(i * pl.n_q + k) * 3 + d] = grads[i * 3 + d];
-        pl.shape[pl.n_loc * pl.n_q * 3 + (i * pl.n_q + k) * 3 + d] = q.w[k] * grads[i * 3 + d];
-      }
+      for (int d = 0; d < 3; ++d) pl.shape[(k * pl.n_loc + i) * 3 + d] = grads[i * 3 + d];
+    pl.shape[pl.n_loc * pl.n_q * 3 + k] = q.w[k];
   }
 }
 
@@ -44,10 +42,11 @@ void upload_cg(Operator& op) {
   op.d_own_start.upload(pl.own_start);
   op.d_ghost_start.upload(pl.ghost_start);
   op.d_ghost_idx.upload(pl.ghost_idx);
-  op.d_icol_ptr.upload(pl.icol_ptr);
-  op.d_icol_off.upload(pl.icol_off);
-  op.d_color_patches.upload(pl.color_patches);
+  op.d_ccol.upload(pl.ccol);
+  op.d_fix_ptr.upload(pl.fix_ptr);
+  op.d_fix_src.upload(pl.fix_src);
   op.d_lidx.upload(pl.lidx);
+  op.d_partial.alloc(std::max<int64_t>(pl.n_ghost_total, 1) * op.wbytes());
   if (op.precision == MF_F64) {
     op.d_metric.upload(pl.metric);
     op.d_shape.upload(pl.shape);
@@ -59,68 +58,65 @@ void upload_cg(Operator& op) {
 }
 
 namespace {
-// Same schedule as cg_patch_kernel, executed serially on the host.
+// Same schedule as cg_wave_kernel + cg_fixup_kernel, executed serially on the host.
 template <typename T>
 void host_apply_cg_t(const Operator& op, int mode, T* dst, const T* src) {
   const CGPlan& pl = op.cg;
   const int nl = pl.n_loc, nq = pl.n_q;
   std::vector<T> G(pl.shape.begin(), pl.shape.end());
-  const T* GW = G.data() + nl * nq * 3;
-  std::vector<T> su, sy;
-  for (int c = 0; c < pl.n_colors; ++c)
-    for (int32_t k = pl.color_ptr[c]; k < pl.color_ptr[c + 1]; ++k) {
-      const int patch = pl.color_patches[k];
-      const int c0 = pl.cell_start[patch], nc = pl.cell_start[patch + 1] - c0;
-      const int o0 = pl.own_start[patch], no = pl.own_start[patch + 1] - o0;
-      const int g0 = pl.ghost_start[patch], ng = pl.ghost_start[patch + 1] - g0;
-      su.assign(no + ng, T(0));
-      sy.assign(no + ng, T(0));
-      if (mode == 0)
-        for (int j = 0; j < no + ng; ++j) su[j] = j < no ? src[o0 + j] : src[pl.ghost_idx[g0 + j - no]];
-      std::vector<T> y((size_t)nc * nl, T(0));
-      for (int lc = 0; lc < nc; ++lc) {
-        const int64_t cell = c0 + lc;
-        T m[6];
-        for (int t = 0; t < 6; ++t) m[t] = (T)pl.metric[(size_t)t * pl.n_cells + cell];
-        T u[20];
-        for (int i = 0; i < nl; ++i) {
-          const uint16_t li = pl.lidx[(size_t)i * pl.n_cells + cell];
-          u[i] = (mode == 0 && li != 0xFFFF) ? su[li] : T(0);
-        }
-        T* yc = &y[(size_t)lc * nl];
-        for (int q = 0; q < nq; ++q) {
-          if (mode == 0) {
-            T g[3] = {0, 0, 0};
-            for (int i = 0; i < nl; ++i)
-              for (int d = 0; d < 3; ++d) g[d] += G[(i * nq + q) * 3 + d] * u[i];
-            const T tx = m[0] * g[0] + m[1] * g[1] + m[2] * g[2];
-            const T ty = m[1] * g[0] + m[3] * g[1] + m[4] * g[2];
-            const T tz = m[2] * g[0] + m[4] * g[1] + m[5] * g[2];
-            for (int i = 0; i < nl; ++i)
-              yc[i] += GW[(i * nq + q) * 3] * tx + GW[(i * nq + q) * 3 + 1] * ty + GW[(i * nq + q) * 3 + 2] * tz;
-          } else {
-            for (int i = 0; i < nl; ++i) {
-              const T* g = &G[(i * nq + q) * 3];
-              const T tx = m[0] * g[0] + m[1] * g[1] + m[2] * g[2];
-              const T ty = m[1] * g[0] + m[3] * g[1] + m[4] * g[2];
-              const T tz = m[2] * g[0] + m[4] * g[1] + m[5] * g[2];
-              yc[i] += GW[(i * nq + q) * 3] * tx + GW[(i * nq + q) * 3 + 1] * ty + GW[(i * nq + q) * 3 + 2] * tz;
-            }
-          }
-        }
+  const T* W = G.data() + nl * nq * 3;
+  std::vector<T> su, sy, partial(pl.n_ghost_total, T(0));
+  for (int64_t patch = 0; patch < pl.n_patches; ++patch) {
+    const int c0 = pl.cell_start[patch], nc = pl.cell_start[patch + 1] - c0;
+    const int o0 = pl.own_start[patch], no = pl.own_start[patch + 1] - o0;
+    const int g0 = pl.ghost_start[patch], ng = pl.ghost_start[patch + 1] - g0;
+    su.assign(no + ng, T(0));
+    sy.assign(no + ng, T(0));
+    if (mode == 0)
+      for (int j = 0; j < no + ng; ++j) su[j] = j < no ? src[o0 + j] : src[pl.ghost_idx[g0 + j - no]];
+    for (int lc = 0; lc < nc; ++lc) {
+      const int64_t cell = c0 + lc;
+      T m[6];
+      for (int t = 0; t < 6; ++t) m[t] = (T)pl.metric[(size_t)t * pl.n_cells + cell];
+      T u[20], y[20];
+      uint16_t li[20];
+      for (int i = 0; i < nl; ++i) {
+        li[i] = pl.lidx[(size_t)i * pl.n_cells + cell];
+        u[i] = (mode == 0 && li[i] != 0xFFFF) ? su[li[i]] : T(0);
+        y[i] = T(0);
       }
-      const int cb = pl.icol_ptr[patch], ncol = pl.icol_ptr[patch + 1] - cb - 1;
-      for (int col = 0; col < ncol; ++col)
-        for (int lc = pl.icol_off[cb + col]; lc < pl.icol_off[cb + col + 1]; ++lc)
+      for (int q = 0; q < nq; ++q) {
+        const T* Gq = &G[(size_t)q * nl * 3];
+        if (mode == 0) {
+          T g[3] = {0, 0, 0};
+          for (int i = 0; i < nl; ++i)
+            for (int d = 0; d < 3; ++d) g[d] += Gq[i * 3 + d] * u[i];
+          const T tx = W[q] * (m[0] * g[0] + m[1] * g[1] + m[2] * g[2]);
+          const T ty = W[q] * (m[1] * g[0] + m[3] * g[1] + m[4] * g[2]);
+          const T tz = W[q] * (m[2] * g[0] + m[4] * g[1] + m[5] * g[2]);
+          for (int i = 0; i < nl; ++i) y[i] += Gq[i * 3] * tx + Gq[i * 3 + 1] * ty + Gq[i * 3 + 2] * tz;
+        } else {
           for (int i = 0; i < nl; ++i) {
-            const uint16_t li = pl.lidx[(size_t)i * pl.n_cells + c0 + lc];
-            if (li != 0xFFFF) sy[li] += y[(size_t)lc * nl + i];
+            const T* g = &Gq[i * 3];
+            y[i] += W[q] * (g[0] * (m[0] * g[0] + m[1] * g[1] + m[2] * g[2]) +
+                            g[1] * (m[1] * g[0] + m[3] * g[1] + m[4] * g[2]) +
+                            g[2] * (m[2] * g[0] + m[4] * g[1] + m[5] * g[2]));
           }
-      for (int j = 0; j < no + ng; ++j) {
-        if (j < no) dst[o0 + j] = sy[j];
-        else dst[pl.ghost_idx[g0 + j - no]] += sy[j];
+        }
       }
+      for (int i = 0; i < nl; ++i)
+        if (li[i] != 0xFFFF) sy[li[i]] += y[i];
     }
+    for (int j = 0; j < no + ng; ++j) {
+      if (j < no) dst[o0 + j] = sy[j];
+      else partial[g0 + j - no] = sy[j];
+    }
+  }
+  for (int64_t t = 0; t < pl.n_free - pl.n_int; ++t) {
+    T s = partial[pl.fix_src[pl.fix_ptr[t]]];
+    for (int32_t k = pl.fix_ptr[t] + 1; k < pl.fix_ptr[t + 1]; ++k) s += partial[pl.fix_src[k]];
+    dst[pl.n_int + t] = s;
+  }
   for (int64_t i = pl.n_free; i < pl.n_dofs; ++i) dst[i] = mode == 0 ? src[i] : T(1);
 }
 }  // namespace
@@ -271,7 +267,7 @@ int mf_get_info(mf_op_t h, mf_info* info) {
     info->n_q_points = pl.n_q;
     info->n_components = op.ncomp;
     info->n_patches = pl.n_patches;
-    info->n_patch_colors = pl.n_colors;
+    info->n_patch_colors = 0;
     info->max_cell_colors = pl.max_icolors;
     info->max_local_dofs = pl.max_local;
     info->precision = op.precision;

@@ -1,19 +1,21 @@
 // Continuous-Galerkin cell kernels for gfx950.
 //
-// One workgroup (256 threads, 4 wave64) processes one patch of up to
-// 256*CPT cells of the hierarchical reordering:
+// One wavefront (64 lanes) processes one patch of cells of the hierarchical
+// reordering, in rounds of 64 cells (one cell per lane):
 //   1. the patch's DoF values are staged in LDS (owned DoFs are a contiguous
 //      global range -> coalesced loads; ghost DoFs through an index list);
-//   2. every thread evaluates CPT cells: gradients at the quadrature points
+//   2. each lane evaluates its cell: gradients at the quadrature points
 //      (interpolation with the reference shape-gradient matrix), the affine
-//      quadrature-point operator w_q |det J| J^-1 J^-T, and the transposed
-//      interpolation (integration) -- all in registers with FMA;
-//   3. cell results are summed into an LDS accumulator colour by colour (cells
-//      of one colour share no DoF), so the sums are deterministic;
-//   4. owned DoFs are stored, ghost DoFs added to the value written by the
-//      patch colours launched earlier.
-// Patch colours are separate launches in a fixed order, so the whole vmult is
-// bit-reproducible.
+//      quadrature-point operator w_q |det J| J^-1 J^-T and the transposed
+//      interpolation (integration), in registers with FP64/FP32 FMA;
+//   3. the round's cell vectors are added into an LDS accumulator colour by
+//      colour (cells of one colour share no DoF; LDS executes one wave's
+//      instructions in order), so every DoF is summed in cell order;
+//   4. owned DoFs are stored; ghost DoFs (owned by another patch) are written
+//      as partial sums that a fix-up kernel adds to the owner's value in patch
+//      order.  No floating-point atomics reach global memory: the result is
+//      bit-reproducible.
+// All patches run in one launch.
 //
 // Reference: do_cell_integral_masked_gather, CG/cells_batched/application.cc:
 // 1236-1416 (masked gather 1272-1306, interpolation 1314-1319, affine D_e
@@ -21,6 +23,7 @@
 // identity rows 1173-1175).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 
 #include "op.h"
@@ -29,7 +32,7 @@ namespace smf {
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kWave = 64;
 constexpr unsigned short kInvalid = 0xFFFF;
 
 template <int P>
@@ -39,19 +42,19 @@ struct CGShape {
 };
 
 // MODE 0: y = A_K u;  MODE 1: y_i = (A_K)_ii
-template <int P, typename T, int CPT, int MODE>
-__global__ __launch_bounds__(kThreads) void cg_patch_kernel(
-    const int32_t* __restrict__ color_patches, int32_t patch_base,
+// shape: G[(q*NL + i)*3 + d] = d phi_i / d xhat_d at point q, then w[q].
+template <int P, typename T, int MODE>
+__global__ __launch_bounds__(kWave) void cg_wave_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ own_start,
     const int32_t* __restrict__ ghost_start, const int32_t* __restrict__ ghost_idx,
-    const int32_t* __restrict__ icol_ptr, const uint16_t* __restrict__ icol_off,
-    const uint16_t* __restrict__ lidx, int64_t ncells_total, const T* __restrict__ metric,
-    const T* __restrict__ shape, const T* __restrict__ src, T* __restrict__ dst) {
+    const uint8_t* __restrict__ ccol, const uint16_t* __restrict__ lidx, int64_t ncells_total,
+    const T* __restrict__ metric, const T* __restrict__ shape, const T* __restrict__ src,
+    T* __restrict__ dst, T* __restrict__ partial) {
   constexpr int NL = CGShape<P>::NL, NQ = CGShape<P>::NQ;
   extern __shared__ unsigned char smem_raw[];
   T* su = reinterpret_cast<T*>(smem_raw);
-  const int tid = threadIdx.x;
-  const int patch = color_patches[patch_base + blockIdx.x];
+  const int lane = threadIdx.x;
+  const int patch = blockIdx.x;
   const int c0 = cell_start[patch], nc = cell_start[patch + 1] - c0;
   const int o0 = own_start[patch], no = own_start[patch + 1] - o0;
   const int g0 = ghost_start[patch], ng = ghost_start[patch + 1] - g0;
@@ -59,125 +62,135 @@ __global__ __launch_bounds__(kThreads) void cg_patch_kernel(
   T* sy = su + nloc;
 
   // 1. stage the patch DoFs
-  for (int j = tid; j < nloc; j += kThreads) {
+  for (int j = lane; j < nloc; j += kWave) {
     if (MODE == 0) su[j] = j < no ? src[o0 + j] : src[ghost_idx[g0 + j - no]];
     sy[j] = T(0);
   }
   __syncthreads();
 
-  // shape tables: G[(i*NQ + q)*3 + d] then GW = w_q G at offset NL*NQ*3
-  const T* G = shape;
-  const T* GW = shape + NL * NQ * 3;
-
-  // 2. cell evaluation
-  T y[CPT][NL];
-  uint16_t li[CPT][NL];
+  const T* W = shape + NQ * NL * 3;
+  // per-round cell data, loaded one round ahead (software pipelining)
+  uint16_t li_n[NL];
+  T m_n[6];
+  int col_n;
+  auto load_round = [&](int r0) {
+    const int64_t cell = c0 + min(r0 + lane, nc - 1);
 #pragma unroll
-  for (int k = 0; k < CPT; ++k) {
-    const int lc = tid + k * kThreads;
+    for (int i = 0; i < NL; ++i) li_n[i] = lidx[(int64_t)i * ncells_total + cell];
 #pragma unroll
-    for (int i = 0; i < NL; ++i) y[k][i] = T(0);
-    if (lc < nc) {
-      const int64_t cell = c0 + lc;
+    for (int t = 0; t < 6; ++t) m_n[t] = metric[t * ncells_total + cell];
+    col_n = ccol[cell];
+  };
+  if (nc > 0) load_round(0);
+  for (int r0 = 0; r0 < nc; r0 += kWave) {
+    const int lc = r0 + lane;
+    const bool active = lc < nc;
+    uint16_t li[NL];
+    T y[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      li[i] = li_n[i];
+      y[i] = T(0);
+    }
+    const T m0 = m_n[0], m1 = m_n[1], m2 = m_n[2], m3 = m_n[3], m4 = m_n[4], m5 = m_n[5];
+    const int mycol = col_n;
+    if (r0 + kWave < nc) load_round(r0 + kWave);
+    if (MODE == 0) {
       T u[NL];
 #pragma unroll
-      for (int i = 0; i < NL; ++i) {
-        li[k][i] = lidx[(int64_t)i * ncells_total + cell];
-        if (MODE == 0) u[i] = li[k][i] != kInvalid ? su[li[k][i]] : T(0);
-      }
-      const T m0 = metric[0 * ncells_total + cell], m1 = metric[1 * ncells_total + cell],
-              m2 = metric[2 * ncells_total + cell], m3 = metric[3 * ncells_total + cell],
-              m4 = metric[4 * ncells_total + cell], m5 = metric[5 * ncells_total + cell];
-      if (MODE == 0) {
+      for (int i = 0; i < NL; ++i) u[i] = li[i] != kInvalid ? su[li[i]] : T(0);
+#pragma unroll(P == 1 ? NQ : 1)
+      for (int q = 0; q < NQ; ++q) {
+        const T* G = shape + q * NL * 3;
+        T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          T gx = T(0), gy = T(0), gz = T(0);
-#pragma unroll
-          for (int i = 0; i < NL; ++i) {
-            gx += G[(i * NQ + q) * 3 + 0] * u[i];
-            gy += G[(i * NQ + q) * 3 + 1] * u[i];
-            gz += G[(i * NQ + q) * 3 + 2] * u[i];
-          }
-          const T tx = m0 * gx + m1 * gy + m2 * gz;
-          const T ty = m1 * gx + m3 * gy + m4 * gz;
-          const T tz = m2 * gx + m4 * gy + m5 * gz;
-#pragma unroll
-          for (int i = 0; i < NL; ++i)
-            y[k][i] += GW[(i * NQ + q) * 3 + 0] * tx + GW[(i * NQ + q) * 3 + 1] * ty +
-                       GW[(i * NQ + q) * 3 + 2] * tz;
+        for (int i = 0; i < NL; ++i) {
+          gx += G[i * 3 + 0] * u[i];
+          gy += G[i * 3 + 1] * u[i];
+          gz += G[i * 3 + 2] * u[i];
         }
-      } else {
+        const T w = W[q];
+        const T tx = w * (m0 * gx + m1 * gy + m2 * gz);
+        const T ty = w * (m1 * gx + m3 * gy + m4 * gz);
+        const T tz = w * (m2 * gx + m4 * gy + m5 * gz);
+#pragma unroll
+        for (int i = 0; i < NL; ++i) y[i] += G[i * 3 + 0] * tx + G[i * 3 + 1] * ty + G[i * 3 + 2] * tz;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const T* G = shape + q * NL * 3;
+        const T w = W[q];
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          const T gx = G[i * 3 + 0], gy = G[i * 3 + 1], gz = G[i * 3 + 2];
+          y[i] += w * (gx * (m0 * gx + m1 * gy + m2 * gz) + gy * (m1 * gx + m3 * gy + m4 * gz) +
+                       gz * (m2 * gx + m4 * gy + m5 * gz));
+        }
+      }
+    }
+    // 3. colour-ordered accumulation (colours ascend with the cell index)
+    const int clo = __builtin_amdgcn_readfirstlane(mycol);
+    const int last = min(r0 + kWave, nc) - 1;
+    const int chi = ccol[c0 + last];
+    for (int c = clo; c <= chi; ++c) {
+      if (active && mycol == c) {
 #pragma unroll
         for (int i = 0; i < NL; ++i)
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            const T gx = G[(i * NQ + q) * 3 + 0], gy = G[(i * NQ + q) * 3 + 1],
-                    gz = G[(i * NQ + q) * 3 + 2];
-            const T tx = m0 * gx + m1 * gy + m2 * gz;
-            const T ty = m1 * gx + m3 * gy + m4 * gz;
-            const T tz = m2 * gx + m4 * gy + m5 * gz;
-            y[k][i] += GW[(i * NQ + q) * 3 + 0] * tx + GW[(i * NQ + q) * 3 + 1] * ty +
-                       GW[(i * NQ + q) * 3 + 2] * tz;
-          }
+          if (li[i] != kInvalid) atomicAdd(&sy[li[i]], y[i]);
       }
     }
   }
+  __syncthreads();
 
-  // 3. colour-ordered accumulation into LDS
-  const int cb = icol_ptr[patch], ncol = icol_ptr[patch + 1] - cb - 1;
-  for (int col = 0; col < ncol; ++col) {
-    const int lo = icol_off[cb + col], hi = icol_off[cb + col + 1];
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-      const int lc = tid + k * kThreads;
-      if (lc >= lo && lc < hi) {
-#pragma unroll
-        for (int i = 0; i < NL; ++i)
-          if (li[k][i] != kInvalid) sy[li[k][i]] += y[k][i];
-      }
-    }
-    __syncthreads();
-  }
-
-  // 4. owned DoFs: store; ghost DoFs: add to what earlier colours wrote
-  for (int j = tid; j < nloc; j += kThreads) {
+  // 4. owned DoFs: store; ghost DoFs: partial sums for the fix-up
+  for (int j = lane; j < nloc; j += kWave) {
     if (j < no)
       dst[o0 + j] = sy[j];
     else
-      dst[ghost_idx[g0 + j - no]] += sy[j];
+      partial[g0 + j - no] = sy[j];
   }
 }
 
-// Dirichlet rows: dst = src (MODE 0) or 1 (MODE 1) on [n_free, n_dofs)
+// Interface DoF n_int + t = sum of its patch partials partial[fix_src[k]],
+// k in [fix_ptr[t], fix_ptr[t+1]) (patch order); Dirichlet rows dst = src
+// (MODE 0) or 1 (MODE 1).
 template <typename T, int MODE>
-__global__ void cg_constrained_kernel(int64_t n0, int64_t n1, const T* __restrict__ src,
-                                      T* __restrict__ dst) {
-  const int64_t i = n0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n1) dst[i] = MODE == 0 ? src[i] : T(1);
+__global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* __restrict__ fix_ptr,
+                                const int32_t* __restrict__ fix_src, const T* __restrict__ partial,
+                                int64_t n0, int64_t n1, const T* __restrict__ src,
+                                T* __restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n_iface) {
+    const int32_t k0 = fix_ptr[t], k1 = fix_ptr[t + 1];
+    T s = partial[fix_src[k0]];
+    for (int32_t k = k0 + 1; k < k1; ++k) s += partial[fix_src[k]];
+    dst[n_int + t] = s;
+  }
+  const int64_t j = n0 + t;
+  if (j < n1) dst[j] = MODE == 0 ? src[j] : T(1);
 }
 
 template <int P, typename T, int MODE>
 void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
-  constexpr int CPT = P == 1 ? 8 : (P == 2 ? 4 : 2);
   const CGPlan& pl = op.cg;
-  if (pl.max_patch_cells > CPT * kThreads) throw std::runtime_error("patch larger than kernel capacity");
   const size_t shmem = 2 * (size_t)pl.max_local * sizeof(T);
-  auto kern = cg_patch_kernel<P, T, CPT, MODE>;
-  if (shmem > 65536) SMF_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-  for (int c = 0; c < pl.n_colors; ++c) {
-    const int32_t base = pl.color_ptr[c], cnt = pl.color_ptr[c + 1] - base;
-    if (cnt == 0) continue;
-    hipLaunchKernelGGL(kern, dim3(cnt), dim3(kThreads), shmem, s, op.d_color_patches.as<int32_t>(),
-                       base, op.d_cell_start.as<int32_t>(), op.d_own_start.as<int32_t>(),
+  auto kern = cg_wave_kernel<P, T, MODE>;
+  if (shmem > 65536)
+    SMF_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  if (pl.n_patches > 0)
+    hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_patches), dim3(kWave), shmem, s,
+                       op.d_cell_start.as<int32_t>(), op.d_own_start.as<int32_t>(),
                        op.d_ghost_start.as<int32_t>(), op.d_ghost_idx.as<int32_t>(),
-                       op.d_icol_ptr.as<int32_t>(), op.d_icol_off.as<uint16_t>(),
-                       op.d_lidx.as<uint16_t>(), pl.n_cells, op.d_metric.as<T>(),
-                       op.d_shape.as<T>(), src, dst);
-  }
+                       op.d_ccol.as<uint8_t>(), op.d_lidx.as<uint16_t>(), pl.n_cells,
+                       op.d_metric.as<T>(), op.d_shape.as<T>(), src, dst, op.d_partial.as<T>());
+  const int64_t nif = pl.n_free - pl.n_int;
   const int64_t ncon = pl.n_dofs - pl.n_free;
-  if (ncon > 0)
-    hipLaunchKernelGGL((cg_constrained_kernel<T, MODE>), dim3((ncon + 255) / 256), dim3(256), 0, s,
-                       pl.n_free, pl.n_dofs, src, dst);
+  const int64_t nthreads = std::max(nif, ncon);
+  if (nthreads > 0)
+    hipLaunchKernelGGL((cg_fixup_kernel<T, MODE>), dim3((unsigned)((nthreads + 255) / 256)), dim3(256),
+                       0, s, pl.n_int, nif, op.d_fix_ptr.as<int32_t>(), op.d_fix_src.as<int32_t>(),
+                       op.d_partial.as<T>(), pl.n_free, pl.n_dofs, src, dst);
   SMF_HIP(hipGetLastError());
 }
 

@@ -36,9 +36,8 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   plan.n_cells = mesh.nc;
   const int64_t nc = mesh.nc, nv = mesh.nv;
 
-  int patch = opt.patch_cells > 0 ? opt.patch_cells : (p == 1 ? 2048 : (p == 2 ? 1024 : 512));
-  const int patch_cap = p == 1 ? 2048 : (p == 2 ? 1024 : 512);
-  if (patch > patch_cap) throw std::runtime_error("patch_cells exceeds the kernel capacity for this degree");
+  const int patch = opt.patch_cells > 0 ? opt.patch_cells : (p == 1 ? 512 : (p == 2 ? 256 : 128));
+  if (patch > 4096) throw std::runtime_error("patch_cells must be <= 4096");
   plan.max_patch_cells = 0;
 
   // ---- entities and raw DoFs (entity-based numbering) ----
@@ -74,46 +73,15 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   };
 
   // ---- hierarchical patches ----
+  plan.cell_dofs.clear();
   std::vector<int64_t> perm, patch_ptr;
   reorder_cells(mesh, opt.reorder != 0, patch, perm, patch_ptr);
   const int64_t np = (int64_t)patch_ptr.size() - 1;
   plan.n_patches = np;
 
-  // ---- patch colouring: patches sharing a vertex conflict ----
-  std::vector<int32_t> pcolor(np, -1);
-  {
-    std::vector<std::pair<int32_t, int32_t>> vp;  // (vertex, patch)
-    vp.reserve(4 * nc);
-    for (int64_t q = 0; q < np; ++q)
-      for (int64_t k = patch_ptr[q]; k < patch_ptr[q + 1]; ++k)
-        for (int t = 0; t < 4; ++t) vp.push_back({mesh.cells[4 * perm[k] + t], (int32_t)q});
-    std::sort(vp.begin(), vp.end());
-    vp.erase(std::unique(vp.begin(), vp.end()), vp.end());
-    std::vector<int64_t> vptr(nv + 1, 0);
-    for (auto& e : vp) vptr[e.first + 1]++;
-    for (int64_t v = 0; v < nv; ++v) vptr[v + 1] += vptr[v];
-    // per patch, list of its vertices
-    std::vector<std::vector<int32_t>> pverts(np);
-    for (auto& e : vp) pverts[e.second].push_back(e.first);
-    std::vector<int64_t> seen(256, -1);
-    for (int64_t q = 0; q < np; ++q) {
-      std::vector<char> used(256, 0);
-      for (int32_t v : pverts[q])
-        for (int64_t k = vptr[v]; k < vptr[v + 1]; ++k) {
-          const int32_t o = vp[k].second;
-          if (pcolor[o] >= 0) used[pcolor[o]] = 1;
-        }
-      int col = 0;
-      while (col < 256 && used[col]) ++col;
-      if (col == 256) throw std::runtime_error("patch colouring needs more than 256 colours");
-      pcolor[q] = col;
-      plan.n_colors = std::max(plan.n_colors, col + 1);
-    }
-  }
-
   // ---- intra-patch cell colouring (cells sharing a vertex conflict), cells
   //      of a patch sorted by colour ----
-  plan.icol_ptr.assign(1, 0);
+  plan.ccol.assign(nc, 0);
   {
     std::vector<uint64_t> vmask(2 * nv, 0);
     std::vector<int64_t> newperm(nc);
@@ -132,6 +100,7 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
         if (~u0) c = __builtin_ctzll(~u0);
         else if (~u1) c = 64 + __builtin_ctzll(~u1);
         else throw std::runtime_error("cell colouring needs more than 128 colours");
+        if (c > 254) throw std::runtime_error("cell colouring needs more than 255 colours");
         col[k - b] = c;
         ncol = std::max(ncol, c + 1);
         for (int t = 0; t < 4; ++t) vmask[2 * v[t] + (c >> 6)] |= 1ull << (c & 63);
@@ -145,10 +114,12 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
       std::vector<int> cnt(ncol + 1, 0);
       for (int c : col) cnt[c + 1]++;
       for (int c = 0; c < ncol; ++c) cnt[c + 1] += cnt[c];
-      for (int c = 0; c <= ncol; ++c) plan.icol_off.push_back((uint16_t)cnt[c]);
-      plan.icol_ptr.push_back((int32_t)plan.icol_off.size());
       std::vector<int> pos(cnt.begin(), cnt.end() - 1);
-      for (int64_t k = b; k < e; ++k) newperm[b + pos[col[k - b]]++] = perm[k];
+      for (int64_t k = b; k < e; ++k) {
+        const int64_t dstk = b + pos[col[k - b]]++;
+        newperm[dstk] = perm[k];
+        plan.ccol[dstk] = (uint8_t)col[k - b];
+      }
       plan.max_icolors = std::max(plan.max_icolors, ncol);
       plan.max_patch_cells = std::max<int>(plan.max_patch_cells, (int)(e - b));
     }
@@ -156,33 +127,38 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   }
   plan.cell_perm = perm;
 
-  // ---- owners: the first toucher in processing order (colour, patch) ----
-  std::vector<int64_t> owner_key(n_raw, INT64_MAX);
-  std::vector<int32_t> patch_of(nc);
+  // ---- interior vs interface DoFs: a free DoF touched by one patch only is
+  //      interior to it; touched by several it is an interface DoF ----
+  std::vector<int32_t> first_patch(n_raw, -1);
+  std::vector<uint8_t> multi(n_raw, 0);
   for (int64_t q = 0; q < np; ++q)
-    for (int64_t k = patch_ptr[q]; k < patch_ptr[q + 1]; ++k) patch_of[k] = (int32_t)q;
-  for (int64_t k = 0; k < nc; ++k) {
-    const int64_t key = (int64_t)pcolor[patch_of[k]] * np + patch_of[k];
-    for (int i = 0; i < nl; ++i) {
-      const int64_t r = raw_of(perm[k], i);
-      if (!raw_con[r]) owner_key[r] = std::min(owner_key[r], key);
-    }
-  }
+    for (int64_t k = patch_ptr[q]; k < patch_ptr[q + 1]; ++k)
+      for (int i = 0; i < nl; ++i) {
+        const int64_t r = raw_of(perm[k], i);
+        if (raw_con[r]) continue;
+        if (first_patch[r] < 0) first_patch[r] = (int32_t)q;
+        else if (first_patch[r] != q) multi[r] = 1;
+      }
 
-  // ---- numbering: free DoFs patch by patch (owned ranges contiguous), then
-  //      constrained DoFs ----
+  // ---- numbering: interior DoFs patch by patch (contiguous per patch), then
+  //      interface DoFs, then constrained DoFs, each in first-touch order ----
   std::vector<int64_t> number(n_raw, -1);
   int64_t next = 0;
   plan.own_start.assign(1, 0);
   for (int64_t q = 0; q < np; ++q) {
-    const int64_t key = (int64_t)pcolor[q] * np + q;
     for (int64_t k = patch_ptr[q]; k < patch_ptr[q + 1]; ++k)
       for (int i = 0; i < nl; ++i) {
         const int64_t r = raw_of(perm[k], i);
-        if (!raw_con[r] && owner_key[r] == key && number[r] < 0) number[r] = next++;
+        if (!raw_con[r] && !multi[r] && number[r] < 0) number[r] = next++;
       }
     plan.own_start.push_back((int32_t)next);
   }
+  plan.n_int = next;
+  for (int64_t k = 0; k < nc; ++k)
+    for (int i = 0; i < nl; ++i) {
+      const int64_t r = raw_of(perm[k], i);
+      if (!raw_con[r] && number[r] < 0) number[r] = next++;
+    }
   plan.n_free = next;
   for (int64_t k = 0; k < nc; ++k)
     for (int i = 0; i < nl; ++i) {
@@ -212,45 +188,56 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
     }
   }
 
-  // ---- patch-local tables ----
+  // ---- patch-local tables: [interior DoFs of the patch | interface DoFs it
+  //      touches]; cell indices are 16-bit offsets into that list ----
   plan.cell_start.resize(np + 1);
   for (int64_t q = 0; q <= np; ++q) plan.cell_start[q] = (int32_t)patch_ptr[q];
   plan.ghost_start.assign(1, 0);
   plan.lidx.assign((size_t)nl * nc, 0xFFFF);
-  std::vector<int32_t> stamp(plan.n_dofs, -1), loc(plan.n_dofs, 0);
-  for (int64_t q = 0; q < np; ++q) {
-    const int32_t o0 = plan.own_start[q], o1 = plan.own_start[q + 1];
-    int32_t nloc_p = o1 - o0;
-    for (int64_t k = patch_ptr[q]; k < patch_ptr[q + 1]; ++k)
-      for (int i = 0; i < nl; ++i) {
-        const int64_t d = plan.cell_dofs[k * nl + i];
-        if (d >= plan.n_free) continue;  // constrained: 0xFFFF
-        int32_t li;
-        if (d >= o0 && d < o1) {
-          li = (int32_t)(d - o0);
-        } else if (stamp[d] == q) {
-          li = loc[d];
-        } else {
-          stamp[d] = (int32_t)q;
-          li = loc[d] = nloc_p++;
-          plan.ghost_idx.push_back((int32_t)d);
+  {
+    std::vector<int32_t> stamp(plan.n_dofs, -1), loc(plan.n_dofs, 0);
+    for (int64_t q = 0; q < np; ++q) {
+      const int32_t o0 = plan.own_start[q], o1 = plan.own_start[q + 1];
+      int32_t nloc_p = o1 - o0;
+      for (int64_t k = patch_ptr[q]; k < patch_ptr[q + 1]; ++k)
+        for (int i = 0; i < nl; ++i) {
+          const int64_t d = plan.cell_dofs[k * nl + i];
+          if (d >= plan.n_free) continue;  // constrained: 0xFFFF
+          int32_t li;
+          if (d < plan.n_int) {
+            if (d < o0 || d >= o1) throw std::runtime_error("interior DoF outside its patch range");
+            li = (int32_t)(d - o0);
+          } else if (stamp[d] == q) {
+            li = loc[d];
+          } else {
+            stamp[d] = (int32_t)q;
+            li = loc[d] = nloc_p++;
+            plan.ghost_idx.push_back((int32_t)d);
+          }
+          if (li >= 0xFFFF) throw std::runtime_error("patch has too many local DoFs");
+          plan.lidx[(size_t)i * nc + k] = (uint16_t)li;
         }
-        if (li >= 0xFFFF) throw std::runtime_error("patch has too many local DoFs");
-        plan.lidx[(size_t)i * nc + k] = (uint16_t)li;
-      }
-    plan.ghost_start.push_back((int32_t)plan.ghost_idx.size());
-    plan.max_local = std::max(plan.max_local, nloc_p);
+      plan.ghost_start.push_back((int32_t)plan.ghost_idx.size());
+      plan.max_local = std::max(plan.max_local, nloc_p);
+    }
   }
   plan.n_ghost_total = (int64_t)plan.ghost_idx.size();
 
-  // ---- patches grouped by colour ----
-  plan.color_ptr.assign(plan.n_colors + 1, 0);
-  for (int64_t q = 0; q < np; ++q) plan.color_ptr[pcolor[q] + 1]++;
-  for (int c = 0; c < plan.n_colors; ++c) plan.color_ptr[c + 1] += plan.color_ptr[c];
-  plan.color_patches.resize(np);
+  // ---- interface fix-up: interface DoF n_int + t is the sum of its patch
+  //      partials partial[fix_src[k]], k in [fix_ptr[t], fix_ptr[t+1]), in
+  //      patch order ----
   {
-    std::vector<int32_t> pos(plan.color_ptr.begin(), plan.color_ptr.end() - 1);
-    for (int64_t q = 0; q < np; ++q) plan.color_patches[pos[pcolor[q]]++] = (int32_t)q;
+    const int64_t ng = plan.n_ghost_total, nif = plan.n_free - plan.n_int;
+    std::vector<int32_t> cnt(nif + 1, 0);
+    for (int64_t k = 0; k < ng; ++k) cnt[plan.ghost_idx[k] - plan.n_int + 1]++;
+    for (int64_t t = 0; t < nif; ++t) {
+      if (cnt[t + 1] < 2) throw std::runtime_error("interface DoF with fewer than two patches");
+      cnt[t + 1] += cnt[t];
+    }
+    plan.fix_ptr.assign(cnt.begin(), cnt.end());
+    plan.fix_src.resize(ng);
+    std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t k = 0; k < ng; ++k) plan.fix_src[pos[plan.ghost_idx[k] - plan.n_int]++] = (int32_t)k;
   }
 
   // ---- geometry ----

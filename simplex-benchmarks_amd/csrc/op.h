@@ -57,18 +57,20 @@ struct DevBuf {
 // colourings, local index tables and geometry (all in internal cell order).
 struct CGPlan {
   int p = 0, n_loc = 0, n_q = 0, ncomp = 1;
-  int64_t n_cells = 0, n_dofs = 0, n_free = 0;  // scalar DoFs (per component)
+  // scalar DoFs (per component): [0,n_int) patch-interior, [n_int,n_free)
+  // patch-interface, [n_free,n_dofs) Dirichlet-constrained
+  int64_t n_cells = 0, n_dofs = 0, n_free = 0, n_int = 0;
   std::vector<int64_t> cell_perm;   // internal cell -> input cell
   std::vector<int64_t> cell_dofs;   // n_cells*n_loc global DoF per local node
   std::vector<double> dof_points;   // 3*n_dofs support points
   // patches: cells [cell_start[p], cell_start[p+1]); owned DoFs
   // [own_start[p], own_start[p+1]); ghost DoFs ghost_idx[ghost_start[p]..)
   int64_t n_patches = 0;
-  int n_colors = 0, max_icolors = 0, max_local = 0, max_patch_cells = 0;
+  int max_icolors = 0, max_local = 0, max_patch_cells = 0;
   std::vector<int32_t> cell_start, own_start, ghost_start, ghost_idx;
-  std::vector<int32_t> icol_ptr;     // per patch offsets into icol_off
-  std::vector<uint16_t> icol_off;    // per patch: colour boundaries (cells sorted by colour)
-  std::vector<int32_t> color_patches, color_ptr;  // patches grouped by patch colour
+  std::vector<uint8_t> ccol;         // per cell: colour within its patch (cells sorted by colour)
+  // interface fix-up: DoF n_int+t = sum of partial[fix_src[k]], k in [fix_ptr[t], fix_ptr[t+1])
+  std::vector<int32_t> fix_ptr, fix_src;
   std::vector<uint16_t> lidx;        // n_loc x n_cells patch-local index (0xFFFF = constrained)
   std::vector<double> metric;        // 6 x n_cells: |det J| J^-1 J^-T (xx xy xz yy yz zz)
   std::vector<double> jit;           // 9 x n_cells: J^-T row-major
@@ -83,8 +85,8 @@ struct Operator {
   Mesh mesh;            // kept for transfers and RHS assembly
   CGPlan cg;
   // device data
-  DevBuf d_cell_start, d_own_start, d_ghost_start, d_ghost_idx, d_icol_ptr, d_icol_off,
-      d_color_patches, d_lidx, d_metric, d_shape;
+  DevBuf d_cell_start, d_own_start, d_ghost_start, d_ghost_idx, d_ccol, d_fix_ptr,
+      d_fix_src, d_lidx, d_metric, d_shape, d_partial;
   bool on_device = false;
   int64_t m() const { return cg.n_dofs * ncomp; }
   size_t wbytes() const { return precision == MF_F64 ? 8 : 4; }

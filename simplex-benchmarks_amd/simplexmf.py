@@ -280,11 +280,11 @@ class Operator:
         self.mesh = mesh
         self.kind = KINDS[kind] if isinstance(kind, str) else int(kind)
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        self.dtype = NP_DTYPES[self.precision]
         h = C.c_void_p()
         d = mesh.desc()
         check(L.mf_create(C.byref(d), degree, self.kind, self.precision, C.byref(opt), C.byref(h)))
         self.handle = h.value
+        self.dtype = NP_DTYPES[self.precision]
         inf = Info()
         check(L.mf_get_info(self.handle, C.byref(inf)))
         self.info = inf.as_dict()
