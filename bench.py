@@ -26,6 +26,11 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "vmult GDoF/s + %HBM peak; time to 1e-10 solve: learned vs plain hybrid MG"
 HBM_PEAK_GBS = 8000.0
+KERNEL = {"cg": "cg_wave_kernel + cg_fixup_kernel (one launch each per vmult)",
+          "dg": "dg_cell_kernel (one launch per vmult)", "dg_vector": "dg_cell_kernel",
+          "dg_helmholtz": "dg_cell_kernel"}
+WORKLOAD = {"cg": "CG Poisson", "dg": "DG SIPG Poisson", "dg_vector": "DG SIPG vector Laplace",
+            "dg_helmholtz": "DG Helmholtz-like vector (mass 1 + Laplace 1)"}
 
 
 def parse():
@@ -115,7 +120,7 @@ def main():
         with open(prof) as f:
             traffic = json.load(f).get("bytes_per_vmult")
     base = None
-    if args.cpu_baseline:
+    if args.cpu_baseline and args.kind == "cg":
         log("cpu baseline ...")
         base = cpu_baseline(op, x, args.cpu_seconds)
     prec = {"f64": "f64", "f32": "f32"}[args.precision]
@@ -134,7 +139,7 @@ def main():
         "dtype": prec,
         "data": "synthetic: seeded perturbed Kuhn mesh, x ~ U[-1,1]",
         "config": {
-            "workload": f"{args.kind.upper()} Poisson P{args.degree} vmult, {args.n}^3 x 6 perturbed "
+            "workload": f"{WORKLOAD[args.kind]} P{args.degree} vmult, {args.n}^3 x 6 perturbed "
                         f"Kuhn tets ({ncell} cells, {op.m()} DoFs), {prec.upper()}",
             "kind": args.kind, "degree": args.degree, "n": args.n, "cells": ncell,
             "dofs": op.m(), "reorder": not args.no_reorder, "parallelism": "single GPU",
@@ -148,7 +153,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "algorithmic_bytes": bytes_alg,
-            "kernel": "cg_patch_kernel (one launch per patch colour per vmult)",
+            "kernel": KERNEL[args.kind],
         },
         "cpu_baseline": base,
     }

@@ -103,11 +103,15 @@ int64_t mf_m(mf_op_t op);   /* number of rows (all components) */
 int mf_get_info(mf_op_t op, mf_info* info);
 /* internal cell i is input cell perm[i] (n_cells int64) */
 int mf_get_cell_order(mf_op_t op, int64_t* perm);
-/* per internal cell, the global scalar DoF of each local node (n_cells*dofs_per_cell int64);
- * for n_components > 1, component c of scalar DoF d is row c*n_scalar + d (CG)
- * or, for DG, rows are cell-contiguous: see mf_get_info. */
+/* per internal cell, the row of each local node: CG n_cells*dofs_per_cell
+ * scalar DoFs; DG n_cells*n_components*dofs_per_cell rows, component-major
+ * within the cell (row (k*n_components + c)*dofs_per_cell + i). */
 int mf_get_cell_dofs(mf_op_t op, int64_t* dofs);
-/* support point of every scalar DoF (3*n_scalar_dofs doubles) */
+/* vertex ids of every internal cell in the operator's local vertex order
+ * (4*n_cells int32; DG orders them ascending) */
+int mf_get_cell_vertices(mf_op_t op, int32_t* verts);
+/* support point of every scalar DoF (3*n_scalar_dofs doubles; DG: node
+ * (k, i) at index k*dofs_per_cell + i) */
 int mf_get_dof_points(mf_op_t op, double* xyz);
 /* 1 for Dirichlet-constrained scalar DoFs (n_scalar_dofs bytes) */
 int mf_get_constrained(mf_op_t op, uint8_t* mask);

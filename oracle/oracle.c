@@ -64,14 +64,16 @@ static int monomials(int p, int (*e)[3]) {
   return n;
 }
 
-static double ipow(double x, int k) {
-  double r = 1.0;
+static long double ipow(long double x, int k) {
+  long double r = 1.0L;
   for (int i = 0; i < k; ++i) r *= x;
   return r;
 }
 
-/* coefficient matrices C[p][k][i]: phi_i = sum_k C[k][i] m_k */
-static double g_coef[4][MAXL][MAXL];
+/* coefficient matrices C[p][k][i]: phi_i = sum_k C[k][i] m_k, solved and
+ * evaluated in extended precision (the P3 monomial Vandermonde matrix has a
+ * condition number near 1e4) */
+static long double g_coef[4][MAXL][MAXL];
 static int g_coef_ready[4];
 
 static int build_coef(int p) {
@@ -82,7 +84,7 @@ static int build_coef(int p) {
   node_points(p, x);
   if (monomials(p, e) != n) return -1;
   /* Gauss-Jordan on [V | I] with V[j][k] = m_k(x_j) */
-  double a[MAXL][2 * MAXL];
+  long double a[MAXL][2 * MAXL];
   for (int j = 0; j < n; ++j) {
     for (int k = 0; k < n; ++k) a[j][k] = ipow(x[j][0], e[k][0]) * ipow(x[j][1], e[k][1]) * ipow(x[j][2], e[k][2]);
     for (int k = 0; k < n; ++k) a[j][n + k] = (j == k) ? 1.0 : 0.0;
@@ -90,19 +92,19 @@ static int build_coef(int p) {
   for (int col = 0; col < n; ++col) {
     int piv = col;
     for (int r = col + 1; r < n; ++r)
-      if (fabs(a[r][col]) > fabs(a[piv][col])) piv = r;
-    if (fabs(a[piv][col]) < 1e-14) return -1;
+      if (fabsl(a[r][col]) > fabsl(a[piv][col])) piv = r;
+    if (fabsl(a[piv][col]) < 1e-14L) return -1;
     for (int k = 0; k < 2 * n; ++k) {
-      double t = a[col][k];
+      long double t = a[col][k];
       a[col][k] = a[piv][k];
       a[piv][k] = t;
     }
-    const double inv = 1.0 / a[col][col];
+    const long double inv = 1.0L / a[col][col];
     for (int k = 0; k < 2 * n; ++k) a[col][k] *= inv;
     for (int r = 0; r < n; ++r)
       if (r != col) {
-        const double f = a[r][col];
-        if (f != 0.0)
+        const long double f = a[r][col];
+        if (f != 0.0L)
           for (int k = 0; k < 2 * n; ++k) a[r][k] -= f * a[col][k];
       }
   }
@@ -120,28 +122,29 @@ int orc_basis(int p, const double* x, double* values, double* grads) {
   const int n = nloc_of(p);
   int e[MAXL][3];
   monomials(p, e);
-  double mv[MAXL], mg[MAXL][3];
+  long double mv[MAXL], mg[MAXL][3];
+  const long double x0 = x[0], x1 = x[1], x2 = x[2];
   for (int k = 0; k < n; ++k) {
     const int a = e[k][0], b = e[k][1], c = e[k][2];
-    mv[k] = ipow(x[0], a) * ipow(x[1], b) * ipow(x[2], c);
-    mg[k][0] = a ? a * ipow(x[0], a - 1) * ipow(x[1], b) * ipow(x[2], c) : 0.0;
-    mg[k][1] = b ? b * ipow(x[0], a) * ipow(x[1], b - 1) * ipow(x[2], c) : 0.0;
-    mg[k][2] = c ? c * ipow(x[0], a) * ipow(x[1], b) * ipow(x[2], c - 1) : 0.0;
+    mv[k] = ipow(x0, a) * ipow(x1, b) * ipow(x2, c);
+    mg[k][0] = a ? a * ipow(x0, a - 1) * ipow(x1, b) * ipow(x2, c) : 0.0L;
+    mg[k][1] = b ? b * ipow(x0, a) * ipow(x1, b - 1) * ipow(x2, c) : 0.0L;
+    mg[k][2] = c ? c * ipow(x0, a) * ipow(x1, b) * ipow(x2, c - 1) : 0.0L;
   }
   for (int i = 0; i < n; ++i) {
-    double v = 0, g0 = 0, g1 = 0, g2 = 0;
+    long double v = 0, g0 = 0, g1 = 0, g2 = 0;
     for (int k = 0; k < n; ++k) {
-      const double ck = g_coef[p][k][i];
+      const long double ck = g_coef[p][k][i];
       v += ck * mv[k];
       g0 += ck * mg[k][0];
       g1 += ck * mg[k][1];
       g2 += ck * mg[k][2];
     }
-    if (values) values[i] = v;
+    if (values) values[i] = (double)v;
     if (grads) {
-      grads[3 * i] = g0;
-      grads[3 * i + 1] = g1;
-      grads[3 * i + 2] = g2;
+      grads[3 * i] = (double)g0;
+      grads[3 * i + 1] = (double)g1;
+      grads[3 * i + 2] = (double)g2;
     }
   }
   return 0;

@@ -55,6 +55,20 @@ int orc_cg_matrix_vmult(const orc_mesh* m, int p, const int64_t* cell_dofs,
 int orc_cg_dense(const orc_mesh* m, int p, const int64_t* cell_dofs, const uint8_t* constrained,
                  int64_t n_dofs, double* A);
 
+/* ---- DG SIPG (DG/cells_batched/application.cc:1170-1490) ----------------
+ * DoFs: cell_dofs[c*n_loc + i] (any numbering).  Interior faces and boundary
+ * faces are found from the mesh; penalty
+ *   tau_K = (sum_{F bdry} |F| + 1/2 sum_{F int} |F|) / |K|,
+ *   sigma_F = (p+1)(p+3)/3 * max(tau-, tau+),  sigma_B = (p+1)(p+3)/3 * tau_K.
+ * mass != 0 adds mass * int u v (Helmholtz-like operator); ncomp components
+ * share geometry, component c of DoF d is row c*n_dofs + d.                */
+int orc_dg_vmult(const orc_mesh* m, int p, const int64_t* cell_dofs, int64_t n_dofs, int ncomp,
+                 double mass, double laplace, const double* src, double* dst);
+int orc_dg_diagonal(const orc_mesh* m, int p, const int64_t* cell_dofs, int64_t n_dofs, int ncomp,
+                    double mass, double laplace, double* diag);
+/* per-cell penalty tau_K (n_cells) */
+int orc_dg_tau(const orc_mesh* m, double* tau);
+
 #ifdef __cplusplus
 }
 #endif

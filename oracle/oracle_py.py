@@ -27,7 +27,8 @@ def lib():
             raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
         _lib = C.CDLL(LIB_PATH)
         vp = C.c_void_p
-        for name in ["orc_cg_vmult", "orc_cg_diagonal", "orc_cg_matrix_vmult", "orc_cg_dense"]:
+        for name in ["orc_cg_vmult", "orc_cg_diagonal", "orc_cg_matrix_vmult", "orc_cg_dense",
+                     "orc_dg_vmult", "orc_dg_diagonal", "orc_dg_tau"]:
             getattr(_lib, name).restype = C.c_int
         _lib.orc_basis.argtypes = [C.c_int, vp, vp, vp]
         _lib.orc_tet_rule.argtypes = [C.c_int, vp, vp]
@@ -50,8 +51,7 @@ class OracleProblem:
 
     @staticmethod
     def from_operator(op):
-        perm = op.cell_order()
-        return OracleProblem(op.mesh.vertices, op.mesh.cells[perm], op.cell_dofs(),
+        return OracleProblem(op.mesh.vertices, op.cell_vertices(), op.cell_dofs(),
                              op.constrained(), op.info["degree"])
 
     def _call(self, fn, *args):
@@ -105,6 +105,81 @@ class OracleProblem:
                    C.c_void_p(ccells.ctypes.data), C.c_int(n), C.c_void_p(src.ctypes.data),
                    C.c_void_p(dst.ctypes.data))
         return dst
+
+
+class DGOracleProblem:
+    """DG SIPG operator (optionally vector / Helmholtz) on the operator's
+    internal cells; rows (k*ncomp + c)*n_loc + i."""
+
+    def __init__(self, vertices, cells, degree, ncomp=1, mass=0.0, laplace=1.0):
+        self.vertices = np.ascontiguousarray(vertices, np.float64)
+        self.cells = np.ascontiguousarray(cells, np.int32)
+        self.degree = int(degree)
+        self.ncomp = int(ncomp)
+        self.mass = float(mass)
+        self.laplace = float(laplace)
+        nl = (self.degree + 1) * (self.degree + 2) * (self.degree + 3) // 6
+        nc = self.cells.shape[0]
+        self.n_loc = nl
+        self.n_scalar = nc * nl
+        # oracle layout: component c of scalar DoF d is row c*n_scalar + d, with
+        # d = k*nl + i; map to the operator layout (k*ncomp + c)*nl + i
+        k = np.arange(nc)[:, None, None]
+        c = np.arange(self.ncomp)[None, :, None]
+        i = np.arange(nl)[None, None, :]
+        self.op_rows = ((k * self.ncomp + c) * nl + i)            # [k, c, i]
+        self.orc_rows = (c * self.n_scalar + k * nl + i)          # same entries, oracle layout
+        self.cell_dofs = np.ascontiguousarray(np.arange(self.n_scalar, dtype=np.int64))
+        self._m = OrcMesh(self.vertices.shape[0], self.vertices.ctypes.data, nc, self.cells.ctypes.data)
+
+    @staticmethod
+    def from_operator(op, mass=0.0, laplace=1.0):
+        return DGOracleProblem(op.mesh.vertices, op.cell_vertices(), op.info["degree"],
+                               op.info["n_components"], mass, laplace)
+
+    def _to_orc(self, x):
+        y = np.empty(x.size)
+        y[self.orc_rows.ravel()] = x[self.op_rows.ravel()]
+        return y
+
+    def _from_orc(self, y):
+        x = np.empty(y.size)
+        x[self.op_rows.ravel()] = y[self.orc_rows.ravel()]
+        return x
+
+    def vmult(self, src):
+        s = self._to_orc(np.asarray(src, np.float64))
+        d = np.empty_like(s)
+        rc = lib().orc_dg_vmult(C.byref(self._m), self.degree, C.c_void_p(self.cell_dofs.ctypes.data),
+                                C.c_int64(self.n_scalar), self.ncomp, C.c_double(self.mass),
+                                C.c_double(self.laplace), C.c_void_p(s.ctypes.data), C.c_void_p(d.ctypes.data))
+        if rc:
+            raise RuntimeError("orc_dg_vmult failed")
+        return self._from_orc(d)
+
+    def diagonal(self):
+        d = np.empty(self.n_scalar * self.ncomp)
+        rc = lib().orc_dg_diagonal(C.byref(self._m), self.degree, C.c_void_p(self.cell_dofs.ctypes.data),
+                                   C.c_int64(self.n_scalar), self.ncomp, C.c_double(self.mass),
+                                   C.c_double(self.laplace), C.c_void_p(d.ctypes.data))
+        if rc:
+            raise RuntimeError("orc_dg_diagonal failed")
+        return self._from_orc(d)
+
+    def tau(self):
+        t = np.empty(self.cells.shape[0])
+        lib().orc_dg_tau(C.byref(self._m), C.c_void_p(t.ctypes.data))
+        return t
+
+    def dense(self):
+        n = self.n_scalar * self.ncomp
+        A = np.empty((n, n))
+        e = np.zeros(n)
+        for j in range(n):
+            e[j] = 1.0
+            A[:, j] = self.vmult(e)
+            e[j] = 0.0
+        return A
 
 
 def basis(p, x):

@@ -57,6 +57,22 @@ void upload_cg(Operator& op) {
   op.on_device = true;
 }
 
+void upload_dg(Operator& op) {
+  DGPlan& pl = op.dg;
+  op.d_dg_nbr.upload(pl.nbr);
+  op.d_dg_nface.upload(pl.nface);
+  if (op.precision == MF_F64) {
+    op.d_dg_geo.upload(pl.geo);
+    op.d_dg_cell_tab.upload(pl.cell_tab);
+    op.d_dg_face_tab.upload(pl.face_tab);
+  } else {
+    op.d_dg_geo.upload(convert<float>(pl.geo));
+    op.d_dg_cell_tab.upload(convert<float>(pl.cell_tab));
+    op.d_dg_face_tab.upload(convert<float>(pl.face_tab));
+  }
+  op.on_device = true;
+}
+
 namespace {
 // Same schedule as cg_wave_kernel + cg_fixup_kernel, executed serially on the host.
 template <typename T>
@@ -158,7 +174,8 @@ int mf_debug_host_apply(mf_op_t h, int32_t mode, void* dst, const void* src) {
   MF_TRY({
     if (!h || !dst || (mode == 0 && !src)) throw std::runtime_error("null argument");
     if (mode != 0 && mode != 1) throw std::runtime_error("mode must be 0 or 1");
-    host_apply_cg(h->op, mode, dst, src);
+    if (h->op.is_dg()) host_apply_dg(h->op, mode, dst, src);
+    else host_apply_cg(h->op, mode, dst, src);
   });
 }
 
@@ -190,7 +207,8 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
     *out = nullptr;
     if (degree < 1 || degree > 3) throw std::runtime_error("degree must be 1, 2 or 3");
     if (precision != MF_F64 && precision != MF_F32) throw std::runtime_error("precision must be MF_F64 or MF_F32");
-    if (kind != MF_CG_POISSON) throw std::runtime_error("operator kind not available in this build");
+    if (kind != MF_CG_POISSON && kind != MF_DG_SIPG && kind != MF_DG_VECTOR && kind != MF_DG_HELMHOLTZ)
+      throw std::runtime_error("operator kind not available in this build");
     if (desc->n_vertices <= 0 || desc->n_cells < 0 || !desc->vertices || (!desc->cells && desc->n_cells))
       throw std::runtime_error("invalid mesh description");
     auto h = std::make_unique<mf_operator>();
@@ -206,12 +224,16 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
     for (int32_t v : op.mesh.cells)
       if (v < 0 || v >= desc->n_vertices) throw std::runtime_error("cell references a missing vertex");
     op.mesh.kuhn_n = desc->kuhn_n;
-    build_cg_plan(op.mesh, degree, op.opt, op.cg);
-    make_cg_shape(op.cg);
-    if (!op.opt.host_only) {
-      upload_cg(op);
-      SMF_HIP(hipDeviceSynchronize());
+    if (op.is_dg()) {
+      op.ncomp = kind == MF_DG_SIPG ? 1 : 3;
+      build_dg_plan(op.mesh, degree, op.ncomp, op.opt, kind == MF_DG_HELMHOLTZ, op.dg);
+      if (!op.opt.host_only) upload_dg(op);
+    } else {
+      build_cg_plan(op.mesh, degree, op.opt, op.cg);
+      make_cg_shape(op.cg);
+      if (!op.opt.host_only) upload_cg(op);
     }
+    if (!op.opt.host_only) SMF_HIP(hipDeviceSynchronize());
     *out = h.release();
   });
 }
@@ -225,7 +247,8 @@ int mf_vmult(mf_op_t h, void* dst, const void* src, void* stream) {
     if (!h || !dst || !src) throw std::runtime_error("null argument");
     if (dst == src) throw std::runtime_error("dst and src must not alias");
     require_device(h->op);
-    launch_cg(h->op, 0, dst, src, static_cast<hipStream_t>(stream));
+    if (h->op.is_dg()) launch_dg(h->op, 0, dst, src, static_cast<hipStream_t>(stream));
+    else launch_cg(h->op, 0, dst, src, static_cast<hipStream_t>(stream));
   });
 }
 
@@ -237,7 +260,8 @@ int mf_compute_diagonal(mf_op_t h, void* diag, void* stream) {
   MF_TRY({
     if (!h || !diag) throw std::runtime_error("null argument");
     require_device(h->op);
-    launch_cg(h->op, 1, diag, nullptr, static_cast<hipStream_t>(stream));
+    if (h->op.is_dg()) launch_dg(h->op, 1, diag, nullptr, static_cast<hipStream_t>(stream));
+    else launch_cg(h->op, 1, diag, nullptr, static_cast<hipStream_t>(stream));
   });
 }
 
@@ -246,7 +270,8 @@ int mf_compute_inverse_diagonal(mf_op_t h, void* inv, void* stream) {
     if (!h || !inv) throw std::runtime_error("null argument");
     require_device(h->op);
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    launch_cg(h->op, 1, inv, nullptr, s);
+    if (h->op.is_dg()) launch_dg(h->op, 1, inv, nullptr, s);
+    else launch_cg(h->op, 1, inv, nullptr, s);
     launch_invert_diagonal(h->op.precision, h->op.m(), inv, s);
   });
 }
@@ -257,53 +282,93 @@ int mf_get_info(mf_op_t h, mf_info* info) {
   MF_TRY({
     if (!h || !info) throw std::runtime_error("null argument");
     const Operator& op = h->op;
-    const CGPlan& pl = op.cg;
     std::memset(info, 0, sizeof(*info));
-    info->n_dofs = op.m();
-    info->n_cells = pl.n_cells;
-    info->n_constrained = pl.n_dofs - pl.n_free;
-    info->degree = op.degree;
-    info->dofs_per_cell = pl.n_loc;
-    info->n_q_points = pl.n_q;
-    info->n_components = op.ncomp;
-    info->n_patches = pl.n_patches;
-    info->n_patch_colors = 0;
-    info->max_cell_colors = pl.max_icolors;
-    info->max_local_dofs = pl.max_local;
-    info->precision = op.precision;
     const double w = (double)op.wbytes();
-    // reference data model: src + dst, 32-bit index per cell DoF, J^-T + det J per cell
-    info->algorithmic_bytes = 2.0 * w * pl.n_dofs + 4.0 * pl.n_loc * pl.n_cells + 10.0 * w * pl.n_cells;
-    info->flops = (12.0 * pl.n_loc * pl.n_q + 18.0 * pl.n_q) * pl.n_cells;
+    info->n_dofs = op.m();
+    info->degree = op.degree;
+    info->n_components = op.ncomp;
+    info->precision = op.precision;
+    if (op.is_dg()) {
+      const DGPlan& pl = op.dg;
+      info->n_cells = pl.n_cells;
+      info->dofs_per_cell = pl.n_loc;
+      info->n_q_points = pl.n_q;
+      info->n_interior_faces = pl.n_interior_faces;
+      info->n_boundary_faces = pl.n_boundary_faces;
+      // reference data model: src + dst; per cell J^-T, det J, tau + start index;
+      // per interior face nJ-, nJ+, face det J + 2 cell ids; per boundary face nJ, det J + id
+      info->algorithmic_bytes = 2.0 * w * pl.n_dofs + pl.n_cells * (11.0 * w + 4) +
+                                pl.n_interior_faces * (7.0 * w + 8) + pl.n_boundary_faces * (4.0 * w + 4);
+      info->flops = pl.ncomp * pl.n_cells *
+                    ((12.0 * pl.n_loc * pl.n_q + 18.0 * pl.n_q) + 4.0 * pl.n_qf * 24.0 * pl.n_loc);
+    } else {
+      const CGPlan& pl = op.cg;
+      info->n_cells = pl.n_cells;
+      info->n_constrained = pl.n_dofs - pl.n_free;
+      info->dofs_per_cell = pl.n_loc;
+      info->n_q_points = pl.n_q;
+      info->n_patches = pl.n_patches;
+      info->max_cell_colors = pl.max_icolors;
+      info->max_local_dofs = pl.max_local;
+      // reference data model: src + dst, 32-bit index per cell DoF, J^-T + det J per cell
+      info->algorithmic_bytes = 2.0 * w * pl.n_dofs + 4.0 * pl.n_loc * pl.n_cells + 10.0 * w * pl.n_cells;
+      info->flops = (12.0 * pl.n_loc * pl.n_q + 18.0 * pl.n_q) * pl.n_cells;
+    }
   });
 }
 
 int mf_get_cell_order(mf_op_t h, int64_t* perm) {
   MF_TRY({
     if (!h || !perm) throw std::runtime_error("null argument");
-    std::memcpy(perm, h->op.cg.cell_perm.data(), h->op.cg.cell_perm.size() * sizeof(int64_t));
+    const auto& v = h->op.is_dg() ? h->op.dg.cell_perm : h->op.cg.cell_perm;
+    std::memcpy(perm, v.data(), v.size() * sizeof(int64_t));
+  });
+}
+
+int mf_get_cell_vertices(mf_op_t h, int32_t* verts) {
+  MF_TRY({
+    if (!h || !verts) throw std::runtime_error("null argument");
+    const Operator& op = h->op;
+    if (op.is_dg()) {
+      std::memcpy(verts, op.dg.cell_verts.data(), op.dg.cell_verts.size() * sizeof(int32_t));
+    } else {
+      for (int64_t k = 0; k < op.cg.n_cells; ++k)
+        for (int t = 0; t < 4; ++t) verts[4 * k + t] = op.mesh.cells[4 * op.cg.cell_perm[k] + t];
+    }
   });
 }
 
 int mf_get_cell_dofs(mf_op_t h, int64_t* dofs) {
   MF_TRY({
     if (!h || !dofs) throw std::runtime_error("null argument");
-    std::memcpy(dofs, h->op.cg.cell_dofs.data(), h->op.cg.cell_dofs.size() * sizeof(int64_t));
+    const Operator& op = h->op;
+    if (op.is_dg()) {
+      const int64_t n = op.dg.n_dofs;
+      for (int64_t r = 0; r < n; ++r) dofs[r] = r;
+    } else {
+      std::memcpy(dofs, op.cg.cell_dofs.data(), op.cg.cell_dofs.size() * sizeof(int64_t));
+    }
   });
 }
 
 int mf_get_dof_points(mf_op_t h, double* xyz) {
   MF_TRY({
     if (!h || !xyz) throw std::runtime_error("null argument");
-    std::memcpy(xyz, h->op.cg.dof_points.data(), h->op.cg.dof_points.size() * sizeof(double));
+    const auto& v = h->op.is_dg() ? h->op.dg.dof_points : h->op.cg.dof_points;
+    std::memcpy(xyz, v.data(), v.size() * sizeof(double));
   });
 }
 
 int mf_get_constrained(mf_op_t h, uint8_t* mask) {
   MF_TRY({
     if (!h || !mask) throw std::runtime_error("null argument");
-    const CGPlan& pl = h->op.cg;
-    for (int64_t i = 0; i < pl.n_dofs; ++i) mask[i] = i >= pl.n_free ? 1 : 0;
+    const Operator& op = h->op;
+    if (op.is_dg()) {
+      std::memset(mask, 0, op.dg.n_cells * op.dg.n_loc);
+    } else {
+      const CGPlan& pl = op.cg;
+      for (int64_t i = 0; i < pl.n_dofs; ++i) mask[i] = i >= pl.n_free ? 1 : 0;
+    }
   });
 }
 

@@ -1,0 +1,253 @@
+// Discontinuous-Galerkin SIPG kernels for gfx950 (scalar and
+// components-batched vector / Helmholtz-like operators).
+//
+// One lane per cell; a workgroup (4 wave64) shares the reference tables in
+// LDS.  A lane evaluates the cell integral and all four face integrals of its
+// cell and writes only its own DoFs, so every DoF is written once, with no
+// atomics and no colouring: the result is bit-reproducible.  Interior faces
+// are evaluated by both adjacent cells (each computing its own test side);
+// the neighbour's trace uses the same canonical face parametrisation
+// (ascending global vertex order), so its table is indexed by its local face
+// number only.
+//
+// Reference: vmult_manual, DG/cells_batched/application.cc:1318-1332 with the
+// cell integral (1493-1735), the SIPG flux of interior faces (jump, average
+// normal derivative, sigma = (p+1)(p+3)/3 max(tau-,tau+), 2104-2187) and the
+// mirror-principle boundary flux (2688-2701); components batched as in
+// DG/components_batched/application.cc:715-1535; Helmholtz mass term as in
+// DG/components_batched/application_helmholtz.cc:1025-1111.
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+
+#include "op.h"
+
+namespace smf {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+template <int P, bool MASS>
+struct DGShape {
+  static constexpr int NL = (P + 1) * (P + 2) * (P + 3) / 6;
+  // cell rule: stiffness exact (2p-2) or, with a mass term, 2p
+  static constexpr int NQ = MASS ? (P == 1 ? 4 : (P == 2 ? 11 : 24)) : (P == 1 ? 1 : (P == 2 ? 4 : 11));
+  static constexpr int NQF = P == 1 ? 3 : (P == 2 ? 6 : 12);
+};
+
+__device__ __constant__ double kNhatD[4][3] = {{1, 1, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+
+// MODE 0: dst = A src; MODE 1: dst = diag(A)
+template <int P, typename T, int NC, bool MASS, int MODE>
+__global__ __launch_bounds__(kThreads) void dg_cell_kernel(
+    int64_t ncells, const T* __restrict__ geo, const int32_t* __restrict__ nbr,
+    const uint32_t* __restrict__ nface, const T* __restrict__ cell_tab,
+    const T* __restrict__ face_tab, T factor, T mass, T lap, const T* __restrict__ src,
+    T* __restrict__ dst) {
+  using S = DGShape<P, MASS>;
+  constexpr int NL = S::NL, NQ = S::NQ, NQF = S::NQF;
+  constexpr int CT = NQ * NL * 4 + NQ;            // cell table size
+  constexpr int FS = NQF * NL * 4;                // per-face stride
+  constexpr int FT = 4 * FS + NQF;                // face table size
+  extern __shared__ unsigned char smem_raw[];
+  T* sct = reinterpret_cast<T*>(smem_raw);
+  T* sft = sct + CT;
+  for (int j = threadIdx.x; j < CT; j += kThreads) sct[j] = cell_tab[j];
+  for (int j = threadIdx.x; j < FT; j += kThreads) sft[j] = face_tab[j];
+  __syncthreads();
+
+  const int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (k >= ncells) return;
+  const T* CG = sct;
+  const T* CW = sct + NQ * NL * 3;
+  const T* CV = CW + NQ;
+  const T* FW = sft + 4 * FS;
+
+  T jit[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) jit[t] = geo[t * ncells + k];
+  const T adet = geo[9 * ncells + k], tau = geo[10 * ncells + k];
+  const T M0 = adet * (jit[0] * jit[0] + jit[3] * jit[3] + jit[6] * jit[6]);
+  const T M1 = adet * (jit[0] * jit[1] + jit[3] * jit[4] + jit[6] * jit[7]);
+  const T M2 = adet * (jit[0] * jit[2] + jit[3] * jit[5] + jit[6] * jit[8]);
+  const T M3 = adet * (jit[1] * jit[1] + jit[4] * jit[4] + jit[7] * jit[7]);
+  const T M4 = adet * (jit[1] * jit[2] + jit[4] * jit[5] + jit[7] * jit[8]);
+  const T M5 = adet * (jit[2] * jit[2] + jit[5] * jit[5] + jit[8] * jit[8]);
+  const uint32_t nf = nface[k];
+
+#pragma unroll 1
+  for (int c = 0; c < NC; ++c) {
+    const int64_t base = (k * NC + c) * NL;
+    T u[NL], y[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      u[i] = MODE == 0 ? src[base + i] : T(0);
+      y[i] = T(0);
+    }
+    // ---- cell integral ----
+#pragma unroll 1
+    for (int q = 0; q < NQ; ++q) {
+      const T* G = CG + q * NL * 3;
+      const T* V = CV + q * NL;
+      const T w = CW[q];
+      if (MODE == 0) {
+        T gx = T(0), gy = T(0), gz = T(0), uv = T(0);
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          gx += G[3 * i] * u[i];
+          gy += G[3 * i + 1] * u[i];
+          gz += G[3 * i + 2] * u[i];
+          if (MASS) uv += V[i] * u[i];
+        }
+        const T wl = w * lap;
+        const T tx = wl * (M0 * gx + M1 * gy + M2 * gz);
+        const T ty = wl * (M1 * gx + M3 * gy + M4 * gz);
+        const T tz = wl * (M2 * gx + M4 * gy + M5 * gz);
+        const T tv = MASS ? w * mass * adet * uv : T(0);
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          y[i] += G[3 * i] * tx + G[3 * i + 1] * ty + G[3 * i + 2] * tz;
+          if (MASS) y[i] += V[i] * tv;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          const T gx = G[3 * i], gy = G[3 * i + 1], gz = G[3 * i + 2];
+          y[i] += w * lap * (gx * (M0 * gx + M1 * gy + M2 * gz) + gy * (M1 * gx + M3 * gy + M4 * gz) +
+                             gz * (M2 * gx + M4 * gy + M5 * gz));
+          if (MASS) y[i] += w * mass * adet * V[i] * V[i];
+        }
+      }
+    }
+    // ---- face integrals ----
+#pragma unroll 1
+    for (int f = 0; f < 4; ++f) {
+      const T* FV = sft + f * FS;
+      const T* FG = FV + NQF * NL;
+      T nj0 = jit[0] * (T)kNhatD[f][0] + jit[1] * (T)kNhatD[f][1] + jit[2] * (T)kNhatD[f][2];
+      T nj1 = jit[3] * (T)kNhatD[f][0] + jit[4] * (T)kNhatD[f][1] + jit[5] * (T)kNhatD[f][2];
+      T nj2 = jit[6] * (T)kNhatD[f][0] + jit[7] * (T)kNhatD[f][1] + jit[8] * (T)kNhatD[f][2];
+      const T len = sqrt(nj0 * nj0 + nj1 * nj1 + nj2 * nj2);
+      const T il = T(1) / len;
+      const T n0 = nj0 * il, n1 = nj1 * il, n2 = nj2 * il;
+      const T area2 = adet * len * lap;
+      const T nJ0 = jit[0] * n0 + jit[3] * n1 + jit[6] * n2;
+      const T nJ1 = jit[1] * n0 + jit[4] * n1 + jit[7] * n2;
+      const T nJ2 = jit[2] * n0 + jit[5] * n1 + jit[8] * n2;
+      const int32_t kb = nbr[f * ncells + k];
+      const bool interior = kb >= 0;
+      T sigma = factor * tau;
+      T nb0 = T(0), nb1 = T(0), nb2 = T(0);
+      T ub[NL];
+      const T* FVb = sft;
+      if (interior) {
+        const int fb = (nf >> (8 * f)) & 3;
+        FVb = sft + fb * FS;
+        T jb[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) jb[t] = geo[t * ncells + kb];
+        nb0 = jb[0] * n0 + jb[3] * n1 + jb[6] * n2;
+        nb1 = jb[1] * n0 + jb[4] * n1 + jb[7] * n2;
+        nb2 = jb[2] * n0 + jb[5] * n1 + jb[8] * n2;
+        const T taub = geo[10 * ncells + kb];
+        sigma = factor * (tau > taub ? tau : taub);
+        if (MODE == 0) {
+#pragma unroll
+          for (int i = 0; i < NL; ++i) ub[i] = src[(kb * NC + c) * NL + i];
+        }
+      }
+      const T* FGb = FVb + NQF * NL;
+#pragma unroll 1
+      for (int q = 0; q < NQF; ++q) {
+        const T* V = FV + q * NL;
+        const T* G = FG + q * NL * 3;
+        const T jxw = FW[q] * area2;
+        if (MODE == 0) {
+          T uv = T(0), dn = T(0);
+#pragma unroll
+          for (int i = 0; i < NL; ++i) {
+            uv += V[i] * u[i];
+            dn += (G[3 * i] * nJ0 + G[3 * i + 1] * nJ1 + G[3 * i + 2] * nJ2) * u[i];
+          }
+          T up = -uv, dnp = dn;
+          if (interior) {
+            const T* Vb = FVb + q * NL;
+            const T* Gb = FGb + q * NL * 3;
+            up = T(0);
+            dnp = T(0);
+#pragma unroll
+            for (int i = 0; i < NL; ++i) {
+              up += Vb[i] * ub[i];
+              dnp += (Gb[3 * i] * nb0 + Gb[3 * i + 1] * nb1 + Gb[3 * i + 2] * nb2) * ub[i];
+            }
+          }
+          const T jump = uv - up, avg = T(0.5) * (dn + dnp);
+          const T tv = (sigma * jump - avg) * jxw, tg = T(-0.5) * jump * jxw;
+#pragma unroll
+          for (int i = 0; i < NL; ++i)
+            y[i] += V[i] * tv + (G[3 * i] * nJ0 + G[3 * i + 1] * nJ1 + G[3 * i + 2] * nJ2) * tg;
+        } else {
+          const T s = interior ? T(1) : T(2);
+#pragma unroll
+          for (int i = 0; i < NL; ++i) {
+            const T vi = V[i];
+            const T di = G[3 * i] * nJ0 + G[3 * i + 1] * nJ1 + G[3 * i + 2] * nJ2;
+            const T jump = s * vi, avg = T(0.5) * s * di;
+            y[i] += jxw * ((sigma * jump - avg) * vi - T(0.5) * jump * di);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NL; ++i) dst[base + i] = y[i];
+  }
+}
+
+template <int P, typename T, int NC, bool MASS, int MODE>
+void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
+  using S = DGShape<P, MASS>;
+  const DGPlan& pl = op.dg;
+  if (pl.n_q != S::NQ || pl.n_qf != S::NQF) throw std::runtime_error("DG table size mismatch");
+  const size_t shmem = (size_t)(S::NQ * S::NL * 4 + S::NQ + 4 * S::NQF * S::NL * 4 + S::NQF) * sizeof(T);
+  auto kern = dg_cell_kernel<P, T, NC, MASS, MODE>;
+  if (shmem > 65536)
+    SMF_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  if (pl.n_cells > 0)
+    hipLaunchKernelGGL(kern, dim3((unsigned)((pl.n_cells + kThreads - 1) / kThreads)), dim3(kThreads), shmem, s,
+                       pl.n_cells, op.d_dg_geo.as<T>(), op.d_dg_nbr.as<int32_t>(), op.d_dg_nface.as<uint32_t>(),
+                       op.d_dg_cell_tab.as<T>(), op.d_dg_face_tab.as<T>(), (T)pl.penalty_factor, (T)pl.mass,
+                       (T)pl.laplace, src, dst);
+  SMF_HIP(hipGetLastError());
+}
+
+template <int P, typename T, int MODE>
+void launch_dg_p(const Operator& op, T* dst, const T* src, hipStream_t s) {
+  if (op.kind == MF_DG_SIPG) launch_dg_t<P, T, 1, false, MODE>(op, dst, src, s);
+  else if (op.kind == MF_DG_VECTOR) launch_dg_t<P, T, 3, false, MODE>(op, dst, src, s);
+  else launch_dg_t<P, T, 3, true, MODE>(op, dst, src, s);
+}
+
+template <typename T>
+void launch_dg_prec(const Operator& op, int mode, T* dst, const T* src, hipStream_t s) {
+  switch (op.dg.p * 2 + mode) {
+    case 2: launch_dg_p<1, T, 0>(op, dst, src, s); break;
+    case 3: launch_dg_p<1, T, 1>(op, dst, src, s); break;
+    case 4: launch_dg_p<2, T, 0>(op, dst, src, s); break;
+    case 5: launch_dg_p<2, T, 1>(op, dst, src, s); break;
+    case 6: launch_dg_p<3, T, 0>(op, dst, src, s); break;
+    case 7: launch_dg_p<3, T, 1>(op, dst, src, s); break;
+    default: throw std::runtime_error("unsupported DG degree");
+  }
+}
+
+}  // namespace
+
+void launch_dg(const Operator& op, int mode, void* dst, const void* src, hipStream_t s) {
+  if (op.precision == MF_F64)
+    launch_dg_prec<double>(op, mode, static_cast<double*>(dst), static_cast<const double*>(src), s);
+  else
+    launch_dg_prec<float>(op, mode, static_cast<float*>(dst), static_cast<const float*>(src), s);
+}
+
+}  // namespace smf

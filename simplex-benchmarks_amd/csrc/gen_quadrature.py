@@ -9,8 +9,9 @@ moment equations
     sum_k w_k * x_k^a y_k^b z_k^c = a! b! c! / (a+b+c+3)!      (tetrahedron)
     sum_k w_k * x_k^a y_k^b       = a! b!   / (a+b+2)!          (triangle)
 
-for every monomial of total degree <= d, and the result is accepted only if all
-of those equations hold to 1e-14.  The reference element is the unit simplex
+for every monomial of total degree <= d, refined by Gauss-Newton in 40-digit arithmetic,
+and accepted only if all of those equations hold to 1e-30 before rounding to
+double.  The reference element is the unit simplex
 with vertices 0, e_x, e_y (, e_z).
 
 Rules (degree of exactness, number of points):
@@ -23,6 +24,7 @@ import itertools
 import math
 import sys
 
+import mpmath
 import numpy as np
 from scipy.optimize import least_squares
 
@@ -102,13 +104,90 @@ def residual(dim, deg, orbits, params):
     return np.array(res)
 
 
+def refine_mp(dim, deg, orbits, x0):
+    """Gauss-Newton in 40-digit arithmetic (mpmath) on the moment equations,
+    starting from the double-precision solution."""
+    mpmath.mp.dps = 40
+    x = [mpmath.mpf(float(v)) for v in x0]
+    mons = list(monomials(dim, deg))
+
+    def res(xv):
+        pts, wts = build_mp(dim, orbits, xv)
+        out = []
+        for e in mons:
+            val = mpmath.mpf(0)
+            for pt, w in zip(pts, wts):
+                term = w
+                for a, xa in zip(e, pt):
+                    term *= xa ** a
+                val += term
+            num = mpmath.mpf(1)
+            for a in e:
+                num *= mpmath.factorial(a)
+            out.append(val - num / mpmath.factorial(sum(e) + dim))
+        return mpmath.matrix(out)
+
+    for _ in range(30):
+        r = res(x)
+        h = mpmath.mpf(10) ** -25
+        Jm = mpmath.matrix(len(r), len(x))
+        for j in range(len(x)):
+            xp = list(x)
+            xp[j] += h
+            rp = res(xp)
+            for i in range(len(r)):
+                Jm[i, j] = (rp[i] - r[i]) / h
+        dx = mpmath.lu_solve(Jm.T * Jm, Jm.T * r)
+        x = [xi - dxi for xi, dxi in zip(x, dx)]
+        if mpmath.norm(dx) < mpmath.mpf(10) ** -32:
+            break
+    return x, float(mpmath.norm(res(x), mpmath.inf))
+
+
+def build_mp(dim, orbits, params):
+    pts, wts = [], []
+    k = 0
+    one, half = mpmath.mpf(1), mpmath.mpf(1) / 2
+    for kind in orbits:
+        npar = NPAR[kind]
+        p = params[k: k + npar]
+        w = params[k + npar]
+        k += npar + 1
+        if kind == "S4":
+            orb = [(one / 4,) * 4]
+        elif kind == "S31":
+            a = p[0]
+            orb = sorted(set(itertools.permutations((a, a, a, one - 3 * a))))
+        elif kind == "S22":
+            a = p[0]
+            orb = sorted(set(itertools.permutations((a, a, half - a, half - a))))
+        elif kind == "S211":
+            a, c = p
+            orb = sorted(set(itertools.permutations((a, a, c, one - 2 * a - c))))
+        elif kind == "S3":
+            orb = [(one / 3,) * 3]
+        elif kind == "S21":
+            a = p[0]
+            orb = sorted(set(itertools.permutations((a, a, one - 2 * a))))
+        elif kind == "S111":
+            a, c = p
+            orb = sorted(set(itertools.permutations((a, c, one - a - c))))
+        for bary in orb:
+            pts.append(bary[1:])
+            wts.append(w)
+    return pts, wts
+
+
 def solve(dim, deg, orbits, guess):
     sol = least_squares(lambda p: residual(dim, deg, orbits, p), guess,
                         xtol=1e-15, ftol=1e-15, gtol=1e-15, max_nfev=20000)
-    r = np.max(np.abs(residual(dim, deg, orbits, sol.x)))
-    if r > 1e-14:
-        raise RuntimeError(f"rule dim={dim} deg={deg} not exact: {r}")
-    pts, wts = build(dim, orbits, sol.x)
+    xmp, rmp = refine_mp(dim, deg, orbits, sol.x)
+    if rmp > 1e-30:
+        raise RuntimeError(f"rule dim={dim} deg={deg} not exact in extended precision: {rmp}")
+    r = np.max(np.abs(residual(dim, deg, orbits, np.array([float(v) for v in xmp]))))
+    pts_mp, wts_mp = build_mp(dim, orbits, xmp)
+    pts = np.array([[float(v) for v in pt] for pt in pts_mp])
+    wts = np.array([float(w) for w in wts_mp])
     # all points must lie inside the simplex
     bary0 = 1 - pts.sum(axis=1)
     if np.min(pts) < -1e-14 or np.min(bary0) < -1e-14:

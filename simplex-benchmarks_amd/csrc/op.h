@@ -79,16 +79,39 @@ struct CGPlan {
   std::vector<double> shape;         // G (n_loc x n_q x 3) then w_q G
 };
 
+// Host-side plan of a discontinuous-Galerkin (SIPG) operator.  Cell local
+// vertices are ordered by ascending global id, so the two cells of a face
+// parametrise it identically (no orientation tables).  DoFs are
+// cell-contiguous: cell k, component c, node i -> (k*ncomp + c)*n_loc + i.
+struct DGPlan {
+  int p = 0, n_loc = 0, n_q = 0, n_qf = 0, ncomp = 1;
+  int64_t n_cells = 0, n_dofs = 0;  // n_dofs counts all components
+  int64_t n_interior_faces = 0, n_boundary_faces = 0;
+  std::vector<int64_t> cell_perm;   // internal cell -> input cell
+  std::vector<int32_t> cell_verts;  // 4*n_cells, ascending global ids (internal order)
+  std::vector<double> geo;          // 11 x n_cells: J^-T (row-major 9), |det J|, tau
+  std::vector<int32_t> nbr;         // 4 x n_cells: neighbour across local face f, -1 on the boundary
+  std::vector<uint32_t> nface;      // per cell: neighbour's local face for f in bits 8f..8f+1
+  std::vector<double> dof_points;   // 3 * n_cells * n_loc
+  // tables: cell gradients [q][i][d] + w[q], cell values [q][i]; per face f:
+  // values [f][q][i], gradients [f][q][i][d], weights [q]
+  std::vector<double> cell_tab, face_tab;
+  double penalty_factor = 0, mass = 0, laplace = 1;
+};
+
 struct Operator {
   int kind = MF_CG_POISSON, precision = MF_F64, degree = 1, ncomp = 1;
   mf_options opt{};
   Mesh mesh;            // kept for transfers and RHS assembly
   CGPlan cg;
+  DGPlan dg;
+  bool is_dg() const { return kind == MF_DG_SIPG || kind == MF_DG_VECTOR || kind == MF_DG_HELMHOLTZ; }
   // device data
   DevBuf d_cell_start, d_own_start, d_ghost_start, d_ghost_idx, d_ccol, d_fix_ptr,
       d_fix_src, d_lidx, d_metric, d_shape, d_partial;
+  DevBuf d_dg_geo, d_dg_nbr, d_dg_nface, d_dg_cell_tab, d_dg_face_tab;
   bool on_device = false;
-  int64_t m() const { return cg.n_dofs * ncomp; }
+  int64_t m() const { return is_dg() ? dg.n_dofs : cg.n_dofs * ncomp; }
   size_t wbytes() const { return precision == MF_F64 ? 8 : 4; }
 };
 
@@ -99,5 +122,11 @@ void host_apply_cg(const Operator& op, int mode, void* dst, const void* src);
 void upload_cg(Operator& op);
 // mode 0: dst = A src; mode 1: dst = diag(A)
 void launch_cg(const Operator& op, int mode, void* dst, const void* src, hipStream_t s);
+
+void build_dg_plan(const Mesh& mesh, int p, int ncomp, const mf_options& opt, bool helmholtz,
+                   DGPlan& plan);
+void upload_dg(Operator& op);
+void launch_dg(const Operator& op, int mode, void* dst, const void* src, hipStream_t s);
+void host_apply_dg(const Operator& op, int mode, void* dst, const void* src);
 
 }  // namespace smf

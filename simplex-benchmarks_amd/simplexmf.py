@@ -79,6 +79,7 @@ def lib():
         "mf_get_info": (C.c_int, [vp, C.POINTER(Info)]),
         "mf_get_cell_order": (C.c_int, [vp, vp]),
         "mf_get_cell_dofs": (C.c_int, [vp, vp]),
+        "mf_get_cell_vertices": (C.c_int, [vp, vp]),
         "mf_get_dof_points": (C.c_int, [vp, vp]),
         "mf_get_constrained": (C.c_int, [vp, vp]),
         "mf_debug_host_apply": (C.c_int, [vp, i32, vp, vp]),
@@ -126,7 +127,7 @@ def exported_symbols():
 _SIGNATURE_NAMES = [
     "mf_mesh_kuhn_cube", "mf_default_options", "mf_create", "mf_destroy", "mf_vmult",
     "mf_tvmult", "mf_compute_diagonal", "mf_compute_inverse_diagonal", "mf_m", "mf_get_info",
-    "mf_get_cell_order", "mf_get_cell_dofs", "mf_get_dof_points", "mf_get_constrained",
+    "mf_get_cell_order", "mf_get_cell_dofs", "mf_get_cell_vertices", "mf_get_dof_points", "mf_get_constrained",
     "mf_debug_host_apply", "mf_last_error", "mf_device_count", "mf_device_name", "mf_malloc",
     "mf_free", "mf_memcpy_htod", "mf_memcpy_dtoh", "mf_memcpy_dtod", "mf_memset",
     "mf_synchronize", "mf_stream_create", "mf_stream_destroy", "mf_stream_synchronize",
@@ -331,12 +332,23 @@ class Operator:
         check(lib().mf_get_cell_order(self.handle, out.ctypes.data))
         return out
 
+    def cell_vertices(self):
+        out = np.empty((self.info["n_cells"], 4), np.int32)
+        check(lib().mf_get_cell_vertices(self.handle, out.ctypes.data))
+        return out
+
+    def is_dg(self):
+        return self.kind in (MF_DG_SIPG, MF_DG_VECTOR, MF_DG_HELMHOLTZ)
+
     def cell_dofs(self):
-        out = np.empty((self.info["n_cells"], self.info["dofs_per_cell"]), np.int64)
+        w = self.info["dofs_per_cell"] * (self.info["n_components"] if self.is_dg() else 1)
+        out = np.empty((self.info["n_cells"], w), np.int64)
         check(lib().mf_get_cell_dofs(self.handle, out.ctypes.data))
         return out
 
     def n_scalar_dofs(self):
+        if self.is_dg():
+            return self.info["n_cells"] * self.info["dofs_per_cell"]
         return self.info["n_dofs"] // max(1, self.info["n_components"])
 
     def dof_points(self):

@@ -70,7 +70,7 @@ def test_basis_kronecker_and_closed_form(orc, p):
         v, g = orc.basis(p, x)
         assert abs(v.sum() - 1) < 1e-13
         assert np.allclose(g.sum(axis=0), 0, atol=1e-12)
-        assert np.allclose(v, closed_form(p, x), atol=1e-12)
+        assert np.allclose(v, closed_form(p, x), atol=2e-15)
         # gradient against a central difference of the closed form
         h = 1e-6
         for d in range(3):
