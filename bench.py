@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--precision", default="f64")
     ap.add_argument("--no-reorder", action="store_true")
     ap.add_argument("--patch-cells", type=int, default=0)
+    ap.add_argument("--accumulate", type=int, default=0, help="CG LDS accumulation: 0 auto, 1 atomics, 2 read-add-write")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="0 to skip the CPU leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--seed", type=int, default=7)
@@ -91,7 +92,8 @@ def main():
     t0 = time.time()
     mesh = smf.Mesh.kuhn_cube(args.n, 0.2, args.seed)
     op = smf.Operator(mesh, args.degree, kind=args.kind, precision=args.precision,
-                      reorder=not args.no_reorder, patch_cells=args.patch_cells)
+                      reorder=not args.no_reorder, patch_cells=args.patch_cells,
+                      accumulate=args.accumulate)
     info = op.info
     log(f"setup {time.time() - t0:.1f}s: {info}")
     x = np.random.default_rng(args.seed).uniform(-1, 1, op.m())

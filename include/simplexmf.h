@@ -53,6 +53,7 @@ typedef struct mf_options {
   double laplace_coefficient; /* Helmholtz Laplace coefficient (default 1) */
   int32_t penalty_kind;     /* 0: (p+1)(p+3)/3 max(tau-,tau+) (reference); 1: (p+1)(p+2)/h_face */
   int32_t host_only;        /* 1: build the host plan only (no device memory); device entry points fail */
+  int32_t accumulate;       /* CG LDS accumulation: 0 auto, 1 LDS atomics, 2 read-add-write per colour */
 } mf_options;
 
 typedef struct mf_info {

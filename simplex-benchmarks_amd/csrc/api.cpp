@@ -189,6 +189,7 @@ void mf_default_options(mf_options* opt) {
   opt->laplace_coefficient = 1.0;
   opt->penalty_kind = 0;
   opt->host_only = 0;
+  opt->accumulate = 0;
 }
 
 int mf_mesh_kuhn_cube(int32_t n, double perturb, uint64_t seed, double* vertices, int32_t* cells) {
@@ -206,6 +207,7 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
     if (!desc || !out) throw std::runtime_error("null argument");
     *out = nullptr;
     if (degree < 1 || degree > 3) throw std::runtime_error("degree must be 1, 2 or 3");
+    if (opt && (opt->accumulate < 0 || opt->accumulate > 2)) throw std::runtime_error("accumulate must be 0, 1 or 2");
     if (precision != MF_F64 && precision != MF_F32) throw std::runtime_error("precision must be MF_F64 or MF_F32");
     if (kind != MF_CG_POISSON && kind != MF_DG_SIPG && kind != MF_DG_VECTOR && kind != MF_DG_HELMHOLTZ)
       throw std::runtime_error("operator kind not available in this build");

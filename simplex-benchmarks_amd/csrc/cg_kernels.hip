@@ -27,6 +27,7 @@
 #include <stdexcept>
 
 #include "op.h"
+#include "shape_tables.h"
 
 namespace smf {
 
@@ -43,7 +44,9 @@ struct CGShape {
 
 // MODE 0: y = A_K u;  MODE 1: y_i = (A_K)_ii
 // shape: G[(q*NL + i)*3 + d] = d phi_i / d xhat_d at point q, then w[q].
-template <int P, typename T, int MODE>
+// ATOMIC: accumulate with ds_add (fast for f64 on gfx950) or with a plain
+// LDS read-add-write per colour (ds_add_f32 is slow on gfx950).
+template <int P, typename T, int MODE, bool ATOMIC>
 __global__ __launch_bounds__(kWave) void cg_wave_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ own_start,
     const int32_t* __restrict__ ghost_start, const int32_t* __restrict__ ghost_idx,
@@ -59,16 +62,17 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
   const int o0 = own_start[patch], no = own_start[patch + 1] - o0;
   const int g0 = ghost_start[patch], ng = ghost_start[patch + 1] - g0;
   const int nloc = no + ng;
-  T* sy = su + nloc;
+  // slot nloc is a dummy for constrained DoFs (index 0xFFFF clamps to it):
+  // it reads as zero and absorbs their contributions, so no branches
+  T* sy = su + nloc + 1;
 
   // 1. stage the patch DoFs
-  for (int j = lane; j < nloc; j += kWave) {
-    if (MODE == 0) su[j] = j < no ? src[o0 + j] : src[ghost_idx[g0 + j - no]];
+  for (int j = lane; j <= nloc; j += kWave) {
+    if (MODE == 0) su[j] = j < no ? src[o0 + j] : (j < nloc ? src[ghost_idx[g0 + j - no]] : T(0));
     sy[j] = T(0);
   }
   __syncthreads();
 
-  const T* W = shape + NQ * NL * 3;
   // per-round cell data, loaded one round ahead (software pipelining)
   uint16_t li_n[NL];
   T m_n[6];
@@ -89,43 +93,84 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
     T y[NL];
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
-      li[i] = li_n[i];
+      li[i] = min((int)li_n[i], nloc);
       y[i] = T(0);
     }
     const T m0 = m_n[0], m1 = m_n[1], m2 = m_n[2], m3 = m_n[3], m4 = m_n[4], m5 = m_n[5];
     const int mycol = col_n;
     if (r0 + kWave < nc) load_round(r0 + kWave);
-    if (MODE == 0) {
-      T u[NL];
+    // P1/P2: compile-time tables (literal constants after unrolling); P3
+    // has too many distinct values for scalar registers and reads the table
+    // from memory one quadrature point at a time.
+    if constexpr (P <= 2) {
+      constexpr tab::CellGrad<P> TB = tab::make_cell_grad<P>();
+      if (MODE == 0) {
+        T u[NL];
 #pragma unroll
-      for (int i = 0; i < NL; ++i) u[i] = li[i] != kInvalid ? su[li[i]] : T(0);
-#pragma unroll(P == 1 ? NQ : 1)
-      for (int q = 0; q < NQ; ++q) {
-        const T* G = shape + q * NL * 3;
-        T gx = T(0), gy = T(0), gz = T(0);
+        for (int i = 0; i < NL; ++i) u[i] = su[li[i]];
 #pragma unroll
-        for (int i = 0; i < NL; ++i) {
-          gx += G[i * 3 + 0] * u[i];
-          gy += G[i * 3 + 1] * u[i];
-          gz += G[i * 3 + 2] * u[i];
+        for (int q = 0; q < NQ; ++q) {
+          T gx = T(0), gy = T(0), gz = T(0);
+#pragma unroll
+          for (int i = 0; i < NL; ++i) {
+            gx += T(TB.g[q][i][0]) * u[i];
+            gy += T(TB.g[q][i][1]) * u[i];
+            gz += T(TB.g[q][i][2]) * u[i];
+          }
+          const T w = T(TB.w[q]);
+          const T tx = w * (m0 * gx + m1 * gy + m2 * gz);
+          const T ty = w * (m1 * gx + m3 * gy + m4 * gz);
+          const T tz = w * (m2 * gx + m4 * gy + m5 * gz);
+#pragma unroll
+          for (int i = 0; i < NL; ++i)
+            y[i] += T(TB.g[q][i][0]) * tx + T(TB.g[q][i][1]) * ty + T(TB.g[q][i][2]) * tz;
         }
-        const T w = W[q];
-        const T tx = w * (m0 * gx + m1 * gy + m2 * gz);
-        const T ty = w * (m1 * gx + m3 * gy + m4 * gz);
-        const T tz = w * (m2 * gx + m4 * gy + m5 * gz);
+      } else {
 #pragma unroll
-        for (int i = 0; i < NL; ++i) y[i] += G[i * 3 + 0] * tx + G[i * 3 + 1] * ty + G[i * 3 + 2] * tz;
+        for (int q = 0; q < NQ; ++q) {
+          const T w = T(TB.w[q]);
+#pragma unroll
+          for (int i = 0; i < NL; ++i) {
+            const T gx = T(TB.g[q][i][0]), gy = T(TB.g[q][i][1]), gz = T(TB.g[q][i][2]);
+            y[i] += w * (gx * (m0 * gx + m1 * gy + m2 * gz) + gy * (m1 * gx + m3 * gy + m4 * gz) +
+                         gz * (m2 * gx + m4 * gy + m5 * gz));
+          }
+        }
       }
     } else {
+      const T* W = shape + NQ * NL * 3;
+      if (MODE == 0) {
+        T u[NL];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const T* G = shape + q * NL * 3;
-        const T w = W[q];
+        for (int i = 0; i < NL; ++i) u[i] = su[li[i]];
+#pragma unroll 1
+        for (int q = 0; q < NQ; ++q) {
+          const T* G = shape + q * NL * 3;
+          T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
-        for (int i = 0; i < NL; ++i) {
-          const T gx = G[i * 3 + 0], gy = G[i * 3 + 1], gz = G[i * 3 + 2];
-          y[i] += w * (gx * (m0 * gx + m1 * gy + m2 * gz) + gy * (m1 * gx + m3 * gy + m4 * gz) +
-                       gz * (m2 * gx + m4 * gy + m5 * gz));
+          for (int i = 0; i < NL; ++i) {
+            gx += G[i * 3 + 0] * u[i];
+            gy += G[i * 3 + 1] * u[i];
+            gz += G[i * 3 + 2] * u[i];
+          }
+          const T w = W[q];
+          const T tx = w * (m0 * gx + m1 * gy + m2 * gz);
+          const T ty = w * (m1 * gx + m3 * gy + m4 * gz);
+          const T tz = w * (m2 * gx + m4 * gy + m5 * gz);
+#pragma unroll
+          for (int i = 0; i < NL; ++i) y[i] += G[i * 3 + 0] * tx + G[i * 3 + 1] * ty + G[i * 3 + 2] * tz;
+        }
+      } else {
+#pragma unroll 1
+        for (int q = 0; q < NQ; ++q) {
+          const T* G = shape + q * NL * 3;
+          const T w = W[q];
+#pragma unroll
+          for (int i = 0; i < NL; ++i) {
+            const T gx = G[i * 3 + 0], gy = G[i * 3 + 1], gz = G[i * 3 + 2];
+            y[i] += w * (gx * (m0 * gx + m1 * gy + m2 * gz) + gy * (m1 * gx + m3 * gy + m4 * gz) +
+                         gz * (m2 * gx + m4 * gy + m5 * gz));
+          }
         }
       }
     }
@@ -135,9 +180,18 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
     const int chi = ccol[c0 + last];
     for (int c = clo; c <= chi; ++c) {
       if (active && mycol == c) {
+        if (ATOMIC) {
 #pragma unroll
-        for (int i = 0; i < NL; ++i)
-          if (li[i] != kInvalid) atomicAdd(&sy[li[i]], y[i]);
+          for (int i = 0; i < NL; ++i) atomicAdd(&sy[li[i]], y[i]);
+        } else {
+          // distinct addresses within a colour except the dummy slot, whose
+          // value is never read
+          T v[NL];
+#pragma unroll
+          for (int i = 0; i < NL; ++i) v[i] = sy[li[i]];
+#pragma unroll
+          for (int i = 0; i < NL; ++i) sy[li[i]] = v[i] + y[i];
+        }
       }
     }
   }
@@ -174,8 +228,9 @@ __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* _
 template <int P, typename T, int MODE>
 void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
   const CGPlan& pl = op.cg;
-  const size_t shmem = 2 * (size_t)pl.max_local * sizeof(T);
-  auto kern = cg_wave_kernel<P, T, MODE>;
+  const size_t shmem = 2 * ((size_t)pl.max_local + 1) * sizeof(T);
+  const bool atomic = op.opt.accumulate == 1 || (op.opt.accumulate == 0 && sizeof(T) == 8);
+  auto kern = atomic ? cg_wave_kernel<P, T, MODE, true> : cg_wave_kernel<P, T, MODE, false>;
   if (shmem > 65536)
     SMF_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
   if (pl.n_patches > 0)

@@ -36,7 +36,7 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   plan.n_cells = mesh.nc;
   const int64_t nc = mesh.nc, nv = mesh.nv;
 
-  const int patch = opt.patch_cells > 0 ? opt.patch_cells : (p == 1 ? 512 : (p == 2 ? 256 : 128));
+  const int patch = opt.patch_cells > 0 ? opt.patch_cells : (p == 1 ? 1024 : (p == 2 ? 512 : 128));
   if (patch > 4096) throw std::runtime_error("patch_cells must be <= 4096");
   plan.max_patch_cells = 0;
 

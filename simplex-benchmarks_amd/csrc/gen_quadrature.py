@@ -224,11 +224,11 @@ def main():
         names.append((shape, deg, len(wts), name))
         out.write(f"// {shape} degree {deg}: {len(wts)} points, orbits {'+'.join(orbits)}, "
                   f"max moment residual {r:.1e}\n")
-        out.write(f"static const double {name}_pts[{len(wts)}][{dim}] = {{\n")
+        out.write(f"constexpr double {name}_pts[{len(wts)}][{dim}] = {{\n")
         for p in pts:
             out.write("  {" + ", ".join(f"{v:.17g}" for v in p) + "},\n")
         out.write("};\n")
-        out.write(f"static const double {name}_w[{len(wts)}] = {{\n  ")
+        out.write(f"constexpr double {name}_w[{len(wts)}] = {{\n  ")
         out.write(", ".join(f"{w:.17g}" for w in wts))
         out.write("\n};\n\n")
     out.write("}  // namespace smf\n")

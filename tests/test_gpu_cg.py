@@ -85,3 +85,22 @@ def test_deterministic_and_tvmult(gpu):
     op.Tvmult(c, src)
     assert np.array_equal(a.numpy(), b.numpy())
     assert np.array_equal(a.numpy(), c.numpy())
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("accumulate", [1, 2])
+@pytest.mark.parametrize("patch_cells", [64, 1024])
+def test_accumulation_variants(gpu, orc, precision, accumulate, patch_cells):
+    """Both LDS accumulation schemes and extreme patch sizes give the oracle's result."""
+    smf = gpu
+    print(f"\ncase: CG p=2 {precision} accumulate={accumulate} patch={patch_cells}", flush=True)
+    mesh = smf.Mesh.kuhn_cube(10, 0.2, 3)
+    op = smf.Operator(mesh, 2, precision=precision, accumulate=accumulate, patch_cells=patch_cells)
+    pr = orc.OracleProblem.from_operator(op)
+    x = np.random.default_rng(7).uniform(-1, 1, op.m())
+    src = smf.DeviceVector.from_numpy(x.astype(op.dtype))
+    a, b = op.initialize_dof_vector(), op.initialize_dof_vector()
+    op.vmult(a, src)
+    op.vmult(b, src)
+    assert np.array_equal(a.numpy(), b.numpy())
+    assert rel_err(a.numpy().astype(np.float64), pr.vmult(x)) < TOL[precision]
