@@ -119,6 +119,64 @@ int mf_get_constrained(mf_op_t op, uint8_t* mask);
 
 const char* mf_last_error(void);
 
+/* ---- hybrid multigrid and solvers (device) ---------------------------------
+ * Replaces MultigridPreconditioner + SolverCG of the reference
+ * (CG/cells_batched/application_ph.cc:1754-2026, application_hp.cc:1933-1992):
+ * levels P_k..P_1 on the fine mesh (p-MG, decrease by one), then P1 on Kuhn
+ * meshes of half size (h-MG), Chebyshev-Jacobi smoothers, an FP64 dense
+ * coarse solve (in place of the reference's Trilinos ML V-cycle) and a
+ * V-cycle in the level precision (FP32 by default) inside FP64 CG. */
+typedef struct mf_multigrid* mf_mg_t;
+
+typedef struct mf_mg_options {
+  int32_t smoother_degree;    /* Chebyshev degree (5) */
+  double smoothing_range;     /* lambda_max / lambda_min (20) */
+  int32_t eig_cg_iterations;  /* CG steps of the lambda_max estimate (10) */
+  int32_t level_precision;    /* MF_F32 (default) or MF_F64 */
+  int32_t max_coarse_dofs;    /* limit of the dense coarse solver (4096) */
+} mf_mg_options;
+
+#define MF_MAX_LEVELS 16
+typedef struct mf_mg_info {
+  int32_t n_levels;                     /* [0] = coarsest */
+  int32_t degree[MF_MAX_LEVELS];
+  int32_t kuhn_n[MF_MAX_LEVELS];
+  int64_t n_dofs[MF_MAX_LEVELS];
+  double lambda_max[MF_MAX_LEVELS];     /* Chebyshev upper bound (1.2 x estimate) */
+  int64_t vcycles;                      /* V-cycles applied so far */
+} mf_mg_info;
+
+typedef struct mf_solve_stats {
+  int32_t iterations;
+  int32_t converged;
+  double initial_residual, final_residual, rhs_norm;
+} mf_solve_stats;
+
+void mf_mg_default_options(mf_mg_options* opt);
+/* Builds the hierarchy below a CG Poisson operator (needs kuhn_n > 0 for h-levels). */
+int mf_mg_create(mf_op_t fine, const mf_mg_options* opt, mf_mg_t* out);
+/* dst = V-cycle(src) in the fine operator's precision (PreconditionMG::vmult) */
+int mf_mg_vmult(mf_mg_t mg, void* dst, const void* src, void* stream);
+int mf_mg_get_info(mf_mg_t mg, mf_mg_info* info);
+int mf_mg_destroy(mf_mg_t mg);
+/* (F)CG on an FP64 operator from x = 0 until ||r|| <= rel_tol ||b||; mg may be
+ * NULL; flexible != 0 uses Notay's flexible CG.  Synchronises the stream. */
+int mf_solve_cg(mf_op_t A, mf_mg_t mg, void* x, const void* b, double rel_tol, int32_t max_iter,
+                int32_t flexible, mf_solve_stats* stats, void* stream);
+/* Host right-hand side b_i = int f phi_i (Dirichlet rows 0) of a CG operator:
+ * kind 0 f = 3 (3 pi)^2 prod sin(3 pi x_d); kind 1 seeded Gaussian random
+ * field (random Fourier features, n_features, correlation length). */
+int mf_assemble_rhs(mf_op_t op, int32_t kind, uint64_t seed, int32_t n_features, double corr_length,
+                    double* b);
+/* Transfer between level l-1 and level l (MGTwoLevelTransfer::prolongate_and_add /
+ * restrict_and_add, application_ph.cc:1902-1907): direction 0 dst(level l) = P
+ * src(level l-1), 1 dst(level l-1) = R src(level l); level-precision vectors. */
+int mf_mg_transfer(mf_mg_t mg, int32_t level, int32_t direction, void* dst, const void* src, void* stream);
+/* support points (3*n doubles) and Dirichlet mask (n bytes) of level l's DoFs */
+int mf_mg_level_dofs(mf_mg_t mg, int32_t level, double* xyz, uint8_t* constrained);
+/* interpolant of prod sin(3 pi x_d) (kind-0 exact solution) */
+int mf_exact_solution(mf_op_t op, double* u);
+
 /* Runs the device algorithm's patch/colour schedule on the host, in the
  * operator's precision, from the same tables the kernels read (mode 0: vmult,
  * 1: diagonal).  Host vectors.  A self-check of the setup for machines without

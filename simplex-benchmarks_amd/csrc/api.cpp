@@ -6,6 +6,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "mg.h"
 #include "op.h"
 
 namespace smf {
@@ -226,6 +227,18 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
     for (int32_t v : op.mesh.cells)
       if (v < 0 || v >= desc->n_vertices) throw std::runtime_error("cell references a missing vertex");
     op.mesh.kuhn_n = desc->kuhn_n;
+    if (desc->kuhn_n > 0) {
+      const int64_t n = desc->kuhn_n;
+      const int64_t n1 = n + 1;
+      if (desc->n_vertices != n1 * n1 * n1 || desc->n_cells != 6 * n * n * n)
+        throw std::runtime_error("kuhn_n does not match the mesh sizes");
+      op.mesh.lattice.resize(3 * op.mesh.nv);
+      for (int64_t v = 0; v < op.mesh.nv; ++v) {
+        op.mesh.lattice[3 * v] = (int32_t)(v % n1);
+        op.mesh.lattice[3 * v + 1] = (int32_t)((v / n1) % n1);
+        op.mesh.lattice[3 * v + 2] = (int32_t)(v / (n1 * n1));
+      }
+    }
     if (op.is_dg()) {
       op.ncomp = kind == MF_DG_SIPG ? 1 : 3;
       build_dg_plan(op.mesh, degree, op.ncomp, op.opt, kind == MF_DG_HELMHOLTZ, op.dg);
@@ -371,6 +384,140 @@ int mf_get_constrained(mf_op_t h, uint8_t* mask) {
       const CGPlan& pl = op.cg;
       for (int64_t i = 0; i < pl.n_dofs; ++i) mask[i] = i >= pl.n_free ? 1 : 0;
     }
+  });
+}
+
+struct mf_multigrid {
+  std::unique_ptr<Multigrid> mg;
+};
+
+void mf_mg_default_options(mf_mg_options* opt) {
+  MGOptions d;
+  opt->smoother_degree = d.smoother_degree;
+  opt->smoothing_range = d.smoothing_range;
+  opt->eig_cg_iterations = d.eig_cg_iterations;
+  opt->level_precision = d.level_precision;
+  opt->max_coarse_dofs = d.max_coarse_dofs;
+}
+
+int mf_mg_create(mf_op_t fine, const mf_mg_options* opt, mf_mg_t* out) {
+  MF_TRY({
+    if (!fine || !out) throw std::runtime_error("null argument");
+    *out = nullptr;
+    require_device(fine->op);
+    MGOptions o;
+    if (opt) {
+      o.smoother_degree = opt->smoother_degree;
+      o.smoothing_range = opt->smoothing_range;
+      o.eig_cg_iterations = opt->eig_cg_iterations;
+      o.level_precision = opt->level_precision;
+      o.max_coarse_dofs = opt->max_coarse_dofs;
+    }
+    if (o.smoother_degree < 1 || o.smoothing_range <= 1 || o.eig_cg_iterations < 1)
+      throw std::runtime_error("invalid multigrid options");
+    if (o.level_precision != MF_F64 && o.level_precision != MF_F32) throw std::runtime_error("invalid level precision");
+    auto h = std::make_unique<mf_multigrid>();
+    h->mg = build_multigrid(fine->op, o);
+    *out = h.release();
+  });
+}
+
+int mf_mg_vmult(mf_mg_t mg, void* dst, const void* src, void* stream) {
+  MF_TRY({
+    if (!mg || !dst || !src) throw std::runtime_error("null argument");
+    if (dst == src) throw std::runtime_error("dst and src must not alias");
+    mg_vmult(*mg->mg, dst, src, static_cast<hipStream_t>(stream));
+  });
+}
+
+int mf_mg_get_info(mf_mg_t mg, mf_mg_info* info) {
+  MF_TRY({
+    if (!mg || !info) throw std::runtime_error("null argument");
+    std::memset(info, 0, sizeof(*info));
+    const auto& L = mg->mg->levels;
+    info->n_levels = (int32_t)std::min<size_t>(L.size(), MF_MAX_LEVELS);
+    for (int l = 0; l < info->n_levels; ++l) {
+      info->degree[l] = L[l].degree;
+      info->kuhn_n[l] = L[l].kuhn_n;
+      info->n_dofs[l] = L[l].op->m();
+      info->lambda_max[l] = L[l].lambda_max;
+    }
+    info->vcycles = mg->mg->vcycles;
+  });
+}
+
+int mf_mg_transfer(mf_mg_t mg, int32_t level, int32_t direction, void* dst, const void* src, void* stream) {
+  MF_TRY({
+    if (!mg || !dst || !src) throw std::runtime_error("null argument");
+    auto& L = mg->mg->levels;
+    if (level < 1 || level >= (int)L.size()) throw std::runtime_error("level out of range");
+    if (direction != 0 && direction != 1) throw std::runtime_error("direction must be 0 or 1");
+    const Csr& A = direction == 0 ? L[level].P : L[level].R;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (mg->mg->opt.level_precision == MF_F64)
+      blas_csr<double>(A.nrows, A.d_ptr.as<int64_t>(), A.d_col.as<int32_t>(), A.d_val.as<double>(),
+                       static_cast<const double*>(src), static_cast<double*>(dst), false, s);
+    else
+      blas_csr<float>(A.nrows, A.d_ptr.as<int64_t>(), A.d_col.as<int32_t>(), A.d_val.as<float>(),
+                      static_cast<const float*>(src), static_cast<float*>(dst), false, s);
+  });
+}
+
+int mf_mg_level_dofs(mf_mg_t mg, int32_t level, double* xyz, uint8_t* constrained) {
+  MF_TRY({
+    if (!mg) throw std::runtime_error("null argument");
+    auto& L = mg->mg->levels;
+    if (level < 0 || level >= (int)L.size()) throw std::runtime_error("level out of range");
+    const CGPlan& pl = L[level].op->cg;
+    if (xyz) std::memcpy(xyz, pl.dof_points.data(), pl.dof_points.size() * sizeof(double));
+    if (constrained)
+      for (int64_t i = 0; i < pl.n_dofs; ++i) constrained[i] = i >= pl.n_free ? 1 : 0;
+  });
+}
+
+int mf_mg_destroy(mf_mg_t mg) {
+  MF_TRY({ delete mg; });
+}
+
+int mf_solve_cg(mf_op_t A, mf_mg_t mg, void* x, const void* b, double rel_tol, int32_t max_iter, int32_t flexible,
+                mf_solve_stats* stats, void* stream) {
+  MF_TRY({
+    if (!A || !x || !b) throw std::runtime_error("null argument");
+    require_device(A->op);
+    if (A->op.kind != MF_CG_POISSON) throw std::runtime_error("solver needs a CG Poisson operator");
+    if (mg && mg->mg->n_fine != A->op.m()) throw std::runtime_error("multigrid does not match the operator");
+    SolveStats st = solve_cg(A->op, mg ? mg->mg.get() : nullptr, static_cast<double*>(x),
+                             static_cast<const double*>(b), rel_tol, max_iter, flexible != 0,
+                             static_cast<hipStream_t>(stream));
+    if (stats) {
+      stats->iterations = st.iterations;
+      stats->converged = st.converged;
+      stats->initial_residual = st.r0;
+      stats->final_residual = st.rn;
+      stats->rhs_norm = st.b_norm;
+    }
+  });
+}
+
+int mf_assemble_rhs(mf_op_t op, int32_t kind, uint64_t seed, int32_t n_features, double corr_length, double* b) {
+  MF_TRY({
+    if (!op || !b) throw std::runtime_error("null argument");
+    require_device(op->op);
+    if (kind != 0 && kind != 1) throw std::runtime_error("rhs kind must be 0 or 1");
+    if (kind == 1 && (n_features < 1 || corr_length <= 0)) throw std::runtime_error("invalid random field parameters");
+    std::vector<double> v;
+    assemble_rhs(op->op, kind, seed, n_features, corr_length, v);
+    std::memcpy(b, v.data(), v.size() * sizeof(double));
+  });
+}
+
+int mf_exact_solution(mf_op_t op, double* u) {
+  MF_TRY({
+    if (!op || !u) throw std::runtime_error("null argument");
+    if (op->op.kind != MF_CG_POISSON) throw std::runtime_error("CG operators only");
+    std::vector<double> v;
+    exact_solution(op->op, v);
+    std::memcpy(u, v.data(), v.size() * sizeof(double));
   });
 }
 

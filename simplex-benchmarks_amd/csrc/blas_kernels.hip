@@ -1,0 +1,178 @@
+// Vector, transfer and reduction kernels of the multigrid solver (gfx950).
+//
+// Reductions are deterministic: a fixed grid of 256 blocks writes one FP64
+// partial sum per block (block-local tree in LDS), and a single-block kernel
+// adds the partials in a fixed order -- two runs give bit-identical dots.
+#include <hip/hip_runtime.h>
+
+#include "blas.h"
+
+namespace smf {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRedBlocks = 256;
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <typename T>
+__global__ void axpby_kernel(int64_t n, double a, const T* __restrict__ x, double b, T* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = T(a * (double)x[i] + b * (double)y[i]);
+}
+
+// y = a*x + b*y computed in T (keeps FP32 arithmetic in FP32)
+template <typename T>
+__global__ void axpby_t_kernel(int64_t n, T a, const T* __restrict__ x, T b, T* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+template <typename TO, typename TI>
+__global__ void convert_kernel(int64_t n, const TI* __restrict__ x, TO* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = TO(x[i]);
+}
+
+template <typename T>
+__global__ void scale_kernel(int64_t n, const T* __restrict__ d, const T* __restrict__ x, T* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = d[i] * x[i];
+}
+
+template <typename T>
+__global__ void fill_kernel(int64_t n, T v, T* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
+// partial[b] = sum over block b's grid-stride range of x*y (FP64)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dot_partial_kernel(int64_t n, const T* __restrict__ x,
+                                                             const T* __restrict__ y,
+                                                             double* __restrict__ partial) {
+  __shared__ double s[kBlock];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    acc += (double)x[i] * (double)y[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(kBlock) void reduce_final_kernel(int nparts, const double* __restrict__ partial,
+                                                              double* __restrict__ out) {
+  __shared__ double s[kBlock];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kBlock) acc += partial[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+// y = A x for a CSR matrix (one thread per row, fixed summation order)
+template <typename T>
+__global__ void csr_kernel(int64_t nrows, const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                           const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y, int add) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  T s = T(0);
+  for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k) s += val[k] * x[col[k]];
+  y[r] = add ? y[r] + s : s;
+}
+
+// Chebyshev step (three-term recurrence on the Jacobi-preconditioned
+// residual): d = c1*d + c2*Dinv*(b - Ax); x += d
+template <typename T>
+__global__ void cheby_step_kernel(int64_t n, T c1, T c2, const T* __restrict__ dinv, const T* __restrict__ b,
+                                  const T* __restrict__ ax, T* __restrict__ d, T* __restrict__ x, int first) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T r = b[i] - ax[i];
+  const T di = first ? c2 * dinv[i] * r : c1 * d[i] + c2 * dinv[i] * r;
+  d[i] = di;
+  x[i] += di;
+}
+
+// y = M x for a dense row-major n x n matrix (FP64)
+__global__ void dense_gemv_kernel(int n, const double* __restrict__ M, const double* __restrict__ x,
+                                  double* __restrict__ y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int c = 0; c < n; ++c) s += M[(int64_t)r * n + c] * x[c];
+  y[r] = s;
+}
+
+}  // namespace
+
+template <typename T>
+void blas_axpby(int64_t n, double a, const T* x, double b, T* y, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(axpby_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, x, b, y);
+}
+template <typename T>
+void blas_axpby_t(int64_t n, T a, const T* x, T b, T* y, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(axpby_t_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, x, b, y);
+}
+template <typename TO, typename TI>
+void blas_convert(int64_t n, const TI* x, TO* y, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL((convert_kernel<TO, TI>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, x, y);
+}
+template <typename T>
+void blas_scale(int64_t n, const T* d, const T* x, T* y, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(scale_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, d, x, y);
+}
+template <typename T>
+void blas_fill(int64_t n, T v, T* y, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, v, y);
+}
+template <typename T>
+void blas_dot(int64_t n, const T* x, const T* y, double* partial, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(dot_partial_kernel<T>, dim3(kRedBlocks), dim3(kBlock), 0, s, n, x, y, partial);
+  hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kBlock), 0, s, kRedBlocks, partial, out);
+}
+template <typename T>
+void blas_csr(int64_t nrows, const int64_t* ptr, const int32_t* col, const T* val, const T* x, T* y, bool add,
+              hipStream_t s) {
+  if (nrows > 0)
+    hipLaunchKernelGGL(csr_kernel<T>, dim3(grid_for(nrows)), dim3(kBlock), 0, s, nrows, ptr, col, val, x, y,
+                       add ? 1 : 0);
+}
+template <typename T>
+void blas_cheby_step(int64_t n, T c1, T c2, const T* dinv, const T* b, const T* ax, T* d, T* x, bool first,
+                     hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(cheby_step_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, c1, c2, dinv, b, ax, d, x,
+                       first ? 1 : 0);
+}
+void blas_dense_gemv(int n, const double* M, const double* x, double* y, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(dense_gemv_kernel, dim3((n + 63) / 64), dim3(64), 0, s, n, M, x, y);
+}
+int blas_reduction_parts() { return kRedBlocks; }
+
+#define SMF_INST(T)                                                                                   \
+  template void blas_axpby<T>(int64_t, double, const T*, double, T*, hipStream_t);                 \
+  template void blas_axpby_t<T>(int64_t, T, const T*, T, T*, hipStream_t);                        \
+  template void blas_scale<T>(int64_t, const T*, const T*, T*, hipStream_t);                       \
+  template void blas_fill<T>(int64_t, T, T*, hipStream_t);                                         \
+  template void blas_dot<T>(int64_t, const T*, const T*, double*, double*, hipStream_t);           \
+  template void blas_csr<T>(int64_t, const int64_t*, const int32_t*, const T*, const T*, T*, bool, \
+                            hipStream_t);                                                          \
+  template void blas_cheby_step<T>(int64_t, T, T, const T*, const T*, const T*, T*, T*, bool, hipStream_t);
+SMF_INST(double)
+SMF_INST(float)
+template void blas_convert<float, double>(int64_t, const double*, float*, hipStream_t);
+template void blas_convert<double, float>(int64_t, const float*, double*, hipStream_t);
+template void blas_convert<double, double>(int64_t, const double*, double*, hipStream_t);
+template void blas_convert<float, float>(int64_t, const float*, float*, hipStream_t);
+
+}  // namespace smf

@@ -1,0 +1,620 @@
+// Hybrid multigrid: level hierarchy, transfers, Chebyshev-Jacobi smoothing,
+// FP64 dense coarse solve, mixed-precision V-cycle, (flexible) CG and the
+// right-hand side.  See mg.h for the reference counterparts.
+#include "mg.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace smf {
+
+void launch_invert_diagonal(int precision, int64_t n, void* d, hipStream_t s);
+void launch_rhs_cells(int64_t ncells, int nl, int nq, const double* geo12, const double* qtab, int kind, int nf,
+                      const double* feat, double* out, hipStream_t s);
+
+void Csr::upload(int precision) {
+  d_ptr.upload(ptr);
+  d_col.upload(col);
+  if (precision == MF_F64) {
+    d_val.upload(val);
+  } else {
+    std::vector<float> v(val.begin(), val.end());
+    d_val.upload(v);
+  }
+}
+
+namespace {
+
+const int kPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+// Kuhn topology of size n whose vertex (i,j,k) is the fine vertex (s i, s j, s k)
+Mesh coarse_kuhn(const Mesh& fine, int n) {
+  const int nf = fine.kuhn_n, s = nf / n;
+  Mesh m = kuhn_cube(n, 0.0, 0);
+  const int64_t n1 = n + 1, f1 = nf + 1;
+  for (int64_t k = 0; k < n1; ++k)
+    for (int64_t j = 0; j < n1; ++j)
+      for (int64_t i = 0; i < n1; ++i) {
+        const int64_t v = i + n1 * (j + n1 * k), fv = s * i + f1 * (s * j + f1 * s * k);
+        for (int d = 0; d < 3; ++d) m.x[3 * v + d] = fine.x[3 * fv + d];
+      }
+  return m;
+}
+
+std::unique_ptr<Operator> make_cg_operator(const Mesh& mesh, int degree, int precision, const mf_options& opt) {
+  auto op = std::make_unique<Operator>();
+  op->kind = MF_CG_POISSON;
+  op->precision = precision;
+  op->degree = degree;
+  op->opt = opt;
+  op->opt.host_only = 0;
+  op->mesh = mesh;
+  build_cg_plan(op->mesh, degree, op->opt, op->cg);
+  make_cg_shape(op->cg);
+  upload_cg(*op);
+  return op;
+}
+
+std::vector<int64_t> inverse_perm(const std::vector<int64_t>& perm) {
+  std::vector<int64_t> inv(perm.size());
+  for (size_t k = 0; k < perm.size(); ++k) inv[perm[k]] = (int64_t)k;
+  return inv;
+}
+
+// Solve J xhat = X - L0 for the 3x3 lattice Jacobian (columns L_b - L_0).
+void solve3(const double J[3][3], const double r[3], double x[3]) {
+  const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                     J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  for (int c = 0; c < 3; ++c) {
+    double M[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) M[a][b] = b == c ? r[a] : J[a][b];
+    x[c] = (M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+            M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0])) /
+           det;
+  }
+}
+
+// Prolongation from the coarse level to the fine level by reference-cell
+// interpolation on the unperturbed lattice: fine DoF value = coarse function
+// at the fine node's lattice position (MGTwoLevelTransfer semantics,
+// application_ph.cc:1902-1907).  Dirichlet rows and columns are dropped.
+Csr build_prolongation(const Operator& fine, const Operator& coarse, bool same_mesh) {
+  const CGPlan& fp = fine.cg;
+  const CGPlan& cp = coarse.cg;
+  const Lagrange ffe(fp.p), cfe(cp.p);
+  const std::vector<int64_t> cinv = inverse_perm(cp.cell_perm);
+  const Mesh& fm = fine.mesh;
+  const Mesh& cm = coarse.mesh;
+  std::vector<std::vector<std::pair<int32_t, double>>> rows(fp.n_dofs);
+  std::vector<char> done(fp.n_dofs, 0);
+  std::vector<double> cval(cp.n_loc);
+  for (int64_t k = 0; k < fp.n_cells; ++k) {
+    const int64_t c = fp.cell_perm[k];
+    for (int i = 0; i < fp.n_loc; ++i) {
+      const int64_t d = fp.cell_dofs[k * fp.n_loc + i];
+      if (d >= fp.n_free || done[d]) continue;
+      done[d] = 1;
+      const auto xr = ffe.node_point(i);
+      int64_t cc;
+      double xhat[3];
+      if (same_mesh) {
+        cc = c;
+        for (int a = 0; a < 3; ++a) xhat[a] = xr[a];
+      } else {
+        // lattice position of the fine node, in coarse lattice units
+        double X[3];
+        const int32_t* v = &fm.cells[4 * c];
+        for (int a = 0; a < 3; ++a) {
+          const double L0 = fm.lattice[3 * v[0] + a];
+          double s = L0;
+          for (int b = 0; b < 3; ++b) s += (fm.lattice[3 * v[b + 1] + a] - L0) * xr[b];
+          X[a] = 0.5 * s;
+        }
+        const int nc = cm.kuhn_n;
+        int h[3];
+        double t[3];
+        for (int a = 0; a < 3; ++a) {
+          h[a] = std::min(std::max((int)std::floor(X[a]), 0), nc - 1);
+          t[a] = X[a] - h[a];
+        }
+        int ord[3] = {0, 1, 2};
+        std::sort(ord, ord + 3, [&](int a, int b) { return t[a] > t[b] || (t[a] == t[b] && a < b); });
+        int s = 0;
+        for (; s < 6; ++s)
+          if (kPerm[s][0] == ord[0] && kPerm[s][1] == ord[1] && kPerm[s][2] == ord[2]) break;
+        cc = 6ll * (h[0] + (int64_t)nc * (h[1] + (int64_t)nc * h[2])) + s;
+        const int32_t* w = &cm.cells[4 * cc];
+        double J[3][3], r[3];
+        for (int a = 0; a < 3; ++a) {
+          const double L0 = cm.lattice[3 * w[0] + a];
+          for (int b = 0; b < 3; ++b) J[a][b] = cm.lattice[3 * w[b + 1] + a] - L0;
+          r[a] = X[a] - L0;
+        }
+        solve3(J, r, xhat);
+      }
+      cfe.eval(xhat, cval.data(), nullptr);
+      const int64_t kc = cinv[cc];
+      for (int j = 0; j < cp.n_loc; ++j) {
+        const int64_t dc = cp.cell_dofs[kc * cp.n_loc + j];
+        if (dc >= cp.n_free || std::fabs(cval[j]) < 1e-13) continue;
+        rows[d].push_back({(int32_t)dc, cval[j]});
+      }
+    }
+  }
+  Csr P;
+  P.nrows = fp.n_dofs;
+  P.ncols = cp.n_dofs;
+  P.ptr.assign(1, 0);
+  for (int64_t d = 0; d < fp.n_dofs; ++d) {
+    std::sort(rows[d].begin(), rows[d].end());
+    for (auto& e : rows[d]) {
+      P.col.push_back(e.first);
+      P.val.push_back(e.second);
+    }
+    P.ptr.push_back((int64_t)P.col.size());
+  }
+  return P;
+}
+
+Csr transpose(const Csr& A) {
+  Csr T;
+  T.nrows = A.ncols;
+  T.ncols = A.nrows;
+  T.ptr.assign(T.nrows + 1, 0);
+  for (int32_t c : A.col) T.ptr[c + 1]++;
+  for (int64_t r = 0; r < T.nrows; ++r) T.ptr[r + 1] += T.ptr[r];
+  T.col.resize(A.col.size());
+  T.val.resize(A.val.size());
+  std::vector<int64_t> pos(T.ptr.begin(), T.ptr.end() - 1);
+  for (int64_t r = 0; r < A.nrows; ++r)
+    for (int64_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
+      const int64_t p = pos[A.col[k]]++;
+      T.col[p] = (int32_t)r;
+      T.val[p] = A.val[k];
+    }
+  return T;
+}
+
+// dense FP64 matrix of a CG operator from its element matrices (identity on
+// Dirichlet rows), then its inverse by Cholesky
+std::vector<double> dense_inverse(const Operator& op) {
+  const CGPlan& pl = op.cg;
+  const int64_t n = pl.n_dofs;
+  std::vector<double> A(n * n, 0.0);
+  const int nl = pl.n_loc, nq = pl.n_q;
+  const double* G = pl.shape.data();
+  const double* W = G + nl * nq * 3;
+  for (int64_t k = 0; k < pl.n_cells; ++k) {
+    double m[6];
+    for (int t = 0; t < 6; ++t) m[t] = pl.metric[(size_t)t * pl.n_cells + k];
+    for (int i = 0; i < nl; ++i) {
+      const int64_t di = pl.cell_dofs[k * nl + i];
+      if (di >= pl.n_free) continue;
+      for (int j = 0; j < nl; ++j) {
+        const int64_t dj = pl.cell_dofs[k * nl + j];
+        if (dj >= pl.n_free) continue;
+        double s = 0;
+        for (int q = 0; q < nq; ++q) {
+          const double* a = &G[(q * nl + i) * 3];
+          const double* b = &G[(q * nl + j) * 3];
+          s += W[q] * (a[0] * (m[0] * b[0] + m[1] * b[1] + m[2] * b[2]) +
+                       a[1] * (m[1] * b[0] + m[3] * b[1] + m[4] * b[2]) +
+                       a[2] * (m[2] * b[0] + m[4] * b[1] + m[5] * b[2]));
+        }
+        A[di * n + dj] += s;
+      }
+    }
+  }
+  for (int64_t i = pl.n_free; i < n; ++i) A[i * n + i] = 1.0;
+  // Cholesky A = L L^T (in place, lower)
+  for (int64_t j = 0; j < n; ++j) {
+    double s = A[j * n + j];
+    for (int64_t k = 0; k < j; ++k) s -= A[j * n + k] * A[j * n + k];
+    if (!(s > 0)) throw std::runtime_error("coarse matrix not positive definite");
+    const double l = std::sqrt(s);
+    A[j * n + j] = l;
+    for (int64_t i = j + 1; i < n; ++i) {
+      double t = A[i * n + j];
+      for (int64_t k = 0; k < j; ++k) t -= A[i * n + k] * A[j * n + k];
+      A[i * n + j] = t / l;
+    }
+  }
+  // inverse: solve L L^T X = I column by column
+  std::vector<double> inv(n * n);
+  std::vector<double> y(n);
+  for (int64_t c = 0; c < n; ++c) {
+    for (int64_t i = 0; i < n; ++i) {
+      double t = (i == c) ? 1.0 : 0.0;
+      for (int64_t k = 0; k < i; ++k) t -= A[i * n + k] * y[k];
+      y[i] = t / A[i * n + i];
+    }
+    for (int64_t i = n - 1; i >= 0; --i) {
+      double t = y[i];
+      for (int64_t k = i + 1; k < n; ++k) t -= A[k * n + i] * inv[k * n + c];
+      inv[i * n + c] = t / A[i * n + i];
+    }
+  }
+  return inv;
+}
+
+// largest eigenvalue of a symmetric tridiagonal matrix by Sturm bisection
+double tridiag_max_eig(const std::vector<double>& a, const std::vector<double>& b) {
+  const int n = (int)a.size();
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < n; ++i) {
+    const double r = (i > 0 ? std::fabs(b[i - 1]) : 0) + (i + 1 < n ? std::fabs(b[i]) : 0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  auto count_below = [&](double x) {
+    int c = 0;
+    double q = a[0] - x;
+    if (q < 0) ++c;
+    for (int i = 1; i < n; ++i) {
+      q = a[i] - x - b[i - 1] * b[i - 1] / (q != 0 ? q : 1e-300);
+      if (q < 0) ++c;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (count_below(mid) >= n) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
+
+template <typename T>
+T* P_(DevBuf& b) {
+  return b.as<T>();
+}
+
+template <typename T>
+void level_vmult(const MGLevel& L, T* dst, const T* src, hipStream_t s) {
+  launch_cg(*L.op, 0, dst, src, s);
+}
+
+template <typename T>
+double device_dot(Multigrid& mg, int64_t n, const T* x, const T* y, hipStream_t s) {
+  blas_dot<T>(n, x, y, mg.d_partial.as<double>(), mg.d_scalar.as<double>(), s);
+  double h;
+  SMF_HIP(hipMemcpyAsync(&h, mg.d_scalar.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  SMF_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+// lambda_max of D^-1 A by CG iterations (Lanczos), as PreconditionChebyshev's
+// eigenvalue estimate
+template <typename T>
+double estimate_lambda_max(Multigrid& mg, MGLevel& L, int iters, hipStream_t s) {
+  const int64_t n = L.op->m();
+  T* x = L.x.as<T>();
+  T* r = L.r.as<T>();
+  T* z = L.t.as<T>();
+  T* p = L.d.as<T>();
+  T* q = L.b.as<T>();
+  // r = 1 on free rows (constrained rows are identity rows of eigenvalue 1)
+  blas_fill<T>(n, T(1), r, s);
+  blas_fill<T>(n, T(0), x, s);
+  blas_scale<T>(n, L.inv_diag.as<T>(), r, z, s);
+  SMF_HIP(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+  double rz = device_dot<T>(mg, n, r, z, s);
+  std::vector<double> alpha, beta;
+  for (int it = 0; it < iters && rz > 0; ++it) {
+    level_vmult<T>(L, q, p, s);
+    const double pq = device_dot<T>(mg, n, p, q, s);
+    const double a = rz / pq;
+    blas_axpby<T>(n, -a, q, 1.0, r, s);
+    blas_scale<T>(n, L.inv_diag.as<T>(), r, z, s);
+    const double rz_new = device_dot<T>(mg, n, r, z, s);
+    const double b = rz_new / rz;
+    alpha.push_back(a);
+    beta.push_back(b);
+    blas_axpby<T>(n, 1.0, z, b, p, s);
+    rz = rz_new;
+  }
+  const int m = (int)alpha.size();
+  if (m == 0) return 1.0;
+  std::vector<double> da(m), ob(std::max(m - 1, 0));
+  for (int k = 0; k < m; ++k) {
+    da[k] = 1.0 / alpha[k] + (k > 0 ? beta[k - 1] / alpha[k - 1] : 0.0);
+    if (k + 1 < m) ob[k] = std::sqrt(beta[k]) / alpha[k];
+  }
+  return tridiag_max_eig(da, ob);
+}
+
+// Chebyshev smoothing of A x = b on level L (Saad, Iterative Methods, Alg.
+// 12.1, on the Jacobi-preconditioned system), `degree` steps; zero_start
+// skips the first residual product.
+template <typename T>
+void chebyshev(Multigrid& mg, MGLevel& L, T* x, const T* b, bool zero_start, hipStream_t s) {
+  const int64_t n = L.op->m();
+  const double theta = 0.5 * (L.lambda_max + L.lambda_min), delta = 0.5 * (L.lambda_max - L.lambda_min);
+  const double sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  T* ax = L.t.as<T>();
+  T* d = L.d.as<T>();
+  const T* dinv = L.inv_diag.as<T>();
+  for (int k = 0; k < mg.opt.smoother_degree; ++k) {
+    if (k == 0) {
+      if (zero_start) {
+        blas_fill<T>(n, T(0), ax, s);
+        blas_fill<T>(n, T(0), x, s);
+      } else {
+        level_vmult<T>(L, ax, x, s);
+      }
+      blas_cheby_step<T>(n, T(0), T(1.0 / theta), dinv, b, ax, d, x, true, s);
+    } else {
+      const double rho_new = 1.0 / (2.0 * sigma - rho);
+      level_vmult<T>(L, ax, x, s);
+      blas_cheby_step<T>(n, T(rho_new * rho), T(2.0 * rho_new / delta), dinv, b, ax, d, x, false, s);
+      rho = rho_new;
+    }
+  }
+}
+
+template <typename T>
+void vcycle(Multigrid& mg, int l, hipStream_t s) {
+  MGLevel& L = mg.levels[l];
+  const int64_t n = L.op->m();
+  T* x = L.x.as<T>();
+  T* b = L.b.as<T>();
+  if (l == 0) {
+    // FP64 dense coarse solve
+    blas_convert<double, T>(n, b, mg.d_cb.as<double>(), s);
+    blas_dense_gemv(mg.n_coarse, mg.d_coarse_inv.as<double>(), mg.d_cb.as<double>(), mg.d_cx.as<double>(), s);
+    blas_convert<T, double>(n, mg.d_cx.as<double>(), x, s);
+    return;
+  }
+  MGLevel& C = mg.levels[l - 1];
+  chebyshev<T>(mg, L, x, b, true, s);
+  // r = b - A x ; b_c = R r
+  T* r = L.r.as<T>();
+  level_vmult<T>(L, r, x, s);
+  blas_axpby_t<T>(n, T(1), b, T(-1), r, s);
+  blas_csr<T>(L.R.nrows, L.R.d_ptr.as<int64_t>(), L.R.d_col.as<int32_t>(), L.R.d_val.as<T>(), r, C.b.as<T>(), false,
+              s);
+  vcycle<T>(mg, l - 1, s);
+  // x += P x_c
+  blas_csr<T>(L.P.nrows, L.P.d_ptr.as<int64_t>(), L.P.d_col.as<int32_t>(), L.P.d_val.as<T>(), C.x.as<T>(), x, true,
+              s);
+  chebyshev<T>(mg, L, x, b, false, s);
+}
+
+}  // namespace
+
+std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions& opt) {
+  if (fine.kind != MF_CG_POISSON) throw std::runtime_error("multigrid needs a CG Poisson fine operator");
+  auto mg = std::make_unique<Multigrid>();
+  mg->opt = opt;
+  mg->fine_precision = fine.precision;
+  mg->n_fine = fine.m();
+  const int prec = opt.level_precision;
+  mf_options lopt = fine.opt;
+  // levels, finest first: P_k..P_1 on the fine mesh, then P1 on coarser Kuhn meshes
+  std::vector<std::unique_ptr<Operator>> ops;
+  std::vector<int> kn;
+  ops.push_back(make_cg_operator(fine.mesh, fine.degree, prec, lopt));
+  kn.push_back(fine.mesh.kuhn_n);
+  if (ops[0]->cg.cell_dofs != fine.cg.cell_dofs) throw std::runtime_error("level numbering differs from the fine operator");
+  for (int p = fine.degree - 1; p >= 1; --p) {
+    ops.push_back(make_cg_operator(fine.mesh, p, prec, lopt));
+    kn.push_back(fine.mesh.kuhn_n);
+  }
+  int n = fine.mesh.kuhn_n;
+  if (n > 0) {
+    while (n % 2 == 0 && n / 2 >= 2 && ops.back()->cg.n_free > 64) {
+      n /= 2;
+      ops.push_back(make_cg_operator(coarse_kuhn(fine.mesh, n), 1, prec, lopt));
+      kn.push_back(n);
+    }
+  }
+  if (ops.back()->m() > opt.max_coarse_dofs)
+    throw std::runtime_error("coarsest level too large for the dense coarse solver (needs a Kuhn mesh hierarchy)");
+  // assemble levels coarsest first
+  const size_t nl = ops.size();
+  mg->levels.resize(nl);
+  for (size_t i = 0; i < nl; ++i) {
+    MGLevel& L = mg->levels[nl - 1 - i];
+    L.op = std::move(ops[i]);
+    L.degree = L.op->degree;
+    L.kuhn_n = kn[i];
+  }
+  const size_t w = prec == MF_F64 ? 8 : 4;
+  mg->d_partial.alloc(blas_reduction_parts() * sizeof(double));
+  mg->d_scalar.alloc(sizeof(double));
+  hipStream_t s = nullptr;
+  for (size_t l = 0; l < nl; ++l) {
+    MGLevel& L = mg->levels[l];
+    const int64_t m = L.op->m();
+    for (DevBuf* b : {&L.inv_diag, &L.x, &L.b, &L.r, &L.t, &L.d}) b->alloc(m * w);
+    launch_cg(*L.op, 1, L.inv_diag.p, nullptr, s);
+    launch_invert_diagonal(prec, m, L.inv_diag.p, s);
+    if (l > 0) {
+      const bool same = L.op->mesh.nc == mg->levels[l - 1].op->mesh.nc && L.kuhn_n == mg->levels[l - 1].kuhn_n;
+      L.P = build_prolongation(*L.op, *mg->levels[l - 1].op, same);
+      L.R = transpose(L.P);
+      L.P.upload(prec);
+      L.R.upload(prec);
+    }
+  }
+  SMF_HIP(hipDeviceSynchronize());
+  // smoother intervals: [1.2 lmax / range, 1.2 lmax]
+  for (size_t l = 1; l < nl; ++l) {
+    MGLevel& L = mg->levels[l];
+    const double lmax = prec == MF_F64 ? estimate_lambda_max<double>(*mg, L, opt.eig_cg_iterations, s)
+                                       : estimate_lambda_max<float>(*mg, L, opt.eig_cg_iterations, s);
+    L.lambda_max = 1.2 * lmax;
+    L.lambda_min = L.lambda_max / opt.smoothing_range;
+  }
+  // coarse solver
+  const Operator& c0 = *mg->levels[0].op;
+  std::vector<double> inv;
+  {
+    // the dense inverse needs the FP64 element data of the coarsest level
+    inv = dense_inverse(c0);
+  }
+  mg->n_coarse = (int)c0.m();
+  mg->d_coarse_inv.upload(inv);
+  mg->d_cb.alloc(mg->n_coarse * sizeof(double));
+  mg->d_cx.alloc(mg->n_coarse * sizeof(double));
+  SMF_HIP(hipDeviceSynchronize());
+  return mg;
+}
+
+void mg_vmult(Multigrid& mg, void* z, const void* r, hipStream_t s) {
+  MGLevel& top = mg.levels.back();
+  const int64_t n = top.op->m();
+  const bool f32 = mg.opt.level_precision == MF_F32;
+  // copy_to_mg
+  if (mg.fine_precision == MF_F64) {
+    if (f32) blas_convert<float, double>(n, static_cast<const double*>(r), top.b.as<float>(), s);
+    else SMF_HIP(hipMemcpyAsync(top.b.p, r, n * 8, hipMemcpyDeviceToDevice, s));
+  } else {
+    if (f32) SMF_HIP(hipMemcpyAsync(top.b.p, r, n * 4, hipMemcpyDeviceToDevice, s));
+    else blas_convert<double, float>(n, static_cast<const float*>(r), top.b.as<double>(), s);
+  }
+  const int ltop = (int)mg.levels.size() - 1;
+  if (f32) vcycle<float>(mg, ltop, s);
+  else vcycle<double>(mg, ltop, s);
+  // copy_from_mg
+  if (mg.fine_precision == MF_F64) {
+    if (f32) blas_convert<double, float>(n, top.x.as<float>(), static_cast<double*>(z), s);
+    else SMF_HIP(hipMemcpyAsync(z, top.x.p, n * 8, hipMemcpyDeviceToDevice, s));
+  } else {
+    if (f32) SMF_HIP(hipMemcpyAsync(z, top.x.p, n * 4, hipMemcpyDeviceToDevice, s));
+    else blas_convert<float, double>(n, top.x.as<double>(), static_cast<float*>(z), s);
+  }
+  mg.vcycles++;
+}
+
+SolveStats solve_cg(const Operator& A, Multigrid* mg, double* x, const double* b, double rel_tol, int max_iter,
+                    bool flexible, hipStream_t s) {
+  if (A.precision != MF_F64) throw std::runtime_error("the outer solver runs in FP64");
+  const int64_t n = A.m();
+  DevBuf r, z, p, q, zold, partial, scalar;
+  for (DevBuf* v : {&r, &z, &p, &q}) v->alloc(n * 8);
+  partial.alloc(blas_reduction_parts() * sizeof(double));
+  scalar.alloc(sizeof(double));
+  auto dot = [&](const double* u, const double* v) {
+    blas_dot<double>(n, u, v, partial.as<double>(), scalar.as<double>(), s);
+    double h;
+    SMF_HIP(hipMemcpyAsync(&h, scalar.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    SMF_HIP(hipStreamSynchronize(s));
+    return h;
+  };
+  auto precond = [&](double* zz, const double* rr) {
+    if (mg) mg_vmult(*mg, zz, rr, s);
+    else SMF_HIP(hipMemcpyAsync(zz, rr, n * 8, hipMemcpyDeviceToDevice, s));
+  };
+  SolveStats st;
+  (void)zold;
+  blas_fill<double>(n, 0.0, x, s);
+  SMF_HIP(hipMemcpyAsync(r.p, b, n * 8, hipMemcpyDeviceToDevice, s));
+  st.b_norm = std::sqrt(dot(b, b));
+  st.r0 = st.b_norm;
+  st.rn = st.r0;
+  const double tol = rel_tol * st.b_norm;
+  if (st.r0 <= tol) {
+    st.converged = 1;
+    return st;
+  }
+  precond(z.as<double>(), r.as<double>());
+  SMF_HIP(hipMemcpyAsync(p.p, z.p, n * 8, hipMemcpyDeviceToDevice, s));
+  double rz = dot(r.as<double>(), z.as<double>());
+  for (int it = 1; it <= max_iter; ++it) {
+    launch_cg(A, 0, q.p, p.p, s);
+    const double pq = dot(p.as<double>(), q.as<double>());
+    const double alpha = rz / pq;
+    blas_axpby<double>(n, alpha, p.as<double>(), 1.0, x, s);
+    blas_axpby<double>(n, -alpha, q.as<double>(), 1.0, r.as<double>(), s);
+    st.rn = std::sqrt(dot(r.as<double>(), r.as<double>()));
+    st.iterations = it;
+    if (st.rn <= tol || !std::isfinite(st.rn)) {
+      st.converged = st.rn <= tol;
+      break;
+    }
+    precond(z.as<double>(), r.as<double>());
+    const double rz_new = dot(r.as<double>(), z.as<double>());
+    // flexible CG (Notay 2000): beta = z_new.(r_new - r_old)/rz_old with
+    // r_new - r_old = -alpha q
+    double beta = rz_new / rz;
+    if (flexible) beta = -alpha * dot(z.as<double>(), q.as<double>()) / rz;
+    blas_axpby<double>(n, 1.0, z.as<double>(), beta, p.as<double>(), s);
+    rz = rz_new;
+  }
+  SMF_HIP(hipStreamSynchronize(s));
+  return st;
+}
+
+void assemble_rhs(const Operator& op, int kind, uint64_t seed, int n_features, double corr_length,
+                  std::vector<double>& b) {
+  if (op.kind != MF_CG_POISSON) throw std::runtime_error("right-hand side assembly is for CG operators");
+  const CGPlan& pl = op.cg;
+  const Mesh& m = op.mesh;
+  const int nl = pl.n_loc;
+  // per cell x0 and J (row-major), internal order
+  std::vector<double> geo(12 * pl.n_cells);
+  for (int64_t k = 0; k < pl.n_cells; ++k) {
+    const int32_t* v = &m.cells[4 * pl.cell_perm[k]];
+    for (int a = 0; a < 3; ++a) {
+      geo[12 * k + a] = m.x[3 * v[0] + a];
+      for (int c = 0; c < 3; ++c) geo[12 * k + 3 + 3 * a + c] = m.x[3 * v[c + 1] + a] - m.x[3 * v[0] + a];
+    }
+  }
+  const Quadrature q = tet_rule(6);
+  const Lagrange fe(pl.p);
+  std::vector<double> qtab(q.size() * (4 + nl)), val(nl);
+  for (int k = 0; k < q.size(); ++k) {
+    fe.eval(q.pts[k].data(), val.data(), nullptr);
+    for (int a = 0; a < 3; ++a) qtab[k * (4 + nl) + a] = q.pts[k][a];
+    qtab[k * (4 + nl) + 3] = q.w[k];
+    for (int i = 0; i < nl; ++i) qtab[k * (4 + nl) + 4 + i] = val[i];
+  }
+  std::vector<double> feat;
+  if (kind == 1) {
+    // omega ~ N(0, I / l^2) by Box-Muller from splitmix64, phase ~ U[0, 2 pi)
+    SplitMix64 rng(seed);
+    auto unif = [&]() { return ((rng.next() >> 11) + 0.5) * 0x1.0p-53; };
+    for (int f = 0; f < n_features; ++f) {
+      for (int a = 0; a < 3; ++a) {
+        const double u1 = unif(), u2 = unif();
+        feat.push_back(std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2) / corr_length);
+      }
+      feat.push_back(2.0 * M_PI * unif());
+    }
+  } else {
+    feat.assign(4, 0.0);
+  }
+  DevBuf dgeo, dq, dfeat, dout;
+  dgeo.upload(geo);
+  dq.upload(qtab);
+  dfeat.upload(feat);
+  dout.alloc(pl.n_cells * nl * sizeof(double));
+  launch_rhs_cells(pl.n_cells, nl, q.size(), dgeo.as<double>(), dq.as<double>(), kind, kind == 1 ? n_features : 0,
+                   dfeat.as<double>(), dout.as<double>(), nullptr);
+  std::vector<double> local(pl.n_cells * nl);
+  SMF_HIP(hipMemcpy(local.data(), dout.p, local.size() * sizeof(double), hipMemcpyDeviceToHost));
+  b.assign(pl.n_dofs, 0.0);
+  for (int64_t k = 0; k < pl.n_cells; ++k)
+    for (int i = 0; i < nl; ++i) {
+      const int64_t d = pl.cell_dofs[k * nl + i];
+      if (d < pl.n_free) b[d] += local[k * nl + i];
+    }
+}
+
+void exact_solution(const Operator& op, std::vector<double>& u) {
+  const CGPlan& pl = op.cg;
+  u.assign(pl.n_dofs, 0.0);
+  const double a = 3.0 * M_PI;
+  for (int64_t d = 0; d < pl.n_dofs; ++d) {
+    const double* X = &pl.dof_points[3 * d];
+    u[d] = d < pl.n_free ? std::sin(a * X[0]) * std::sin(a * X[1]) * std::sin(a * X[2]) : 0.0;
+  }
+}
+
+}  // namespace smf

@@ -1,0 +1,77 @@
+// Hybrid (p- then h-) multigrid preconditioner and Krylov solvers.
+//
+// Counterpart of MultigridPreconditioner (CG/cells_batched/application_ph.cc:
+// 1754-2026): level operators, MGTwoLevelTransfer per level pair,
+// PreconditionChebyshev<..., DiagonalMatrix> smoothers (degree 5, smoothing
+// range 20, 10 eigenvalue CG iterations), a coarse solver and SolverCG to
+// 1e-10 ||b||.  The coarse solver here is an FP64 dense inverse instead of one
+// Trilinos ML V-cycle (application_hp.cc:1684-1731).
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "blas.h"
+#include "op.h"
+
+namespace smf {
+
+struct Csr {
+  int64_t nrows = 0, ncols = 0;
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> col;
+  std::vector<double> val;
+  DevBuf d_ptr, d_col, d_val;
+  void upload(int precision);
+};
+
+struct MGLevel {
+  std::unique_ptr<Operator> op;  // level operator in the V-cycle precision
+  int degree = 1, kuhn_n = 0;
+  DevBuf inv_diag, x, b, r, t, d;  // level vectors
+  Csr P, R;                        // prolongation from / restriction to the next coarser level
+  double lambda_max = 0, lambda_min = 0;
+};
+
+struct MGOptions {
+  int smoother_degree = 5;
+  double smoothing_range = 20.0;
+  int eig_cg_iterations = 10;
+  int level_precision = MF_F32;
+  int max_coarse_dofs = 4096;
+};
+
+struct Multigrid {
+  MGOptions opt;
+  int fine_precision = MF_F64;
+  int64_t n_fine = 0;
+  std::vector<MGLevel> levels;  // [0] = coarsest
+  // coarse solver: dense FP64 inverse of the coarsest operator
+  int n_coarse = 0;
+  DevBuf d_coarse_inv, d_cb, d_cx;
+  DevBuf d_partial, d_scalar;  // reductions
+  int64_t vcycles = 0;
+};
+
+std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions& opt);
+// z = V-cycle(r); r, z in the fine operator's precision (device)
+void mg_vmult(Multigrid& mg, void* z, const void* r, hipStream_t s);
+
+struct SolveStats {
+  int iterations = 0;
+  double r0 = 0, rn = 0, b_norm = 0;
+  int converged = 0;
+};
+// (F)CG on the fine FP64 operator, x0 = 0 (the reference's SolverCG,
+// application_hp.cc:1950-1960); mg may be null (unpreconditioned)
+SolveStats solve_cg(const Operator& A, Multigrid* mg, double* x, const double* b, double rel_tol,
+                    int max_iter, bool flexible, hipStream_t s);
+
+// b_i = int f phi_i on a CG operator (Dirichlet rows 0): kind 0 f = 3(3pi)^2
+// prod sin(3 pi x_d) (exact solution prod sin(3 pi x_d)); kind 1 a seeded
+// Gaussian random field by random Fourier features (n_features, corr_length).
+void assemble_rhs(const Operator& op, int kind, uint64_t seed, int n_features, double corr_length,
+                  std::vector<double>& b);
+// interpolant of the kind-0 exact solution at the DoF support points
+void exact_solution(const Operator& op, std::vector<double>& u);
+
+}  // namespace smf

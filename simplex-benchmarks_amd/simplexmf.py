@@ -100,6 +100,16 @@ def lib():
         "mf_event_destroy": (C.c_int, [vp]),
         "mf_event_record": (C.c_int, [vp, vp]),
         "mf_event_elapsed_ms": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
+        "mf_mg_default_options": (None, [C.POINTER(MGOptions)]),
+        "mf_mg_create": (C.c_int, [vp, C.POINTER(MGOptions), C.POINTER(vp)]),
+        "mf_mg_vmult": (C.c_int, [vp, vp, vp, vp]),
+        "mf_mg_get_info": (C.c_int, [vp, C.POINTER(MGInfo)]),
+        "mf_mg_destroy": (C.c_int, [vp]),
+        "mf_mg_transfer": (C.c_int, [vp, i32, i32, vp, vp, vp]),
+        "mf_mg_level_dofs": (C.c_int, [vp, i32, vp, vp]),
+        "mf_solve_cg": (C.c_int, [vp, vp, vp, vp, C.c_double, i32, i32, C.POINTER(SolveStats), vp]),
+        "mf_assemble_rhs": (C.c_int, [vp, i32, u64, i32, C.c_double, vp]),
+        "mf_exact_solution": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -107,6 +117,30 @@ def lib():
         f.argtypes = args
     _lib = L
     return L
+
+
+class MGOptions(C.Structure):
+    _fields_ = [("smoother_degree", C.c_int32), ("smoothing_range", C.c_double),
+                ("eig_cg_iterations", C.c_int32), ("level_precision", C.c_int32),
+                ("max_coarse_dofs", C.c_int32)]
+
+
+MF_MAX_LEVELS = 16
+
+
+class MGInfo(C.Structure):
+    _fields_ = [("n_levels", C.c_int32), ("degree", C.c_int32 * MF_MAX_LEVELS),
+                ("kuhn_n", C.c_int32 * MF_MAX_LEVELS), ("n_dofs", C.c_int64 * MF_MAX_LEVELS),
+                ("lambda_max", C.c_double * MF_MAX_LEVELS), ("vcycles", C.c_int64)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32),
+                ("initial_residual", C.c_double), ("final_residual", C.c_double),
+                ("rhs_norm", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class MFError(RuntimeError):
@@ -132,6 +166,8 @@ _SIGNATURE_NAMES = [
     "mf_free", "mf_memcpy_htod", "mf_memcpy_dtoh", "mf_memcpy_dtod", "mf_memset",
     "mf_synchronize", "mf_stream_create", "mf_stream_destroy", "mf_stream_synchronize",
     "mf_event_create", "mf_event_destroy", "mf_event_record", "mf_event_elapsed_ms",
+    "mf_mg_default_options", "mf_mg_create", "mf_mg_vmult", "mf_mg_get_info", "mf_mg_destroy",
+    "mf_mg_transfer", "mf_mg_level_dofs", "mf_solve_cg", "mf_assemble_rhs", "mf_exact_solution",
 ]
 
 
@@ -369,3 +405,75 @@ class Operator:
         check(lib().mf_debug_host_apply(self.handle, mode, dst.ctypes.data,
                                          None if s is None else s.ctypes.data))
         return dst
+
+
+# --------------------------------------------------------------------------
+# multigrid and solvers
+
+class Multigrid:
+    """Hybrid p/h multigrid preconditioner below a CG operator
+    (MultigridPreconditioner of CG/cells_batched/application_ph.cc:1754-2026)."""
+
+    def __init__(self, fine: Operator, smoother_degree=5, smoothing_range=20.0, eig_cg_iterations=10,
+                 level_precision="f32", max_coarse_dofs=4096):
+        L = lib()
+        opt = MGOptions()
+        L.mf_mg_default_options(C.byref(opt))
+        opt.smoother_degree = smoother_degree
+        opt.smoothing_range = smoothing_range
+        opt.eig_cg_iterations = eig_cg_iterations
+        opt.level_precision = PRECISIONS[level_precision] if isinstance(level_precision, str) else level_precision
+        opt.max_coarse_dofs = max_coarse_dofs
+        self.level_dtype = NP_DTYPES[opt.level_precision]
+        self.fine = fine
+        h = C.c_void_p()
+        check(L.mf_mg_create(fine.handle, C.byref(opt), C.byref(h)))
+        self.handle = h.value
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.mf_mg_destroy(self.handle)
+            self.handle = None
+
+    def info(self):
+        inf = MGInfo()
+        check(lib().mf_mg_get_info(self.handle, C.byref(inf)))
+        n = inf.n_levels
+        return {"n_levels": n, "degree": list(inf.degree[:n]), "kuhn_n": list(inf.kuhn_n[:n]),
+                "n_dofs": list(inf.n_dofs[:n]), "lambda_max": list(inf.lambda_max[:n]),
+                "vcycles": inf.vcycles}
+
+    def vmult(self, dst, src, stream=None):
+        check(lib().mf_mg_vmult(self.handle, dst.ptr, src.ptr, stream.handle if stream else None))
+
+    def transfer(self, level, direction, dst, src, stream=None):
+        check(lib().mf_mg_transfer(self.handle, level, direction, dst.ptr, src.ptr,
+                                   stream.handle if stream else None))
+
+    def level_dofs(self, level):
+        n = self.info()["n_dofs"][level]
+        xyz = np.empty((n, 3))
+        con = np.empty(n, np.uint8)
+        check(lib().mf_mg_level_dofs(self.handle, level, xyz.ctypes.data, con.ctypes.data))
+        return xyz, con
+
+
+def solve_cg(A: Operator, x: DeviceVector, b: DeviceVector, mg: Multigrid | None = None, rel_tol=1e-10,
+             max_iter=1000, flexible=False, stream=None):
+    st = SolveStats()
+    check(lib().mf_solve_cg(A.handle, mg.handle if mg is not None else None, x.ptr, b.ptr, rel_tol,
+                            max_iter, 1 if flexible else 0, C.byref(st), stream.handle if stream else None))
+    return st.as_dict()
+
+
+def assemble_rhs(op: Operator, kind="sine", seed=0, n_features=256, corr_length=0.1):
+    b = np.empty(op.m())
+    k = {"sine": 0, "grf": 1}[kind] if isinstance(kind, str) else int(kind)
+    check(lib().mf_assemble_rhs(op.handle, k, seed, n_features, corr_length, b.ctypes.data))
+    return b
+
+
+def exact_solution(op: Operator):
+    u = np.empty(op.m())
+    check(lib().mf_exact_solution(op.handle, u.ctypes.data))
+    return u
