@@ -2,11 +2,13 @@
 """Throughput of the matrix-free tetrahedral vmult on one MI355X.
 
 A step is one vmult over the whole mesh with the input already resident in
-HBM.  Default workload: CG Poisson P2, FP64, unit cube 128^3 hexes x 6
-perturbed Kuhn tets (12,582,912 cells, 16,974,593 DoFs) -- the HBM-resident
-size of the reference's CG cells-batched operator
-(CG/cells_batched/application.cc:1236-1416; metric definition GDoF/s =
-n_dofs * steps / time, application.cc:1871-1889).
+HBM.  Default workload: DG SIPG Poisson P2, FP64, unit cube 64^3 hexes x 6
+perturbed Kuhn tets (1,572,864 cells, 15,728,640 DoFs), the reference's DG
+cells-batched operator (DG/cells_batched/application.cc:1318-1332; metric
+GDoF/s = n_dofs * steps / time, CG/cells_batched/application.cc:1871-1889).
+The other degrees and precisions of that config, the Helmholtz-like vector
+operator (32^3) and the CG P2 operator at 128^3 (the HBM-resident size of
+CG/cells_batched/application.cc:1236-1416) are reported under "variants".
 
 Prints one JSON line with the metric, a roofline object (algorithmic bytes of
 the reference data model / event-timed vmult duration vs 8 TB/s) and a CPU
@@ -38,9 +40,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=128, help="hexes per cube edge")
+    ap.add_argument("--n", type=int, default=64, help="hexes per cube edge")
     ap.add_argument("--degree", type=int, default=2)
-    ap.add_argument("--kind", default="cg")
+    ap.add_argument("--kind", default="dg")
+    ap.add_argument("--variants", type=int, default=1, help="0 to skip the other configs' lines")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="MiB copied device-to-device once before timing (scale for rocprof counters)")
     ap.add_argument("--precision", default="f64")
     ap.add_argument("--no-reorder", action="store_true")
     ap.add_argument("--patch-cells", type=int, default=0)
@@ -48,6 +53,8 @@ def parse():
     ap.add_argument("--cpu-baseline", type=int, default=1, help="0 to skip the CPU leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--mode", default="vmult", choices=["vmult", "solve"])
+    ap.add_argument("--solves", type=int, default=5, help="repeated solves in --mode solve")
     return ap.parse_args()
 
 
@@ -79,8 +86,117 @@ def cpu_baseline(op, x, seconds):
                       f"{(op.info['degree'] + 2) ** 3} points per cell"}
 
 
+def solve_mode(args):
+    """Mixed-precision hybrid-MG-preconditioned CG (FP64 outer CG, FP32
+    V-cycle) on the CG Poisson problem; metrics of application_hp.cc:1954-1988."""
+    import simplexmf as smf
+
+    t0 = time.time()
+    mesh = smf.Mesh.kuhn_cube(args.n, 0.2, args.seed)
+    op = smf.Operator(mesh, args.degree, kind="cg", precision="f64")
+    mg = smf.Multigrid(op)
+    b = smf.DeviceVector.from_numpy(smf.assemble_rhs(op, "grf", seed=args.seed, n_features=256,
+                                                     corr_length=0.1))
+    log(f"setup {time.time() - t0:.1f}s levels {mg.info()}")
+    x = op.initialize_dof_vector()
+    stream = smf.Stream()
+    st = smf.solve_cg(op, x, b, mg, rel_tol=1e-10, stream=stream)  # warm-up
+    times = []
+    for _ in range(args.solves):
+        smf.synchronize()
+        t = time.perf_counter()
+        st = smf.solve_cg(op, x, b, mg, rel_tol=1e-10, stream=stream)
+        smf.synchronize()
+        times.append(time.perf_counter() - t)
+    n = st["iterations"]
+    rho = (st["final_residual"] / st["initial_residual"]) ** (1.0 / n)
+    n10 = -10 * np.log(10) / np.log(rho)
+    tmin = min(times)
+    t10 = tmin * n10 / n
+    out = {
+        "metric": METRIC, "value": round(tmin, 6), "unit": "s to 1e-10 (min over solves)",
+        "n_gpus": 1, "steps": args.solves, "warmup": 1, "ms_per_step": round(tmin * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+        "data": "synthetic: seeded perturbed Kuhn mesh, Gaussian random field RHS (l=0.1)",
+        "config": {"workload": f"MG-CG Poisson P{args.degree}, {args.n}^3 x 6 perturbed Kuhn tets, "
+                               f"FP64 CG + FP32 V-cycle, stop 1e-10",
+                   "dofs": op.m(), "levels": mg.info()["n_dofs"]},
+        "iterations": n, "rho": rho, "n10": n10, "t10_s": t10, "E10_dofs_per_s": op.m() / t10,
+        "mean_s": float(np.mean(times)),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def measure(smf, kind, n, degree, precision, steps, warmup, seed, reorder=True, patch_cells=0,
+            accumulate=0, keep=False):
+    """Event-timed vmults of one operator on `stream`; returns the numbers of
+    one bench line (and the operator and input when keep=True)."""
+    t0 = time.time()
+    mesh = smf.Mesh.kuhn_cube(n, 0.2, seed)
+    op = smf.Operator(mesh, degree, kind=kind, precision=precision, reorder=reorder,
+                      patch_cells=patch_cells, accumulate=accumulate, mass_coefficient=1.0,
+                      laplace_coefficient=1.0)
+    info = op.info
+    log(f"setup {kind} p{degree} {precision} n{n}: {time.time() - t0:.1f}s")
+    x = np.random.default_rng(seed).uniform(-1, 1, op.m())
+    src = smf.DeviceVector.from_numpy(x.astype(op.dtype))
+    dst = op.initialize_dof_vector()
+    stream = smf.Stream()
+    for _ in range(warmup):
+        op.vmult(dst, src, stream)
+    stream.synchronize()
+    smf.synchronize()
+    ev0, ev1 = smf.Event(), smf.Event()
+    ev0.record(stream)
+    for _ in range(steps):
+        op.vmult(dst, src, stream)
+    ev1.record(stream)
+    stream.synchronize()
+    smf.synchronize()
+    t_step = ev0.elapsed_ms(ev1) / 1e3 / steps
+    res = {"kind": kind, "degree": degree, "precision": precision, "n": n, "cells": info["n_cells"],
+           "dofs": op.m(), "ms_per_step": t_step * 1e3, "gdofs": op.m() / t_step / 1e9,
+           "algorithmic_bytes": info["algorithmic_bytes"],
+           "achieved_gbs": info["algorithmic_bytes"] / t_step / 1e9, "flops": info["flops"],
+           "info": info}
+    res["frac"] = res["achieved_gbs"] / HBM_PEAK_GBS
+    if keep:
+        res["op"], res["x"] = op, x
+    return res
+
+
+def cpu_baseline_dg(smf, degree, precision_unused, seconds, seed):
+    """DG oracle (CPU restatement of the reference SIPG vmult) with OpenMP on
+    the host cores, on a bounded sample: the same operator on a 24^3 cube."""
+    import oracle_py
+
+    n = 24
+    op = smf.Operator(smf.Mesh.kuhn_cube(n, 0.2, seed), degree, kind="dg", host_only=True)
+    pr = oracle_py.DGOracleProblem.from_operator(op)
+    x = np.random.default_rng(seed).uniform(-1, 1, op.m())
+    pr.vmult_omp(x)
+    calls, t0 = 0, time.perf_counter()
+    while True:
+        pr.vmult_omp(x)
+        calls += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or calls >= 50:
+            break
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": op.m() * calls / el / 1e9, "unit": "GDoF/s", "cores": cores, "kind": "port",
+            "sample": f"{calls} vmults of the same DG SIPG P{degree} FP64 operator on a {n}^3 x 6 "
+                      f"perturbed Kuhn cube ({op.info['n_cells']} cells, {op.m()} DoFs), {cores} "
+                      f"host threads, {el:.1f} s; the oracle integrates with collapsed Gauss rules "
+                      f"of {(degree + 2) ** 3} (cell) and {(degree + 2) ** 2} (face) points"}
+
+
 def main():
     args = parse()
+    if args.mode == "solve":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this benchmark runs on exactly one GPU"}))
+            sys.exit(2)
+        return solve_mode(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; "
                                    "multi-GPU sharding is a design note only"}))
@@ -89,52 +205,50 @@ def main():
 
     if smf.device_count() < 1:
         raise SystemExit("no HIP device visible")
-    t0 = time.time()
-    mesh = smf.Mesh.kuhn_cube(args.n, 0.2, args.seed)
-    op = smf.Operator(mesh, args.degree, kind=args.kind, precision=args.precision,
-                      reorder=not args.no_reorder, patch_cells=args.patch_cells,
-                      accumulate=args.accumulate)
-    info = op.info
-    log(f"setup {time.time() - t0:.1f}s: {info}")
-    x = np.random.default_rng(args.seed).uniform(-1, 1, op.m())
-    src = smf.DeviceVector.from_numpy(x.astype(op.dtype))
-    dst = op.initialize_dof_vector()
-    stream = smf.Stream()
-    for _ in range(args.warmup):
-        op.vmult(dst, src, stream)
-    stream.synchronize()
-    smf.synchronize()
-    ev0, ev1 = smf.Event(), smf.Event()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        op.vmult(dst, src, stream)
-    ev1.record(stream)
-    stream.synchronize()
-    smf.synchronize()
-    ms = ev0.elapsed_ms(ev1)
-    t_step = ms / 1e3 / args.steps
-    gdofs = op.m() / t_step / 1e9
-    bytes_alg = info["algorithmic_bytes"]
-    achieved = bytes_alg / t_step / 1e9
+    if args.calibrate_copy > 0:
+        nb = args.calibrate_copy << 20
+        a, b = smf.DeviceVector(nb // 8), smf.DeviceVector(nb // 8)
+        a.zero()
+        smf.check(smf.lib().mf_memcpy_dtod(b.ptr, a.ptr, nb, None))
+        smf.synchronize()
+        del a, b
+    head = measure(smf, args.kind, args.n, args.degree, args.precision, args.steps, args.warmup,
+                   args.seed, reorder=not args.no_reorder, patch_cells=args.patch_cells,
+                   accumulate=args.accumulate, keep=args.cpu_baseline and args.kind == "cg")
+    info = head["info"]
     traffic = None
     prof = os.path.join(ROOT, "profiles", f"traffic_{args.kind}_p{args.degree}_{args.precision}_n{args.n}.json")
     if os.path.exists(prof):
         with open(prof) as f:
             traffic = json.load(f).get("bytes_per_vmult")
+    variants = []
+    if args.variants:
+        todo = [("dg", 64, 1, "f64"), ("dg", 64, 3, "f64"), ("dg", 64, 1, "f32"), ("dg", 64, 2, "f32"),
+                ("dg", 64, 3, "f32"), ("dg_helmholtz", 32, 2, "f64"), ("dg_helmholtz", 32, 2, "f32"),
+                ("cg", 128, 2, "f64"), ("cg", 128, 2, "f32")]
+        for kind, n, deg, prec in todo:
+            if (kind, n, deg, prec) == (args.kind, args.n, args.degree, args.precision):
+                continue
+            r = measure(smf, kind, n, deg, prec, max(10, args.steps // 2), 3, args.seed)
+            variants.append({k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()
+                             if k in ("kind", "degree", "precision", "n", "dofs", "ms_per_step", "gdofs",
+                                      "achieved_gbs", "frac")})
     base = None
-    if args.cpu_baseline and args.kind == "cg":
+    if args.cpu_baseline:
         log("cpu baseline ...")
-        base = cpu_baseline(op, x, args.cpu_seconds)
-    prec = {"f64": "f64", "f32": "f32"}[args.precision]
-    ncell = info["n_cells"]
+        if args.kind == "cg":
+            base = cpu_baseline(head["op"], head["x"], args.cpu_seconds)
+        elif args.kind == "dg":
+            base = cpu_baseline_dg(smf, args.degree, args.precision, args.cpu_seconds, args.seed)
+    prec = args.precision
     out = {
         "metric": METRIC,
-        "value": round(gdofs, 4),
+        "value": round(head["gdofs"], 4),
         "unit": "GDoF/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_step * 1e3, 5),
+        "ms_per_step": round(head["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -142,22 +256,22 @@ def main():
         "data": "synthetic: seeded perturbed Kuhn mesh, x ~ U[-1,1]",
         "config": {
             "workload": f"{WORKLOAD[args.kind]} P{args.degree} vmult, {args.n}^3 x 6 perturbed "
-                        f"Kuhn tets ({ncell} cells, {op.m()} DoFs), {prec.upper()}",
-            "kind": args.kind, "degree": args.degree, "n": args.n, "cells": ncell,
-            "dofs": op.m(), "reorder": not args.no_reorder, "parallelism": "single GPU",
-            "patches": info["n_patches"], "patch_colors": info["n_patch_colors"],
+                        f"Kuhn tets ({head['cells']} cells, {head['dofs']} DoFs), {prec.upper()}",
+            "kind": args.kind, "degree": args.degree, "n": args.n, "cells": head["cells"],
+            "dofs": head["dofs"], "reorder": not args.no_reorder, "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": round(achieved, 1),
+            "achieved": round(head["achieved_gbs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": round(head["frac"], 4),
             "traffic": traffic,
-            "algorithmic_bytes": bytes_alg,
+            "algorithmic_bytes": head["algorithmic_bytes"],
             "kernel": KERNEL[args.kind],
         },
         "cpu_baseline": base,
+        "variants": variants,
     }
     print(json.dumps(out), flush=True)
 

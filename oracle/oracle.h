@@ -64,6 +64,9 @@ int orc_cg_dense(const orc_mesh* m, int p, const int64_t* cell_dofs, const uint8
  * share geometry, component c of DoF d is row c*n_dofs + d.                */
 int orc_dg_vmult(const orc_mesh* m, int p, const int64_t* cell_dofs, int64_t n_dofs, int ncomp,
                  double mass, double laplace, const double* src, double* dst);
+/* same, OpenMP over cells and faces (face contributions by atomic adds; CPU baseline) */
+int orc_dg_vmult_omp(const orc_mesh* m, int p, const int64_t* cell_dofs, int64_t n_dofs, int ncomp,
+                     double mass, double laplace, const double* src, double* dst);
 int orc_dg_diagonal(const orc_mesh* m, int p, const int64_t* cell_dofs, int64_t n_dofs, int ncomp,
                     double mass, double laplace, double* diag);
 /* per-cell penalty tau_K (n_cells) */

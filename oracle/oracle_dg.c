@@ -158,7 +158,7 @@ static int tri_rule(int n, double (*bc)[3], double* w) {
 }
 
 typedef struct dgctx {
-  int p, nl, ncomp;
+  int p, nl, ncomp, parallel;
   double mass, laplace, factor;
   const orc_mesh* m;
   const int64_t* cd;
@@ -178,7 +178,8 @@ static int dg_apply(const dgctx* c, const double* src, double* dst, double* diag
   const int nq1 = p + 2;
   orc_tet_rule(nq1, tpts, tw);
   const int nq = nq1 * nq1 * nq1;
-  /* cells */
+  /* cells (each writes only its own rows) */
+#pragma omp parallel for schedule(dynamic, 64) if (c->parallel)
   for (int64_t k = 0; k < m->n_cells; ++k) {
     geo g;
     cell_geo(m, k, &g);
@@ -211,18 +212,29 @@ static int dg_apply(const dgctx* c, const double* src, double* dst, double* diag
       }
     }
   }
-  /* faces */
+  /* faces: group the sorted entries, then evaluate every face (atomic adds
+   * when run in parallel) */
   fent* f = build_faces(m);
   const int64_t nf = 4 * m->n_cells;
   double bc[MAXQF][3], fw[MAXQF];
   const int nqf = tri_rule(p + 2, bc, fw);
+  int64_t* starts = (int64_t*)malloc((nf + 1) * sizeof(int64_t));
+  int64_t ngroups = 0;
   for (int64_t k = 0; k < nf;) {
     int64_t j = k + 1;
     while (j < nf && f[j].v[0] == f[k].v[0] && f[j].v[1] == f[k].v[1] && f[j].v[2] == f[k].v[2]) ++j;
     if (j - k > 2) {
       free(f);
+      free(starts);
       return -1;
     }
+    starts[ngroups++] = k;
+    k = j;
+  }
+  starts[ngroups] = nf;
+#pragma omp parallel for schedule(dynamic, 64) if (c->parallel)
+  for (int64_t g = 0; g < ngroups; ++g) {
+    const int64_t k = starts[g], j = starts[g + 1];
     const int interior = (j - k == 2);
     const int64_t cm = f[k].cell, cp = interior ? f[k + 1].cell : -1;
     geo gm, gp;
@@ -274,34 +286,47 @@ static int dg_apply(const dgctx* c, const double* src, double* dst, double* diag
           const double jump = um - up, avg = 0.5 * (dum + dup);
           const double tv = (sigma * jump - avg) * jxw * c->laplace;
           const double tg = -0.5 * jump * jxw * c->laplace;
-          for (int i = 0; i < nl; ++i) dst[off + dm[i]] += tv * vm[i] + tg * dnm[i];
+          for (int i = 0; i < nl; ++i) {
+            const double v = tv * vm[i] + tg * dnm[i];
+#pragma omp atomic
+            dst[off + dm[i]] += v;
+          }
           if (interior)
-            for (int i = 0; i < nl; ++i) dst[off + dp[i]] += -tv * vp[i] + tg * dnp[i];
+            for (int i = 0; i < nl; ++i) {
+              const double v = -tv * vp[i] + tg * dnp[i];
+#pragma omp atomic
+              dst[off + dp[i]] += v;
+            }
         } else {
           /* diagonal: u = phi_i on one side only */
           for (int i = 0; i < nl; ++i) {
             double jump = interior ? vm[i] : 2.0 * vm[i];
             double avg = interior ? 0.5 * dnm[i] : dnm[i];
-            diag[off + dm[i]] += c->laplace * jxw * ((sigma * jump - avg) * vm[i] - 0.5 * jump * dnm[i]);
+            const double v = c->laplace * jxw * ((sigma * jump - avg) * vm[i] - 0.5 * jump * dnm[i]);
+#pragma omp atomic
+            diag[off + dm[i]] += v;
           }
           if (interior)
             for (int i = 0; i < nl; ++i) {
               const double jump = -vp[i], avg = 0.5 * dnp[i];
-              diag[off + dp[i]] += c->laplace * jxw * (-(sigma * jump - avg) * vp[i] - 0.5 * jump * dnp[i]);
+              const double v = c->laplace * jxw * (-(sigma * jump - avg) * vp[i] - 0.5 * jump * dnp[i]);
+#pragma omp atomic
+              diag[off + dp[i]] += v;
             }
         }
       }
     }
-    k = j;
   }
+  free(starts);
   free(f);
   return 0;
 }
 
 static int dg_run(const orc_mesh* m, int p, const int64_t* cd, int64_t n_dofs, int ncomp, double mass,
-                  double laplace, const double* src, double* dst, double* diag) {
+                  double laplace, const double* src, double* dst, double* diag, int parallel) {
   if (p < 1 || p > 3) return -1;
   dgctx c;
+  c.parallel = parallel;
   c.p = p;
   c.nl = nloc_of(p);
   c.ncomp = ncomp;
@@ -320,10 +345,15 @@ static int dg_run(const orc_mesh* m, int p, const int64_t* cd, int64_t n_dofs, i
 
 int orc_dg_vmult(const orc_mesh* m, int p, const int64_t* cd, int64_t n_dofs, int ncomp, double mass,
                  double laplace, const double* src, double* dst) {
-  return dg_run(m, p, cd, n_dofs, ncomp, mass, laplace, src, dst, NULL);
+  return dg_run(m, p, cd, n_dofs, ncomp, mass, laplace, src, dst, NULL, 0);
+}
+
+int orc_dg_vmult_omp(const orc_mesh* m, int p, const int64_t* cd, int64_t n_dofs, int ncomp, double mass,
+                     double laplace, const double* src, double* dst) {
+  return dg_run(m, p, cd, n_dofs, ncomp, mass, laplace, src, dst, NULL, 1);
 }
 
 int orc_dg_diagonal(const orc_mesh* m, int p, const int64_t* cd, int64_t n_dofs, int ncomp, double mass,
                     double laplace, double* diag) {
-  return dg_run(m, p, cd, n_dofs, ncomp, mass, laplace, NULL, NULL, diag);
+  return dg_run(m, p, cd, n_dofs, ncomp, mass, laplace, NULL, NULL, diag, 0);
 }

@@ -28,7 +28,7 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         for name in ["orc_cg_vmult", "orc_cg_diagonal", "orc_cg_matrix_vmult", "orc_cg_dense",
-                     "orc_dg_vmult", "orc_dg_diagonal", "orc_dg_tau"]:
+                     "orc_dg_vmult", "orc_dg_diagonal", "orc_dg_tau", "orc_dg_vmult_omp"]:
             getattr(_lib, name).restype = C.c_int
         _lib.orc_basis.argtypes = [C.c_int, vp, vp, vp]
         _lib.orc_tet_rule.argtypes = [C.c_int, vp, vp]
@@ -155,6 +155,17 @@ class DGOracleProblem:
                                 C.c_double(self.laplace), C.c_void_p(s.ctypes.data), C.c_void_p(d.ctypes.data))
         if rc:
             raise RuntimeError("orc_dg_vmult failed")
+        return self._from_orc(d)
+
+    def vmult_omp(self, src):
+        s = self._to_orc(np.asarray(src, np.float64))
+        d = np.empty_like(s)
+        rc = lib().orc_dg_vmult_omp(C.byref(self._m), self.degree, C.c_void_p(self.cell_dofs.ctypes.data),
+                                    C.c_int64(self.n_scalar), self.ncomp, C.c_double(self.mass),
+                                    C.c_double(self.laplace), C.c_void_p(s.ctypes.data),
+                                    C.c_void_p(d.ctypes.data))
+        if rc:
+            raise RuntimeError("orc_dg_vmult_omp failed")
         return self._from_orc(d)
 
     def diagonal(self):
