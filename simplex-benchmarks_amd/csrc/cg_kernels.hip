@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
   extern __shared__ unsigned char smem_raw[];
   T* su = reinterpret_cast<T*>(smem_raw);
   const int lane = threadIdx.x;
-  const int patch = blockIdx.x;
+  const int patch = (int)xcd_block(blockIdx.x, gridDim.x);
   const int c0 = cell_start[patch], nc = cell_start[patch + 1] - c0;
   const int o0 = own_start[patch], no = own_start[patch + 1] - o0;
   const int g0 = ghost_start[patch], ng = ghost_start[patch + 1] - g0;

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kThreads) void dg_cell_kernel(
   for (int j = threadIdx.x; j < FT; j += kThreads) sft[j] = face_tab[j];
   __syncthreads();
 
-  const int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int64_t k = xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
   if (k >= ncells) return;
   const T* CG = sct;
   const T* CW = sct + NQ * NL * 3;

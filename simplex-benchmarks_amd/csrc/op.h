@@ -113,6 +113,19 @@ struct DGPlan {
   double penalty_factor = 0, mass = 0, laplace = 1;
 };
 
+#ifdef __HIPCC__
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  This bijection gives XCD x the contiguous logical range
+// [x q + min(x, r), ...) of the nb blocks (q = nb / 8, r = nb % 8), so
+// neighbouring cells/patches, which the hierarchical order keeps close in
+// index, share one L2.
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+  constexpr int kXcd = 8;
+  const int64_t q = nb / kXcd, r = nb % kXcd, x = b % kXcd, i = b / kXcd;
+  return x * q + (x < r ? x : r) + i;
+}
+#endif
+
 struct Operator {
   int kind = MF_CG_POISSON, precision = MF_F64, degree = 1, ncomp = 1;
   mf_options opt{};
