@@ -54,6 +54,7 @@ typedef struct mf_options {
   int32_t penalty_kind;     /* 0: (p+1)(p+3)/3 max(tau-,tau+) (reference); 1: (p+1)(p+2)/h_face */
   int32_t host_only;        /* 1: build the host plan only (no device memory); device entry points fail */
   int32_t accumulate;       /* CG LDS accumulation: 0 auto, 1 LDS atomics, 2 read-add-write per colour */
+  int32_t dg_kernel;        /* DG kernel: 0 auto, 1 LDS tables, 2 compile-time tables + neighbour in face-3 frame */
 } mf_options;
 
 typedef struct mf_info {

@@ -191,6 +191,7 @@ void mf_default_options(mf_options* opt) {
   opt->penalty_kind = 0;
   opt->host_only = 0;
   opt->accumulate = 0;
+  opt->dg_kernel = 0;
 }
 
 int mf_mesh_kuhn_cube(int32_t n, double perturb, uint64_t seed, double* vertices, int32_t* cells) {

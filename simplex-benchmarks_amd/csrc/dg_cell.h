@@ -1,0 +1,396 @@
+// Per-cell SIPG algorithm shared by the HIP kernel and its host execution.
+//
+// A cell K computes its cell integral and, for each local face f, the SIPG
+// flux with the neighbour K' across f, testing only K's basis functions.
+//   * Canonical faces: every cell lists its vertices in ascending global id,
+//     so face f of K and face f' of K' are parametrised identically.
+//   * Neighbour in "face 3" frame: K''s local nodes are renumbered so that
+//     f' becomes local face 3 (vertices (kFV[f'] ascending, f')); then every
+//     neighbour evaluation uses the same compile-time face-3 tables, and only
+//     the DoF gather offsets and the barycentric normal components depend on
+//     f' (small selects).
+//   * Normal derivatives through the gradient field: grad u of a P_p function
+//     lies in (P_{p-1})^3, so grad u . nJ on a face is the P_{p-1} face
+//     interpolant of its values at the P_{p-1} face nodes (3 for P2).
+// Reference: SIPG face flux of DG/cells_batched/application.cc:2104-2187
+// (interior) and 2688-2701 (boundary), cell integral 1493-1735, penalty
+// 1193-1245; components batched per DG/components_batched/application.cc;
+// mass term of application_helmholtz.cc:1025-1111.
+#pragma once
+#include <cmath>
+#include <cstdint>
+
+#include "shape_tables.h"
+
+#ifdef __HIPCC__
+#define SMF_HD __host__ __device__
+#else
+#define SMF_HD
+#endif
+
+namespace smf {
+namespace dgt {
+
+using tab::alpha;
+using tab::basis;
+using tab::kFV;
+using tab::nloc;
+
+template <int P>
+struct FaceRule;
+template <>
+struct FaceRule<1> {
+  static constexpr int NQF = 3;
+  static constexpr const double (*pts)[2] = q_tri_d2_pts;
+  static constexpr const double* w = q_tri_d2_w;
+};
+template <>
+struct FaceRule<2> {
+  static constexpr int NQF = 6;
+  static constexpr const double (*pts)[2] = q_tri_d4_pts;
+  static constexpr const double* w = q_tri_d4_w;
+};
+template <>
+struct FaceRule<3> {
+  static constexpr int NQF = 12;
+  static constexpr const double (*pts)[2] = q_tri_d6_pts;
+  static constexpr const double* w = q_tri_d6_w;
+};
+
+// cell rule: stiffness only -> degree 2p-2; with a mass term -> degree 2p
+template <int P, bool MASS>
+struct DGCellRule {
+  static constexpr int NQ = tab::CellRule<P>::NQ;
+  static constexpr const double (*pts)[3] = tab::CellRule<P>::pts;
+  static constexpr const double* w = tab::CellRule<P>::w;
+};
+template <>
+struct DGCellRule<1, true> {
+  static constexpr int NQ = 4;
+  static constexpr const double (*pts)[3] = q_tet_d2_pts;
+  static constexpr const double* w = q_tet_d2_w;
+};
+template <>
+struct DGCellRule<2, true> {
+  static constexpr int NQ = 11;
+  static constexpr const double (*pts)[3] = q_tet_d4_pts;
+  static constexpr const double* w = q_tet_d4_w;
+};
+template <>
+struct DGCellRule<3, true> {
+  static constexpr int NQ = 24;
+  static constexpr const double (*pts)[3] = q_tet_d6_pts;
+  static constexpr const double* w = q_tet_d6_w;
+};
+
+template <int P, bool MASS>
+struct DGCellTab {
+  static constexpr int NL = nloc(P), NQ = DGCellRule<P, MASS>::NQ;
+  double g[NQ][NL][3];
+  double v[NQ][NL];
+  double w[NQ];
+};
+
+template <int P, bool MASS>
+constexpr DGCellTab<P, MASS> make_cell_tab() {
+  DGCellTab<P, MASS> t{};
+  for (int q = 0; q < DGCellRule<P, MASS>::NQ; ++q) {
+    t.w[q] = DGCellRule<P, MASS>::w[q];
+    for (int i = 0; i < nloc(P); ++i) {
+      t.v[q][i] = basis(P, i, DGCellRule<P, MASS>::pts[q], -1);
+      for (int d = 0; d < 3; ++d) t.g[q][i][d] = basis(P, i, DGCellRule<P, MASS>::pts[q], d);
+    }
+  }
+  return t;
+}
+
+template <int P>
+struct DGFaceTab {
+  static constexpr int NL = nloc(P), NQF = FaceRule<P>::NQF;
+  static constexpr int NFN = (P + 1) * (P + 2) / 2;          // P_p nodes on a face
+  static constexpr int NFG = P == 1 ? 1 : P * (P + 1) / 2;   // P_{p-1} nodes on a face
+  int fn[4][NFN];            // cell-local index of the face nodes of face f
+  double v[4][NQF][NFN];     // their values at the face points
+  double dg[4][NFG][NL][3];  // reference gradients of all basis functions at the P_{p-1} face nodes
+  double psi[4][NQF][NFG];   // P_{p-1} basis of those nodes at the face points
+  double w[NQF];             // face weights (sum 1/2)
+  int perm[4][NL];           // neighbour frame: new node i <- old node perm[f'][i]
+  int sigma[4][4];           // new vertex j <- old vertex sigma[f'][j]
+};
+
+constexpr void face_point(int f, double s, double t, double* x) {
+  const double V[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  const int a = kFV[f][0], b = kFV[f][1], c = kFV[f][2];
+  for (int d = 0; d < 3; ++d) x[d] = V[a][d] + s * (V[b][d] - V[a][d]) + t * (V[c][d] - V[a][d]);
+}
+
+template <int P>
+constexpr DGFaceTab<P> make_face_tab() {
+  DGFaceTab<P> t{};
+  constexpr int NL = nloc(P), NQF = FaceRule<P>::NQF;
+  for (int q = 0; q < NQF; ++q) t.w[q] = FaceRule<P>::w[q];
+  for (int f = 0; f < 4; ++f) {
+    int n = 0;
+    for (int i = 0; i < NL; ++i)
+      if (alpha(P, i).a[f] == 0) t.fn[f][n++] = i;
+    for (int q = 0; q < NQF; ++q) {
+      double x[3] = {0, 0, 0};
+      face_point(f, FaceRule<P>::pts[q][0], FaceRule<P>::pts[q][1], x);
+      for (int k = 0; k < DGFaceTab<P>::NFN; ++k) t.v[f][q][k] = basis(P, t.fn[f][k], x, -1);
+      if (P == 1) {
+        t.psi[f][q][0] = 1.0;
+      } else {
+        int m = 0;
+        for (int j = 0; j < nloc(P - 1); ++j)
+          if (alpha(P - 1, j).a[f] == 0) t.psi[f][q][m++] = basis(P - 1, j, x, -1);
+      }
+    }
+    if (P == 1) {
+      const double x0[3] = {0.25, 0.25, 0.25};
+      for (int i = 0; i < NL; ++i)
+        for (int d = 0; d < 3; ++d) t.dg[f][0][i][d] = basis(P, i, x0, d);
+    } else {
+      int m = 0;
+      for (int j = 0; j < nloc(P - 1); ++j) {
+        const tab::Alpha al = alpha(P - 1, j);
+        if (al.a[f] != 0) continue;
+        const double V[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        double y[3] = {0, 0, 0};
+        for (int kk = 0; kk < 4; ++kk)
+          for (int d = 0; d < 3; ++d) y[d] += al.a[kk] * V[kk][d] / (P - 1);
+        for (int i = 0; i < NL; ++i)
+          for (int d = 0; d < 3; ++d) t.dg[f][m][i][d] = basis(P, i, y, d);
+        ++m;
+      }
+    }
+    // neighbour frame for f' = f: new vertices (kFV[f] ascending, f)
+    const int sg[4] = {kFV[f][0], kFV[f][1], kFV[f][2], f};
+    for (int j = 0; j < 4; ++j) t.sigma[f][j] = sg[j];
+    for (int i = 0; i < NL; ++i) {
+      const tab::Alpha an = alpha(P, i);
+      int ao[4] = {0, 0, 0, 0};
+      for (int j = 0; j < 4; ++j) ao[sg[j]] = an.a[j];
+      t.perm[f][i] = -1;
+      for (int o = 0; o < NL; ++o) {
+        const tab::Alpha b = alpha(P, o);
+        if (b.a[0] == ao[0] && b.a[1] == ao[1] && b.a[2] == ao[2] && b.a[3] == ao[3]) t.perm[f][i] = o;
+      }
+    }
+  }
+  return t;
+}
+
+// outward reference normals of the faces (face f opposite vertex f), unnormalised
+constexpr double kNhat[4][3] = {{1, 1, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+
+template <typename T>
+SMF_HD inline T select4(int f, T a0, T a1, T a2, T a3) {
+  return f == 0 ? a0 : (f == 1 ? a1 : (f == 2 ? a2 : a3));
+}
+
+// One cell k: MODE 0 dst rows of k = (A src) rows; MODE 1 = diagonal of A.
+// geo: SoA [11][ncells] = J^-T (row-major), |det J|, tau
+template <int P, typename T, int NC, bool MASS, int MODE>
+SMF_HD void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, const int32_t* __restrict__ nbr,
+                       const uint32_t* __restrict__ nface, T factor, T mass, T lap, const T* __restrict__ src,
+                       T* __restrict__ dst) {
+  constexpr DGCellTab<P, MASS> CT = make_cell_tab<P, MASS>();
+  constexpr DGFaceTab<P> FT = make_face_tab<P>();
+  constexpr int NL = nloc(P), NQ = DGCellTab<P, MASS>::NQ, NQF = DGFaceTab<P>::NQF;
+  constexpr int NFN = DGFaceTab<P>::NFN, NFG = DGFaceTab<P>::NFG;
+  using std::sqrt;
+
+  T jit[9];
+  for (int t = 0; t < 9; ++t) jit[t] = geo[t * ncells + k];
+  const T adet = geo[9 * ncells + k], tau = geo[10 * ncells + k];
+  const T M0 = adet * (jit[0] * jit[0] + jit[3] * jit[3] + jit[6] * jit[6]);
+  const T M1 = adet * (jit[0] * jit[1] + jit[3] * jit[4] + jit[6] * jit[7]);
+  const T M2 = adet * (jit[0] * jit[2] + jit[3] * jit[5] + jit[6] * jit[8]);
+  const T M3 = adet * (jit[1] * jit[1] + jit[4] * jit[4] + jit[7] * jit[7]);
+  const T M4 = adet * (jit[1] * jit[2] + jit[4] * jit[5] + jit[7] * jit[8]);
+  const T M5 = adet * (jit[2] * jit[2] + jit[5] * jit[5] + jit[8] * jit[8]);
+  const uint32_t nf = nface[k];
+
+#pragma unroll 1
+  for (int c = 0; c < NC; ++c) {
+    const int64_t base = (k * NC + c) * NL;
+    T u[NL], y[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      u[i] = MODE == 0 ? src[base + i] : T(0);
+      y[i] = T(0);
+    }
+    // ---- cell integral ----
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const T w = T(CT.w[q]);
+      if (MODE == 0) {
+        T gx = T(0), gy = T(0), gz = T(0), uv = T(0);
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          gx += T(CT.g[q][i][0]) * u[i];
+          gy += T(CT.g[q][i][1]) * u[i];
+          gz += T(CT.g[q][i][2]) * u[i];
+          if (MASS) uv += T(CT.v[q][i]) * u[i];
+        }
+        const T wl = w * lap;
+        const T tx = wl * (M0 * gx + M1 * gy + M2 * gz);
+        const T ty = wl * (M1 * gx + M3 * gy + M4 * gz);
+        const T tz = wl * (M2 * gx + M4 * gy + M5 * gz);
+        const T tv = MASS ? w * mass * adet * uv : T(0);
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          y[i] += T(CT.g[q][i][0]) * tx + T(CT.g[q][i][1]) * ty + T(CT.g[q][i][2]) * tz;
+          if (MASS) y[i] += T(CT.v[q][i]) * tv;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          const T gx = T(CT.g[q][i][0]), gy = T(CT.g[q][i][1]), gz = T(CT.g[q][i][2]);
+          y[i] += w * lap * (gx * (M0 * gx + M1 * gy + M2 * gz) + gy * (M1 * gx + M3 * gy + M4 * gz) +
+                             gz * (M2 * gx + M4 * gy + M5 * gz));
+          if (MASS) y[i] += w * mass * adet * T(CT.v[q][i]) * T(CT.v[q][i]);
+        }
+      }
+    }
+    // ---- face integrals ----
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+      // keep each face's neighbour loads and temporaries inside its own
+      // iteration (bounds register pressure)
+      __builtin_amdgcn_sched_barrier(0);
+#endif
+      const T nj0 = jit[0] * T(kNhat[f][0]) + jit[1] * T(kNhat[f][1]) + jit[2] * T(kNhat[f][2]);
+      const T nj1 = jit[3] * T(kNhat[f][0]) + jit[4] * T(kNhat[f][1]) + jit[5] * T(kNhat[f][2]);
+      const T nj2 = jit[6] * T(kNhat[f][0]) + jit[7] * T(kNhat[f][1]) + jit[8] * T(kNhat[f][2]);
+      const T len = sqrt(nj0 * nj0 + nj1 * nj1 + nj2 * nj2);
+      const T il = T(1) / len;
+      const T n0 = nj0 * il, n1 = nj1 * il, n2 = nj2 * il;
+      const T area2 = adet * len * lap;
+      // nJ = J^-1 n: d/dn of a function = (reference gradient) . nJ
+      const T nJ[3] = {jit[0] * n0 + jit[3] * n1 + jit[6] * n2, jit[1] * n0 + jit[4] * n1 + jit[7] * n2,
+                       jit[2] * n0 + jit[5] * n1 + jit[8] * n2};
+      const int32_t kb = nbr[f * ncells + k];
+      const bool interior = kb >= 0;
+      const int64_t kbs = interior ? kb : k;
+      const T taub = geo[10 * ncells + kbs];
+      const T sigma = factor * (interior ? (tau > taub ? tau : taub) : tau);
+
+      if (MODE == 0) {
+        // own normal derivative at the P_{p-1} face nodes
+        T gn[NFG];
+#pragma unroll
+        for (int j = 0; j < NFG; ++j) {
+          T sx = T(0), sy = T(0), sz = T(0);
+#pragma unroll
+          for (int i = 0; i < NL; ++i) {
+            sx += T(FT.dg[f][j][i][0]) * u[i];
+            sy += T(FT.dg[f][j][i][1]) * u[i];
+            sz += T(FT.dg[f][j][i][2]) * u[i];
+          }
+          gn[j] = sx * nJ[0] + sy * nJ[1] + sz * nJ[2];
+        }
+        // neighbour, renumbered so that the shared face is its face 3
+        T ub[NL], gb[NFG];
+        if (interior) {
+          const int fb = (nf >> (8 * f)) & 3;
+          const int64_t nbase = ((int64_t)kb * NC + c) * NL;
+#pragma unroll
+          for (int i = 0; i < NL; ++i)
+            ub[i] = src[nbase + select4<int>(fb, FT.perm[0][i], FT.perm[1][i], FT.perm[2][i], FT.perm[3][i])];
+          T jb[9];
+#pragma unroll
+          for (int t = 0; t < 9; ++t) jb[t] = geo[t * ncells + kb];
+          // barycentric normal components beta_a = grad lambda_a . n of the neighbour
+          const T b1 = jb[0] * n0 + jb[3] * n1 + jb[6] * n2;
+          const T b2 = jb[1] * n0 + jb[4] * n1 + jb[7] * n2;
+          const T b3 = jb[2] * n0 + jb[5] * n1 + jb[8] * n2;
+          const T b0 = -(b1 + b2 + b3);
+          const T bo[4] = {b0, b1, b2, b3};
+          T nb[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            nb[d] = select4<T>(fb, bo[FT.sigma[0][d + 1]], bo[FT.sigma[1][d + 1]], bo[FT.sigma[2][d + 1]],
+                               bo[FT.sigma[3][d + 1]]);
+#pragma unroll
+          for (int j = 0; j < NFG; ++j) {
+            T sx = T(0), sy = T(0), sz = T(0);
+#pragma unroll
+            for (int i = 0; i < NL; ++i) {
+              sx += T(FT.dg[3][j][i][0]) * ub[i];
+              sy += T(FT.dg[3][j][i][1]) * ub[i];
+              sz += T(FT.dg[3][j][i][2]) * ub[i];
+            }
+            gb[j] = sx * nb[0] + sy * nb[1] + sz * nb[2];
+          }
+        }
+        T tgj[NFG];
+#pragma unroll
+        for (int j = 0; j < NFG; ++j) tgj[j] = T(0);
+#pragma unroll
+        for (int q = 0; q < NQF; ++q) {
+          T uv = T(0), dn = T(0);
+#pragma unroll
+          for (int kk = 0; kk < NFN; ++kk) uv += T(FT.v[f][q][kk]) * u[FT.fn[f][kk]];
+#pragma unroll
+          for (int j = 0; j < NFG; ++j) dn += T(FT.psi[f][q][j]) * gn[j];
+          T up = -uv, dnp = dn;
+          if (interior) {
+            up = T(0);
+            dnp = T(0);
+#pragma unroll
+            for (int kk = 0; kk < NFN; ++kk) up += T(FT.v[3][q][kk]) * ub[FT.fn[3][kk]];
+#pragma unroll
+            for (int j = 0; j < NFG; ++j) dnp += T(FT.psi[3][q][j]) * gb[j];
+          }
+          const T jxw = T(FT.w[q]) * area2;
+          const T jump = uv - up, avg = T(0.5) * (dn + dnp);
+          const T tv = (sigma * jump - avg) * jxw, tg = T(-0.5) * jump * jxw;
+#pragma unroll
+          for (int kk = 0; kk < NFN; ++kk) y[FT.fn[f][kk]] += T(FT.v[f][q][kk]) * tv;
+#pragma unroll
+          for (int j = 0; j < NFG; ++j) tgj[j] += T(FT.psi[f][q][j]) * tg;
+        }
+        // gradient test: y_i += sum_j tgj[j] * grad phi_i(y_j) . nJ
+#pragma unroll
+        for (int j = 0; j < NFG; ++j) {
+          const T ax = tgj[j] * nJ[0], ay = tgj[j] * nJ[1], az = tgj[j] * nJ[2];
+#pragma unroll
+          for (int i = 0; i < NL; ++i)
+            y[i] += T(FT.dg[f][j][i][0]) * ax + T(FT.dg[f][j][i][1]) * ay + T(FT.dg[f][j][i][2]) * az;
+        }
+      } else {
+        // diagonal: u = phi_i on this cell only (u+ = 0 inside, mirror on the boundary)
+        const T s = interior ? T(1) : T(2);
+        T dnode[NFG][NL];
+#pragma unroll
+        for (int j = 0; j < NFG; ++j)
+#pragma unroll
+          for (int i = 0; i < NL; ++i)
+            dnode[j][i] = T(FT.dg[f][j][i][0]) * nJ[0] + T(FT.dg[f][j][i][1]) * nJ[1] + T(FT.dg[f][j][i][2]) * nJ[2];
+#pragma unroll
+        for (int q = 0; q < NQF; ++q) {
+          const T jxw = T(FT.w[q]) * area2;
+#pragma unroll
+          for (int i = 0; i < NL; ++i) {
+            T vi = T(0);
+#pragma unroll
+            for (int kk = 0; kk < NFN; ++kk)
+              if (FT.fn[f][kk] == i) vi = T(FT.v[f][q][kk]);
+            T di = T(0);
+#pragma unroll
+            for (int j = 0; j < NFG; ++j) di += T(FT.psi[f][q][j]) * dnode[j][i];
+            const T jump = s * vi, avg = T(0.5) * s * di;
+            y[i] += jxw * ((sigma * jump - avg) * vi - T(0.5) * jump * di);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NL; ++i) dst[base + i] = y[i];
+  }
+}
+
+}  // namespace dgt
+}  // namespace smf

@@ -20,6 +20,7 @@
 
 #include <stdexcept>
 
+#include "dg_cell.h"
 #include "op.h"
 
 namespace smf {
@@ -204,10 +205,33 @@ __global__ __launch_bounds__(kThreads) void dg_cell_kernel(
   }
 }
 
+// One lane per cell, compile-time tables (see dg_cell.h).
+template <int P, typename T, int NC, bool MASS, int MODE>
+__global__ __launch_bounds__(kThreads) void dg_cell_kernel_v2(int64_t ncells, const T* __restrict__ geo,
+                                                               const int32_t* __restrict__ nbr,
+                                                               const uint32_t* __restrict__ nface, T factor, T mass,
+                                                               T lap, const T* __restrict__ src,
+                                                               T* __restrict__ dst) {
+  const int64_t k = xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (k >= ncells) return;
+  dgt::apply_cell<P, T, NC, MASS, MODE>(k, ncells, geo, nbr, nface, factor, mass, lap, src, dst);
+}
+
 template <int P, typename T, int NC, bool MASS, int MODE>
 void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
-  using S = DGShape<P, MASS>;
   const DGPlan& pl = op.dg;
+  // compile-time-table kernel except P3 FP64, which exceeds the register file
+  const bool v2 = op.opt.dg_kernel == 2 || (op.opt.dg_kernel == 0 && (P <= 2 || sizeof(T) == 4));
+  if (v2) {
+    if (pl.n_cells > 0)
+      hipLaunchKernelGGL((dg_cell_kernel_v2<P, T, NC, MASS, MODE>),
+                         dim3((unsigned)((pl.n_cells + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, pl.n_cells,
+                         op.d_dg_geo.as<T>(), op.d_dg_nbr.as<int32_t>(), op.d_dg_nface.as<uint32_t>(),
+                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst);
+    SMF_HIP(hipGetLastError());
+    return;
+  }
+  using S = DGShape<P, MASS>;
   if (pl.n_q != S::NQ || pl.n_qf != S::NQF) throw std::runtime_error("DG table size mismatch");
   const size_t shmem = (size_t)(S::NQ * S::NL * 4 + S::NQ + 4 * S::NQF * S::NL * 4 + S::NQF) * sizeof(T);
   auto kern = dg_cell_kernel<P, T, NC, MASS, MODE>;

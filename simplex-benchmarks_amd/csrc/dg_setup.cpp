@@ -14,6 +14,7 @@
 #include <cmath>
 #include <stdexcept>
 
+#include "dg_cell.h"
 #include "op.h"
 
 namespace smf {
@@ -167,138 +168,33 @@ void build_dg_plan(const Mesh& mesh, int p, int ncomp, const mf_options& opt, bo
 }
 
 namespace {
-// The kernel's per-cell algorithm on the host (double or float).
+// The kernel's per-cell algorithm (dg_cell.h), executed on the host.
+template <int P, typename T, int NC, bool MASS>
+void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
+  const DGPlan& pl = op.dg;
+  std::vector<T> geo(pl.geo.begin(), pl.geo.end());
+  for (int64_t k = 0; k < pl.n_cells; ++k) {
+    if (mode == 0)
+      dgt::apply_cell<P, T, NC, MASS, 0>(k, pl.n_cells, geo.data(), pl.nbr.data(), pl.nface.data(),
+                                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst);
+    else
+      dgt::apply_cell<P, T, NC, MASS, 1>(k, pl.n_cells, geo.data(), pl.nbr.data(), pl.nface.data(),
+                                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst);
+  }
+}
+
+template <int P, typename T>
+void host_dg_p(const Operator& op, int mode, T* dst, const T* src) {
+  if (op.kind == MF_DG_SIPG) host_dg_run<P, T, 1, false>(op, mode, dst, src);
+  else if (op.kind == MF_DG_VECTOR) host_dg_run<P, T, 3, false>(op, mode, dst, src);
+  else host_dg_run<P, T, 3, true>(op, mode, dst, src);
+}
+
 template <typename T>
 void host_dg_t(const Operator& op, int mode, T* dst, const T* src) {
-  const DGPlan& pl = op.dg;
-  const int nl = pl.n_loc, nq = pl.n_q, nqf = pl.n_qf, NC = pl.ncomp;
-  const int64_t nc = pl.n_cells;
-  std::vector<T> ct(pl.cell_tab.begin(), pl.cell_tab.end()), ft(pl.face_tab.begin(), pl.face_tab.end());
-  const T* CG = ct.data();
-  const T* CW = ct.data() + (size_t)nq * nl * 3;
-  const T* CV = CW + nq;
-  const size_t fstride = (size_t)nqf * nl * 4;
-  const T* FW = ft.data() + 4 * fstride;
-  const T factor = (T)pl.penalty_factor, mass = (T)pl.mass, lap = (T)pl.laplace;
-  auto geo = [&](int t, int64_t k) { return (T)pl.geo[(size_t)t * nc + k]; };
-  for (int64_t k = 0; k < nc; ++k) {
-    T jit[9];
-    for (int t = 0; t < 9; ++t) jit[t] = geo(t, k);
-    const T adet = geo(9, k), tau = geo(10, k);
-    T M[6];
-    {
-      const int ab[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
-      for (int t = 0; t < 6; ++t) {
-        T s = 0;
-        for (int e = 0; e < 3; ++e) s += jit[3 * e + ab[t][0]] * jit[3 * e + ab[t][1]];
-        M[t] = adet * s;
-      }
-    }
-    for (int c = 0; c < NC; ++c) {
-      const int64_t base = (k * NC + c) * nl;
-      T u[20], y[20];
-      for (int i = 0; i < nl; ++i) {
-        u[i] = mode == 0 ? src[base + i] : T(0);
-        y[i] = T(0);
-      }
-      // cell term
-      for (int q = 0; q < nq; ++q) {
-        const T* G = CG + (size_t)q * nl * 3;
-        const T* V = CV + (size_t)q * nl;
-        if (mode == 0) {
-          T g[3] = {0, 0, 0}, uv = 0;
-          for (int i = 0; i < nl; ++i) {
-            for (int d = 0; d < 3; ++d) g[d] += G[3 * i + d] * u[i];
-            uv += V[i] * u[i];
-          }
-          const T w = CW[q];
-          const T tx = w * lap * (M[0] * g[0] + M[1] * g[1] + M[2] * g[2]);
-          const T ty = w * lap * (M[1] * g[0] + M[3] * g[1] + M[4] * g[2]);
-          const T tz = w * lap * (M[2] * g[0] + M[4] * g[1] + M[5] * g[2]);
-          const T tv = w * mass * adet * uv;
-          for (int i = 0; i < nl; ++i) y[i] += G[3 * i] * tx + G[3 * i + 1] * ty + G[3 * i + 2] * tz + V[i] * tv;
-        } else {
-          for (int i = 0; i < nl; ++i) {
-            const T* g = &G[3 * i];
-            y[i] += CW[q] * (lap * (g[0] * (M[0] * g[0] + M[1] * g[1] + M[2] * g[2]) +
-                                    g[1] * (M[1] * g[0] + M[3] * g[1] + M[4] * g[2]) +
-                                    g[2] * (M[2] * g[0] + M[4] * g[1] + M[5] * g[2])) +
-                             mass * adet * V[i] * V[i]);
-          }
-        }
-      }
-      // faces
-      for (int f = 0; f < 4; ++f) {
-        const T* FV = ft.data() + f * fstride;
-        const T* FG = FV + (size_t)nqf * nl;
-        T nj[3], nn = 0;
-        for (int a = 0; a < 3; ++a) {
-          nj[a] = jit[3 * a] * (T)kNhat[f][0] + jit[3 * a + 1] * (T)kNhat[f][1] + jit[3 * a + 2] * (T)kNhat[f][2];
-          nn += nj[a] * nj[a];
-        }
-        const T len = std::sqrt(nn);
-        T n[3] = {nj[0] / len, nj[1] / len, nj[2] / len};
-        const T area2 = adet * len;  // JxW_q = w_q * area2
-        // nJ = J^-1 n = jit^T n
-        T nJ[3];
-        for (int a = 0; a < 3; ++a) nJ[a] = jit[a] * n[0] + jit[3 + a] * n[1] + jit[6 + a] * n[2];
-        const int32_t kb = pl.nbr[(size_t)f * nc + k];
-        T nJb[3] = {0, 0, 0}, ub[20];
-        T sigma = factor * tau;
-        const T* FVb = nullptr;
-        const T* FGb = nullptr;
-        if (kb >= 0) {
-          const int fb = (pl.nface[k] >> (8 * f)) & 3;
-          FVb = ft.data() + fb * fstride;
-          FGb = FVb + (size_t)nqf * nl;
-          T jb[9];
-          for (int t = 0; t < 9; ++t) jb[t] = geo(t, kb);
-          for (int a = 0; a < 3; ++a) nJb[a] = jb[a] * n[0] + jb[3 + a] * n[1] + jb[6 + a] * n[2];
-          sigma = factor * std::max(tau, geo(10, kb));
-          for (int i = 0; i < nl; ++i) ub[i] = mode == 0 ? src[(kb * NC + c) * nl + i] : T(0);
-        }
-        for (int q = 0; q < nqf; ++q) {
-          const T* V = FV + (size_t)q * nl;
-          const T* G = FG + (size_t)q * nl * 3;
-          const T jxw = FW[q] * area2 * lap;
-          if (mode == 0) {
-            T uv = 0, dn = 0;
-            for (int i = 0; i < nl; ++i) {
-              uv += V[i] * u[i];
-              dn += (G[3 * i] * nJ[0] + G[3 * i + 1] * nJ[1] + G[3 * i + 2] * nJ[2]) * u[i];
-            }
-            T up, dnp;
-            if (kb >= 0) {
-              up = 0;
-              dnp = 0;
-              const T* Vb = FVb + (size_t)q * nl;
-              const T* Gb = FGb + (size_t)q * nl * 3;
-              for (int i = 0; i < nl; ++i) {
-                up += Vb[i] * ub[i];
-                dnp += (Gb[3 * i] * nJb[0] + Gb[3 * i + 1] * nJb[1] + Gb[3 * i + 2] * nJb[2]) * ub[i];
-              }
-            } else {
-              up = -uv;
-              dnp = dn;
-            }
-            const T jump = uv - up, avg = T(0.5) * (dn + dnp);
-            const T tv = (sigma * jump - avg) * jxw, tg = T(-0.5) * jump * jxw;
-            for (int i = 0; i < nl; ++i)
-              y[i] += V[i] * tv + (G[3 * i] * nJ[0] + G[3 * i + 1] * nJ[1] + G[3 * i + 2] * nJ[2]) * tg;
-          } else {
-            const T s = kb >= 0 ? T(1) : T(2);
-            for (int i = 0; i < nl; ++i) {
-              const T vi = V[i];
-              const T di = G[3 * i] * nJ[0] + G[3 * i + 1] * nJ[1] + G[3 * i + 2] * nJ[2];
-              const T jump = s * vi, avg = T(0.5) * s * di;
-              y[i] += jxw * ((sigma * jump - avg) * vi - T(0.5) * jump * di);
-            }
-          }
-        }
-      }
-      for (int i = 0; i < nl; ++i) dst[(k * NC + c) * nl + i] = y[i];
-    }
-  }
+  if (op.dg.p == 1) host_dg_p<1, T>(op, mode, dst, src);
+  else if (op.dg.p == 2) host_dg_p<2, T>(op, mode, dst, src);
+  else host_dg_p<3, T>(op, mode, dst, src);
 }
 }  // namespace
 

@@ -37,7 +37,8 @@ class MeshDesc(C.Structure):
 class Options(C.Structure):
     _fields_ = [("reorder", C.c_int32), ("patch_cells", C.c_int32),
                 ("mass_coefficient", C.c_double), ("laplace_coefficient", C.c_double),
-                ("penalty_kind", C.c_int32), ("host_only", C.c_int32), ("accumulate", C.c_int32)]
+                ("penalty_kind", C.c_int32), ("host_only", C.c_int32), ("accumulate", C.c_int32),
+                ("dg_kernel", C.c_int32)]
 
 
 class Info(C.Structure):
@@ -304,7 +305,7 @@ class Operator:
 
     def __init__(self, mesh: Mesh, degree: int, kind="cg", precision="f64", reorder=True,
                  patch_cells=0, mass_coefficient=1.0, laplace_coefficient=1.0,
-                 penalty_kind=0, host_only=False, accumulate=0):
+                 penalty_kind=0, host_only=False, accumulate=0, dg_kernel=0):
         L = lib()
         opt = Options()
         L.mf_default_options(C.byref(opt))
@@ -315,6 +316,7 @@ class Operator:
         opt.penalty_kind = penalty_kind
         opt.host_only = 1 if host_only else 0
         opt.accumulate = accumulate
+        opt.dg_kernel = dg_kernel
         self.mesh = mesh
         self.kind = KINDS[kind] if isinstance(kind, str) else int(kind)
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)

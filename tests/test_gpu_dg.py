@@ -41,3 +41,17 @@ def test_dg_reorder_off_and_determinism(gpu, orc, p):
     op.Tvmult(b, src)
     assert np.array_equal(a.numpy(), b.numpy())
     assert rel_err(a.numpy(), pr.vmult(x)) < 1e-12
+
+
+@pytest.mark.parametrize("dg_kernel", [1, 2])
+@pytest.mark.parametrize("p", [1, 2])
+def test_dg_kernel_variants(gpu, orc, p, dg_kernel):
+    smf = gpu
+    print(f"\ncase: dg p={p} dg_kernel={dg_kernel}", flush=True)
+    op = smf.Operator(smf.Mesh.kuhn_cube(5, 0.2, 3), p, kind="dg", dg_kernel=dg_kernel)
+    pr = orc.DGOracleProblem.from_operator(op)
+    x = np.random.default_rng(2).uniform(-1, 1, op.m())
+    src = smf.DeviceVector.from_numpy(x)
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, src)
+    assert rel_err(dst.numpy(), pr.vmult(x)) < 1e-12
