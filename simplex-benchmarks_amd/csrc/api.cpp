@@ -47,7 +47,7 @@ void upload_cg(Operator& op) {
   op.d_fix_ptr.upload(pl.fix_ptr);
   op.d_fix_src.upload(pl.fix_src);
   op.d_lidx.upload(pl.lidx);
-  op.d_partial.alloc(std::max<int64_t>(pl.n_ghost_total, 1) * op.wbytes());
+  op.d_partial.alloc(std::max<int64_t>(pl.n_ghost_total, 1) * op.ncomp * op.wbytes());
   if (op.precision == MF_F64) {
     op.d_metric.upload(pl.metric);
     op.d_shape.upload(pl.shape);
@@ -82,6 +82,11 @@ void host_apply_cg_t(const Operator& op, int mode, T* dst, const T* src) {
   const int nl = pl.n_loc, nq = pl.n_q;
   std::vector<T> G(pl.shape.begin(), pl.shape.end());
   const T* W = G.data() + nl * nq * 3;
+  const int NCc = op.ncomp;
+  for (int cc = 0; cc < NCc; ++cc) {
+  const int64_t off = cc * pl.n_dofs;
+  T* dstc = dst + off;
+  const T* srcc = src ? src + off : nullptr;
   std::vector<T> su, sy, partial(pl.n_ghost_total, T(0));
   for (int64_t patch = 0; patch < pl.n_patches; ++patch) {
     const int c0 = pl.cell_start[patch], nc = pl.cell_start[patch + 1] - c0;
@@ -90,7 +95,7 @@ void host_apply_cg_t(const Operator& op, int mode, T* dst, const T* src) {
     su.assign(no + ng, T(0));
     sy.assign(no + ng, T(0));
     if (mode == 0)
-      for (int j = 0; j < no + ng; ++j) su[j] = j < no ? src[o0 + j] : src[pl.ghost_idx[g0 + j - no]];
+      for (int j = 0; j < no + ng; ++j) su[j] = j < no ? srcc[o0 + j] : srcc[pl.ghost_idx[g0 + j - no]];
     for (int lc = 0; lc < nc; ++lc) {
       const int64_t cell = c0 + lc;
       T m[6];
@@ -125,16 +130,17 @@ void host_apply_cg_t(const Operator& op, int mode, T* dst, const T* src) {
         if (li[i] != 0xFFFF) sy[li[i]] += y[i];
     }
     for (int j = 0; j < no + ng; ++j) {
-      if (j < no) dst[o0 + j] = sy[j];
+      if (j < no) dstc[o0 + j] = sy[j];
       else partial[g0 + j - no] = sy[j];
     }
   }
   for (int64_t t = 0; t < pl.n_free - pl.n_int; ++t) {
     T s = partial[pl.fix_src[pl.fix_ptr[t]]];
     for (int32_t k = pl.fix_ptr[t] + 1; k < pl.fix_ptr[t + 1]; ++k) s += partial[pl.fix_src[k]];
-    dst[pl.n_int + t] = s;
+    dstc[pl.n_int + t] = s;
   }
-  for (int64_t i = pl.n_free; i < pl.n_dofs; ++i) dst[i] = mode == 0 ? src[i] : T(1);
+  for (int64_t i = pl.n_free; i < pl.n_dofs; ++i) dstc[i] = mode == 0 ? srcc[i] : T(1);
+  }
 }
 }  // namespace
 
@@ -211,7 +217,8 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
     if (degree < 1 || degree > 3) throw std::runtime_error("degree must be 1, 2 or 3");
     if (opt && (opt->accumulate < 0 || opt->accumulate > 2)) throw std::runtime_error("accumulate must be 0, 1 or 2");
     if (precision != MF_F64 && precision != MF_F32) throw std::runtime_error("precision must be MF_F64 or MF_F32");
-    if (kind != MF_CG_POISSON && kind != MF_DG_SIPG && kind != MF_DG_VECTOR && kind != MF_DG_HELMHOLTZ)
+    if (kind != MF_CG_POISSON && kind != MF_CG_VECTOR && kind != MF_DG_SIPG && kind != MF_DG_VECTOR &&
+        kind != MF_DG_HELMHOLTZ)
       throw std::runtime_error("operator kind not available in this build");
     if (desc->n_vertices <= 0 || desc->n_cells < 0 || !desc->vertices || (!desc->cells && desc->n_cells))
       throw std::runtime_error("invalid mesh description");
@@ -245,6 +252,8 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
       build_dg_plan(op.mesh, degree, op.ncomp, op.opt, kind == MF_DG_HELMHOLTZ, op.dg);
       if (!op.opt.host_only) upload_dg(op);
     } else {
+      op.ncomp = kind == MF_CG_VECTOR ? 3 : 1;
+      op.cg.ncomp = op.ncomp;
       build_cg_plan(op.mesh, degree, op.opt, op.cg);
       make_cg_shape(op.cg);
       if (!op.opt.host_only) upload_cg(op);
@@ -327,8 +336,9 @@ int mf_get_info(mf_op_t h, mf_info* info) {
       info->max_cell_colors = pl.max_icolors;
       info->max_local_dofs = pl.max_local;
       // reference data model: src + dst, 32-bit index per cell DoF, J^-T + det J per cell
-      info->algorithmic_bytes = 2.0 * w * pl.n_dofs + 4.0 * pl.n_loc * pl.n_cells + 10.0 * w * pl.n_cells;
-      info->flops = (12.0 * pl.n_loc * pl.n_q + 18.0 * pl.n_q) * pl.n_cells;
+      info->algorithmic_bytes =
+          2.0 * w * pl.n_dofs * op.ncomp + 4.0 * pl.n_loc * pl.n_cells + 10.0 * w * pl.n_cells;
+      info->flops = op.ncomp * (12.0 * pl.n_loc * pl.n_q + 18.0 * pl.n_q) * pl.n_cells;
     }
   });
 }

@@ -46,30 +46,39 @@ struct CGShape {
 // shape: G[(q*NL + i)*3 + d] = d phi_i / d xhat_d at point q, then w[q].
 // ATOMIC: accumulate with ds_add (fast for f64 on gfx950) or with a plain
 // LDS read-add-write per colour (ds_add_f32 is slow on gfx950).
-template <int P, typename T, int MODE, bool ATOMIC>
+// NC components (components-batched vector Laplacian,
+// CG/components_batched/application.cc:1024-1216) share the index tables and
+// the geometry of a cell; component c of scalar DoF d is row c*cstride + d.
+template <int P, typename T, int MODE, bool ATOMIC, int NC>
 __global__ __launch_bounds__(kWave) void cg_wave_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ own_start,
     const int32_t* __restrict__ ghost_start, const int32_t* __restrict__ ghost_idx,
     const uint8_t* __restrict__ ccol, const uint16_t* __restrict__ lidx, int64_t ncells_total,
     const T* __restrict__ metric, const T* __restrict__ shape, const T* __restrict__ src,
-    T* __restrict__ dst, T* __restrict__ partial) {
+    T* __restrict__ dst, T* __restrict__ partial, int64_t cstride, int64_t pstride) {
   constexpr int NL = CGShape<P>::NL, NQ = CGShape<P>::NQ;
   extern __shared__ unsigned char smem_raw[];
-  T* su = reinterpret_cast<T*>(smem_raw);
+  T* sbase = reinterpret_cast<T*>(smem_raw);
   const int lane = threadIdx.x;
   const int patch = (int)xcd_block(blockIdx.x, gridDim.x);
   const int c0 = cell_start[patch], nc = cell_start[patch + 1] - c0;
   const int o0 = own_start[patch], no = own_start[patch + 1] - o0;
   const int g0 = ghost_start[patch], ng = ghost_start[patch + 1] - g0;
   const int nloc = no + ng;
+  // per component: su (staged values) then sy (accumulator), nloc+1 each;
   // slot nloc is a dummy for constrained DoFs (index 0xFFFF clamps to it):
   // it reads as zero and absorbs their contributions, so no branches
-  T* sy = su + nloc + 1;
+  const int lstride = 2 * (nloc + 1);
 
   // 1. stage the patch DoFs
-  for (int j = lane; j <= nloc; j += kWave) {
-    if (MODE == 0) su[j] = j < no ? src[o0 + j] : (j < nloc ? src[ghost_idx[g0 + j - no]] : T(0));
-    sy[j] = T(0);
+  for (int cc = 0; cc < NC; ++cc) {
+    T* su = sbase + cc * lstride;
+    T* sy = su + nloc + 1;
+    const T* sc = src + cc * cstride;
+    for (int j = lane; j <= nloc; j += kWave) {
+      if (MODE == 0) su[j] = j < no ? sc[o0 + j] : (j < nloc ? sc[ghost_idx[g0 + j - no]] : T(0));
+      sy[j] = T(0);
+    }
   }
   __syncthreads();
 
@@ -90,15 +99,18 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
     const int lc = r0 + lane;
     const bool active = lc < nc;
     uint16_t li[NL];
-    T y[NL];
 #pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      li[i] = min((int)li_n[i], nloc);
-      y[i] = T(0);
-    }
+    for (int i = 0; i < NL; ++i) li[i] = min((int)li_n[i], nloc);
     const T m0 = m_n[0], m1 = m_n[1], m2 = m_n[2], m3 = m_n[3], m4 = m_n[4], m5 = m_n[5];
     const int mycol = col_n;
     if (r0 + kWave < nc) load_round(r0 + kWave);
+#pragma unroll 1
+    for (int cc = 0; cc < NC; ++cc) {
+    T* su = sbase + cc * lstride;
+    T* sy = su + nloc + 1;
+    T y[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) y[i] = T(0);
     // P1/P2: compile-time tables (literal constants after unrolling); P3
     // has too many distinct values for scalar registers and reads the table
     // from memory one quadrature point at a time.
@@ -194,15 +206,21 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
         }
       }
     }
+    }  // components
   }
   __syncthreads();
 
   // 4. owned DoFs: store; ghost DoFs: partial sums for the fix-up
-  for (int j = lane; j < nloc; j += kWave) {
-    if (j < no)
-      dst[o0 + j] = sy[j];
-    else
-      partial[g0 + j - no] = sy[j];
+  for (int cc = 0; cc < NC; ++cc) {
+    const T* sy = sbase + cc * lstride + nloc + 1;
+    T* dc = dst + cc * cstride;
+    T* pc = partial + cc * pstride;
+    for (int j = lane; j < nloc; j += kWave) {
+      if (j < no)
+        dc[o0 + j] = sy[j];
+      else
+        pc[g0 + j - no] = sy[j];
+    }
   }
 }
 
@@ -213,24 +231,27 @@ template <typename T, int MODE>
 __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* __restrict__ fix_ptr,
                                 const int32_t* __restrict__ fix_src, const T* __restrict__ partial,
                                 int64_t n0, int64_t n1, const T* __restrict__ src,
-                                T* __restrict__ dst) {
+                                T* __restrict__ dst, int nc, int64_t cstride, int64_t pstride) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n_iface) {
-    const int32_t k0 = fix_ptr[t], k1 = fix_ptr[t + 1];
-    T s = partial[fix_src[k0]];
-    for (int32_t k = k0 + 1; k < k1; ++k) s += partial[fix_src[k]];
-    dst[n_int + t] = s;
+  for (int c = 0; c < nc; ++c) {
+    const T* pc = partial + c * pstride;
+    if (t < n_iface) {
+      const int32_t k0 = fix_ptr[t], k1 = fix_ptr[t + 1];
+      T s = pc[fix_src[k0]];
+      for (int32_t k = k0 + 1; k < k1; ++k) s += pc[fix_src[k]];
+      dst[c * cstride + n_int + t] = s;
+    }
+    const int64_t j = n0 + t;
+    if (j < n1) dst[c * cstride + j] = MODE == 0 ? src[c * cstride + j] : T(1);
   }
-  const int64_t j = n0 + t;
-  if (j < n1) dst[j] = MODE == 0 ? src[j] : T(1);
 }
 
-template <int P, typename T, int MODE>
+template <int P, typename T, int MODE, int NC>
 void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
   const CGPlan& pl = op.cg;
-  const size_t shmem = 2 * ((size_t)pl.max_local + 1) * sizeof(T);
+  const size_t shmem = NC * 2 * ((size_t)pl.max_local + 1) * sizeof(T);
   const bool atomic = op.opt.accumulate == 1 || (op.opt.accumulate == 0 && sizeof(T) == 8);
-  auto kern = atomic ? cg_wave_kernel<P, T, MODE, true> : cg_wave_kernel<P, T, MODE, false>;
+  auto kern = atomic ? cg_wave_kernel<P, T, MODE, true, NC> : cg_wave_kernel<P, T, MODE, false, NC>;
   if (shmem > 65536)
     SMF_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
   if (pl.n_patches > 0)
@@ -238,28 +259,36 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
                        op.d_cell_start.as<int32_t>(), op.d_own_start.as<int32_t>(),
                        op.d_ghost_start.as<int32_t>(), op.d_ghost_idx.as<int32_t>(),
                        op.d_ccol.as<uint8_t>(), op.d_lidx.as<uint16_t>(), pl.n_cells,
-                       op.d_metric.as<T>(), op.d_shape.as<T>(), src, dst, op.d_partial.as<T>());
+                       op.d_metric.as<T>(), op.d_shape.as<T>(), src, dst, op.d_partial.as<T>(), pl.n_dofs,
+                       std::max<int64_t>(pl.n_ghost_total, 1));
   const int64_t nif = pl.n_free - pl.n_int;
   const int64_t ncon = pl.n_dofs - pl.n_free;
   const int64_t nthreads = std::max(nif, ncon);
   if (nthreads > 0)
     hipLaunchKernelGGL((cg_fixup_kernel<T, MODE>), dim3((unsigned)((nthreads + 255) / 256)), dim3(256),
                        0, s, pl.n_int, nif, op.d_fix_ptr.as<int32_t>(), op.d_fix_src.as<int32_t>(),
-                       op.d_partial.as<T>(), pl.n_free, pl.n_dofs, src, dst);
+                       op.d_partial.as<T>(), pl.n_free, pl.n_dofs, src, dst, NC, pl.n_dofs,
+                       std::max<int64_t>(pl.n_ghost_total, 1));
   SMF_HIP(hipGetLastError());
+}
+
+template <typename T, int NC>
+void launch_cg_nc(const Operator& op, int mode, T* dst, const T* src, hipStream_t s) {
+  switch (op.cg.p * 2 + mode) {
+    case 2: launch_cg_t<1, T, 0, NC>(op, dst, src, s); break;
+    case 3: launch_cg_t<1, T, 1, NC>(op, dst, src, s); break;
+    case 4: launch_cg_t<2, T, 0, NC>(op, dst, src, s); break;
+    case 5: launch_cg_t<2, T, 1, NC>(op, dst, src, s); break;
+    case 6: launch_cg_t<3, T, 0, NC>(op, dst, src, s); break;
+    case 7: launch_cg_t<3, T, 1, NC>(op, dst, src, s); break;
+    default: throw std::runtime_error("unsupported CG degree");
+  }
 }
 
 template <typename T>
 void launch_cg_prec(const Operator& op, int mode, T* dst, const T* src, hipStream_t s) {
-  switch (op.cg.p * 2 + mode) {
-    case 2: launch_cg_t<1, T, 0>(op, dst, src, s); break;
-    case 3: launch_cg_t<1, T, 1>(op, dst, src, s); break;
-    case 4: launch_cg_t<2, T, 0>(op, dst, src, s); break;
-    case 5: launch_cg_t<2, T, 1>(op, dst, src, s); break;
-    case 6: launch_cg_t<3, T, 0>(op, dst, src, s); break;
-    case 7: launch_cg_t<3, T, 1>(op, dst, src, s); break;
-    default: throw std::runtime_error("unsupported CG degree");
-  }
+  if (op.ncomp == 3) launch_cg_nc<T, 3>(op, mode, dst, src, s);
+  else launch_cg_nc<T, 1>(op, mode, dst, src, s);
 }
 
 }  // namespace

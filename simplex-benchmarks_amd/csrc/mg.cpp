@@ -388,7 +388,7 @@ void vcycle(Multigrid& mg, int l, hipStream_t s) {
 }  // namespace
 
 std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions& opt) {
-  if (fine.kind != MF_CG_POISSON) throw std::runtime_error("multigrid needs a CG Poisson fine operator");
+  if (fine.kind != MF_CG_POISSON) throw std::runtime_error("multigrid needs a scalar CG Poisson fine operator");
   auto mg = std::make_unique<Multigrid>();
   mg->opt = opt;
   mg->fine_precision = fine.precision;

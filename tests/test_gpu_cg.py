@@ -104,3 +104,22 @@ def test_accumulation_variants(gpu, orc, precision, accumulate, patch_cells):
     op.vmult(b, src)
     assert np.array_equal(a.numpy(), b.numpy())
     assert rel_err(a.numpy().astype(np.float64), pr.vmult(x)) < TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_cg_vector_parity(gpu, orc, p, precision):
+    smf = gpu
+    print(f"\ncase: CG vector p={p} {precision}", flush=True)
+    op = smf.Operator(smf.Mesh.kuhn_cube(6, 0.2, 5), p, kind="cg_vector", precision=precision)
+    n = op.n_scalar_dofs()
+    pr = orc.OracleProblem.from_operator(op)
+    x = np.random.default_rng(4).uniform(-1, 1, op.m())
+    ref = np.concatenate([pr.vmult(x[c * n:(c + 1) * n]) for c in range(3)])
+    src = smf.DeviceVector.from_numpy(x.astype(op.dtype))
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, src)
+    assert rel_err(dst.numpy().astype(np.float64), ref) < TOL[precision]
+    d = op.initialize_dof_vector()
+    op.compute_diagonal(d)
+    assert rel_err(d.numpy().astype(np.float64), np.tile(pr.diagonal(), 3)) < TOL[precision]

@@ -71,3 +71,19 @@ def test_perturbation_is_seeded_and_bounded(smf):
     assert d.max() <= 0.2 / 5 + 1e-15 and d.max() > 0.15 / 5
     bd = np.any((g == 0) | (g == 1), axis=1)
     assert np.all(d[bd] == 0)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_cg_vector_host_matches_oracle(smf, orc, p, precision):
+    """Components-batched vector Laplacian = the scalar operator on each
+    component block (row c*n + d) (CG/components_batched/application.cc:1024-1216)."""
+    op = smf.Operator(smf.Mesh.kuhn_cube(4, 0.2, 5), p, kind="cg_vector", precision=precision, host_only=True)
+    n = op.n_scalar_dofs()
+    assert op.m() == 3 * n
+    pr = orc.OracleProblem.from_operator(op)
+    x = np.random.default_rng(9).uniform(-1, 1, op.m())
+    ref = np.concatenate([pr.vmult(x[c * n:(c + 1) * n]) for c in range(3)])
+    assert rel_err(op.host_apply(x).astype(float), ref) < TOL[precision]
+    d = np.tile(pr.diagonal(), 3)
+    assert rel_err(op.host_apply(None, mode=1).astype(float), d) < TOL[precision]
