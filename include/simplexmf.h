@@ -178,6 +178,80 @@ int mf_mg_level_dofs(mf_mg_t mg, int32_t level, double* xyz, uint8_t* constraine
 /* interpolant of prod sin(3 pi x_d) (kind-0 exact solution) */
 int mf_exact_solution(mf_op_t op, double* u);
 
+/* ---- learned per-cell smoother + flexible CG ------------------------------
+ * No counterpart in the reference: a new post-smoothing step on top of its
+ * MultigridPreconditioner + SolverCG (application_ph.cc:1754-2026,
+ * application_hp.cc:1950-1960).  Network 20 -> 64 -> 64 -> 10 (GELU), input
+ * = a cell's 10 scaled P2 residuals + h J^-T (9) + |det J|/h^3, output = the
+ * cell's correction in units of |r_K| * mean(D^-1 over K); FP32 master
+ * weights, BF16 MFMA inference with FP32 accumulation, FP32 training. */
+typedef struct mf_learned_smoother* mf_ls_t;
+#define MF_LS_N_INPUTS 20
+#define MF_LS_N_OUTPUTS 10
+#define MF_LS_N_PARAMS 6154 /* W1[64][20] b1[64] W2[64][64] b2[64] W3[10][64] b3[10] */
+
+typedef struct mf_train_options {
+  int32_t n_rhs;           /* random-field right-hand sides (seeds seed .. seed+n_rhs-1) */
+  int32_t pairs_per_rhs;   /* preconditioned CG iterations used per RHS */
+  int32_t steps_per_pair;  /* Adam steps per (residual, error) pair */
+  double lr;
+  int32_t n_features;
+  double corr_length;
+  uint64_t seed;
+  int32_t loss;            /* 0: energy of the remaining error |e' - w c|_A^2/|e'|_A^2 at the
+                              optimal w (the gain inside FCG); 1: mean squared e'_K/(s_K kappa_K) */
+} mf_train_options;
+
+typedef struct mf_train_stats {
+  int32_t pairs, steps;
+  double first_loss;  /* loss of the first pair before its first step (energy loss: the
+                         fraction of the error's A-energy left after the learned step) */
+  double last_loss;   /* loss of the last pair before its first step */
+  double zero_loss;   /* loss of the zero predictor on the first pair of the last RHS */
+} mf_train_stats;
+
+typedef struct mf_fcg_options {
+  double stall_ratio;          /* fall back after two iterations with |r_k| > stall_ratio |r_k-1| */
+  int32_t force_fallback_at;   /* iteration at which to force the fallback, -1: never */
+  int32_t use_learned;         /* 0: plain multigrid FCG (same code path) */
+} mf_fcg_options;
+
+typedef struct mf_fcg_stats {
+  mf_solve_stats base;
+  int32_t fallback;            /* 1 when the learned step was switched off */
+  int32_t fallback_iteration;  /* iteration of the switch, -1 if none */
+  int32_t learned_steps;       /* preconditioner applications with the learned step */
+  double mean_omega;           /* mean energy-optimal step length of the learned correction */
+} mf_fcg_stats;
+
+void mf_ls_default_train_options(mf_train_options* opt);
+void mf_ls_default_fcg_options(mf_fcg_options* opt);
+/* smoother for a CG P2 FP64 Poisson operator; weights initialised from seed */
+int mf_ls_create(mf_op_t fine, uint64_t seed, mf_ls_t* out);
+int mf_ls_destroy(mf_ls_t ls);
+/* FP32 master weights (host, MF_LS_N_PARAMS floats); set refreshes the BF16 copy */
+int mf_ls_set_params(mf_ls_t ls, const float* params, int64_t n);
+int mf_ls_get_params(mf_ls_t ls, float* params, int64_t n);
+/* versioned little-endian state blob (FP32 master, Adam moments, BF16 copy);
+ * size query with buf == NULL */
+int mf_ls_save(mf_ls_t ls, void* buf, uint64_t capacity, uint64_t* size);
+int mf_ls_load(mf_ls_t ls, const void* buf, uint64_t size);
+/* dst = assembled learned correction for the residual src (device FP64, n = mf_m) */
+int mf_ls_apply(mf_ls_t ls, void* dst, const void* src, void* stream);
+/* raw network on n rows: x (device, n x 20 FP32) -> y (device, n x 10 FP32), BF16 MFMA */
+int mf_ls_forward(mf_ls_t ls, const void* x, int64_t n, void* y, void* stream);
+/* per-cell features x (n_cells x 20) and targets t (n_cells x 10) of residual r and error e (device) */
+int mf_ls_features(mf_ls_t ls, const void* r, const void* e, void* x, void* t, void* stream);
+/* FP32 gradient (host, MF_LS_N_PARAMS) and loss of mean_rows |net(x) - t|^2 over n device rows */
+int mf_ls_gradient(mf_ls_t ls, const void* x, const void* t, int64_t n, float* grad, double* loss, void* stream);
+/* gradient on (x, t) then one Adam step (beta 0.9/0.999, eps 1e-8); loss before the step */
+int mf_ls_train_step(mf_ls_t ls, const void* x, const void* t, int64_t n, double lr, double* loss, void* stream);
+/* training on (residual, error) pairs of plain FP64-outer multigrid CG solves */
+int mf_ls_train(mf_ls_t ls, mf_mg_t mg, const mf_train_options* opt, mf_train_stats* stats, void* stream);
+/* flexible CG from x = 0: z = V(r), then z += omega L(r - A z); ls may be NULL */
+int mf_solve_fcg_learned(mf_op_t A, mf_mg_t mg, mf_ls_t ls, void* x, const void* b, double rel_tol,
+                         int32_t max_iter, const mf_fcg_options* opt, mf_fcg_stats* stats, void* stream);
+
 /* Runs the device algorithm's patch/colour schedule on the host, in the
  * operator's precision, from the same tables the kernels read (mode 0: vmult,
  * 1: diagonal).  Host vectors.  A self-check of the setup for machines without

@@ -72,6 +72,17 @@ int orc_dg_diagonal(const orc_mesh* m, int p, const int64_t* cell_dofs, int64_t 
 /* per-cell penalty tau_K (n_cells) */
 int orc_dg_tau(const orc_mesh* m, double* tau);
 
+/* ---- learned-smoother network (oracle_mlp.c) ---------------------------- */
+int orc_mlp_n_params(void);
+uint16_t orc_bf16(float f);
+/* y[n][10] of x[n][20]; bf16 != 0 rounds weights and layer inputs to bfloat16 */
+int orc_mlp_forward(const double* params, const double* x, int64_t n, int bf16, double* y);
+int orc_mlp_loss(const double* params, const double* x, const double* t, int64_t n, double* loss);
+int orc_mlp_loss_grad(const double* params, const double* x, const double* t, int64_t n, double* grad,
+                      double* loss);
+int orc_adam(double* p, const double* g, double* m, double* v, int64_t n, double lr, double b1, double b2,
+             double eps, int64_t step);
+
 #ifdef __cplusplus
 }
 #endif

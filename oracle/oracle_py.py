@@ -209,3 +209,62 @@ def tet_rule(n):
     if lib().orc_tet_rule(n, pts.ctypes.data, w.ctypes.data):
         raise RuntimeError("orc_tet_rule failed")
     return pts, w
+
+
+# ---- learned-smoother network (oracle_mlp.c) --------------------------------
+def _mlp_lib():
+    L = lib()
+    if not getattr(L, "_mlp_ready", False):
+        vp = C.c_void_p
+        L.orc_mlp_n_params.restype = C.c_int
+        L.orc_bf16.argtypes = [C.c_float]
+        L.orc_bf16.restype = C.c_uint16
+        L.orc_mlp_forward.argtypes = [vp, vp, C.c_int64, C.c_int, vp]
+        L.orc_mlp_loss.argtypes = [vp, vp, vp, C.c_int64, vp]
+        L.orc_mlp_loss_grad.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
+        L.orc_adam.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_double, C.c_double, C.c_double,
+                               C.c_double, C.c_int64]
+        L._mlp_ready = True
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def mlp_n_params():
+    return _mlp_lib().orc_mlp_n_params()
+
+
+def bf16_bits(f):
+    return int(_mlp_lib().orc_bf16(float(f)))
+
+
+def mlp_forward(params, x, bf16=False):
+    p, x = _f64(params), _f64(x).reshape(-1, 20)
+    y = np.empty((x.shape[0], 10))
+    _mlp_lib().orc_mlp_forward(p.ctypes.data, x.ctypes.data, x.shape[0], int(bf16), y.ctypes.data)
+    return y
+
+
+def mlp_loss(params, x, t):
+    p, x, t = _f64(params), _f64(x).reshape(-1, 20), _f64(t).reshape(-1, 10)
+    l = np.zeros(1)
+    _mlp_lib().orc_mlp_loss(p.ctypes.data, x.ctypes.data, t.ctypes.data, x.shape[0], l.ctypes.data)
+    return float(l[0])
+
+
+def mlp_loss_grad(params, x, t):
+    p, x, t = _f64(params), _f64(x).reshape(-1, 20), _f64(t).reshape(-1, 10)
+    g = np.empty(p.size)
+    l = np.zeros(1)
+    _mlp_lib().orc_mlp_loss_grad(p.ctypes.data, x.ctypes.data, t.ctypes.data, x.shape[0], g.ctypes.data,
+                                 l.ctypes.data)
+    return float(l[0]), g
+
+
+def adam(p, g, m, v, lr, step, b1=0.9, b2=0.999, eps=1e-8):
+    """In place on float64 arrays p, m, v."""
+    g = _f64(g)
+    _mlp_lib().orc_adam(p.ctypes.data, g.ctypes.data, m.ctypes.data, v.ctypes.data, p.size, lr, b1, b2, eps,
+                        step)

@@ -6,6 +6,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "api_handles.h"
 #include "mg.h"
 #include "op.h"
 
@@ -155,25 +156,6 @@ void host_apply_cg(const Operator& op, int mode, void* dst, const void* src) {
 
 using namespace smf;
 
-struct mf_operator {
-  Operator op;
-};
-
-#define MF_TRY(body)                      \
-  try {                                   \
-    body;                                 \
-    return 0;                             \
-  } catch (const std::exception& e) {     \
-    set_error(e.what());                  \
-    return -1;                            \
-  } catch (...) {                         \
-    set_error("unknown error");           \
-    return -1;                            \
-  }
-
-static void require_device(const Operator& op) {
-  if (!op.on_device) throw std::runtime_error("operator was created with host_only=1: no device data");
-}
 
 extern "C" {
 
@@ -398,9 +380,6 @@ int mf_get_constrained(mf_op_t h, uint8_t* mask) {
   });
 }
 
-struct mf_multigrid {
-  std::unique_ptr<Multigrid> mg;
-};
 
 void mf_mg_default_options(mf_mg_options* opt) {
   MGOptions d;

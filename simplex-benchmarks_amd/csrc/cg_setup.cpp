@@ -36,7 +36,14 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   plan.n_cells = mesh.nc;
   const int64_t nc = mesh.nc, nv = mesh.nv;
 
-  const int patch = opt.patch_cells > 0 ? opt.patch_cells : (p == 1 ? 1024 : (p == 2 ? 512 : 128));
+  // default patch size per degree, halved (down to 64 cells) until there
+  // are >= 2048 patches, so that small operators (multigrid levels) still
+  // give every SIMD of the 256 CUs a wave
+  int patch = opt.patch_cells;
+  if (patch <= 0) {
+    patch = p == 1 ? 1024 : (p == 2 ? 512 : 128);
+    while (patch > 64 && mesh.nc / patch < 2048) patch /= 2;
+  }
   if (patch > 4096) throw std::runtime_error("patch_cells must be <= 4096");
   plan.max_patch_cells = 0;
 

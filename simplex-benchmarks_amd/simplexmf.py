@@ -111,6 +111,22 @@ def lib():
         "mf_solve_cg": (C.c_int, [vp, vp, vp, vp, C.c_double, i32, i32, C.POINTER(SolveStats), vp]),
         "mf_assemble_rhs": (C.c_int, [vp, i32, u64, i32, C.c_double, vp]),
         "mf_exact_solution": (C.c_int, [vp, vp]),
+        "mf_ls_default_train_options": (None, [C.POINTER(TrainOptions)]),
+        "mf_ls_default_fcg_options": (None, [C.POINTER(FcgOptions)]),
+        "mf_ls_create": (C.c_int, [vp, u64, C.POINTER(vp)]),
+        "mf_ls_destroy": (C.c_int, [vp]),
+        "mf_ls_set_params": (C.c_int, [vp, vp, i64]),
+        "mf_ls_get_params": (C.c_int, [vp, vp, i64]),
+        "mf_ls_save": (C.c_int, [vp, vp, u64, C.POINTER(u64)]),
+        "mf_ls_load": (C.c_int, [vp, vp, u64]),
+        "mf_ls_apply": (C.c_int, [vp, vp, vp, vp]),
+        "mf_ls_forward": (C.c_int, [vp, vp, i64, vp, vp]),
+        "mf_ls_features": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+        "mf_ls_gradient": (C.c_int, [vp, vp, vp, i64, vp, C.POINTER(C.c_double), vp]),
+        "mf_ls_train_step": (C.c_int, [vp, vp, vp, i64, C.c_double, C.POINTER(C.c_double), vp]),
+        "mf_ls_train": (C.c_int, [vp, vp, C.POINTER(TrainOptions), C.POINTER(TrainStats), vp]),
+        "mf_solve_fcg_learned": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, i32, C.POINTER(FcgOptions),
+                                           C.POINTER(FcgStats), vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -144,6 +160,37 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TrainOptions(C.Structure):
+    _fields_ = [("n_rhs", C.c_int32), ("pairs_per_rhs", C.c_int32), ("steps_per_pair", C.c_int32),
+                ("lr", C.c_double), ("n_features", C.c_int32), ("corr_length", C.c_double),
+                ("seed", C.c_uint64), ("loss", C.c_int32)]
+
+
+class TrainStats(C.Structure):
+    _fields_ = [("pairs", C.c_int32), ("steps", C.c_int32), ("first_loss", C.c_double),
+                ("last_loss", C.c_double), ("zero_loss", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FcgOptions(C.Structure):
+    _fields_ = [("stall_ratio", C.c_double), ("force_fallback_at", C.c_int32), ("use_learned", C.c_int32)]
+
+
+class FcgStats(C.Structure):
+    _fields_ = [("base", SolveStats), ("fallback", C.c_int32), ("fallback_iteration", C.c_int32),
+                ("learned_steps", C.c_int32), ("mean_omega", C.c_double)]
+
+    def as_dict(self):
+        d = self.base.as_dict()
+        d.update({k: getattr(self, k) for k, _ in self._fields_[1:]})
+        return d
+
+
+LS_N_INPUTS, LS_N_OUTPUTS, LS_N_PARAMS = 20, 10, 6154
+
+
 class MFError(RuntimeError):
     pass
 
@@ -169,6 +216,9 @@ _SIGNATURE_NAMES = [
     "mf_event_create", "mf_event_destroy", "mf_event_record", "mf_event_elapsed_ms",
     "mf_mg_default_options", "mf_mg_create", "mf_mg_vmult", "mf_mg_get_info", "mf_mg_destroy",
     "mf_mg_transfer", "mf_mg_level_dofs", "mf_solve_cg", "mf_assemble_rhs", "mf_exact_solution",
+    "mf_ls_default_train_options", "mf_ls_default_fcg_options", "mf_ls_create", "mf_ls_destroy",
+    "mf_ls_set_params", "mf_ls_get_params", "mf_ls_save", "mf_ls_load", "mf_ls_apply", "mf_ls_forward",
+    "mf_ls_features", "mf_ls_gradient", "mf_ls_train_step", "mf_ls_train", "mf_solve_fcg_learned",
 ]
 
 
@@ -479,3 +529,95 @@ def exact_solution(op: Operator):
     u = np.empty(op.m())
     check(lib().mf_exact_solution(op.handle, u.ctypes.data))
     return u
+
+
+# --------------------------------------------------------------------------
+# learned per-cell smoother and flexible CG
+
+class LearnedSmoother:
+    """Per-cell MLP smoother of a CG P2 FP64 Poisson operator (device weights)."""
+
+    def __init__(self, fine: Operator, seed=0):
+        self._lib = lib()
+        self.op = fine
+        h = C.c_void_p()
+        check(self._lib.mf_ls_create(fine.handle, seed, C.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._lib.mf_ls_destroy(self.handle)
+            self.handle = None
+
+    @property
+    def n_cells(self):
+        return self.op.info["n_cells"]
+
+    def params(self):
+        p = np.empty(LS_N_PARAMS, np.float32)
+        check(self._lib.mf_ls_get_params(self.handle, p.ctypes.data, LS_N_PARAMS))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        check(self._lib.mf_ls_set_params(self.handle, p.ctypes.data, p.size))
+
+    def save(self) -> bytes:
+        n = C.c_uint64()
+        check(self._lib.mf_ls_save(self.handle, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(self._lib.mf_ls_save(self.handle, buf, n.value, C.byref(n)))
+        return buf.raw[:n.value]
+
+    def load(self, blob: bytes):
+        check(self._lib.mf_ls_load(self.handle, blob, len(blob)))
+
+    def apply(self, dst, src, stream=None):
+        check(self._lib.mf_ls_apply(self.handle, dst.ptr, src.ptr, stream.handle if stream else None))
+
+    def forward(self, x: DeviceVector, n, y: DeviceVector, stream=None):
+        check(self._lib.mf_ls_forward(self.handle, x.ptr, n, y.ptr, stream.handle if stream else None))
+
+    def features(self, r, e=None, stream=None):
+        nc = self.n_cells
+        x = DeviceVector(nc * LS_N_INPUTS, np.float32)
+        t = DeviceVector(nc * LS_N_OUTPUTS, np.float32) if e is not None else None
+        check(self._lib.mf_ls_features(self.handle, r.ptr, e.ptr if e is not None else None, x.ptr,
+                                       t.ptr if t is not None else None, stream.handle if stream else None))
+        return x, t
+
+    def gradient(self, x: DeviceVector, t: DeviceVector, n, stream=None):
+        g = np.empty(LS_N_PARAMS, np.float32)
+        loss = C.c_double()
+        check(self._lib.mf_ls_gradient(self.handle, x.ptr, t.ptr, n, g.ctypes.data, C.byref(loss),
+                                       stream.handle if stream else None))
+        return loss.value, g
+
+    def train_step(self, x, t, n, lr=1e-3, stream=None):
+        loss = C.c_double()
+        check(self._lib.mf_ls_train_step(self.handle, x.ptr, t.ptr, n, lr, C.byref(loss),
+                                         stream.handle if stream else None))
+        return loss.value
+
+    def train(self, mg: Multigrid, stream=None, **kw):
+        o = TrainOptions()
+        self._lib.mf_ls_default_train_options(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        st = TrainStats()
+        check(self._lib.mf_ls_train(self.handle, mg.handle, C.byref(o), C.byref(st),
+                                    stream.handle if stream else None))
+        return st.as_dict()
+
+
+def solve_fcg_learned(A: Operator, x: DeviceVector, b: DeviceVector, mg: Multigrid,
+                      ls: LearnedSmoother | None = None, rel_tol=1e-10, max_iter=1000, stream=None, **kw):
+    o = FcgOptions()
+    lib().mf_ls_default_fcg_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    st = FcgStats()
+    check(lib().mf_solve_fcg_learned(A.handle, mg.handle, ls.handle if ls is not None else None, x.ptr, b.ptr,
+                                     rel_tol, max_iter, C.byref(o), C.byref(st),
+                                     stream.handle if stream else None))
+    return st.as_dict()

@@ -1,0 +1,66 @@
+#!/usr/bin/env python3
+"""Trains the learned per-cell smoother on the GPU and writes its state blob.
+
+Training data: (residual, error) pairs of plain multigrid-preconditioned CG
+solves (FP64 outer CG, FP32 V-cycle with Chebyshev of the given degree) for
+seeded Gaussian-random-field right-hand sides on a perturbed Kuhn mesh; the
+loss is the fraction of the post-V-cycle error's energy left after the
+learned step.  The network's inputs are scale-free (h J^-T, |det J|/h^3,
+normalised residuals), so weights trained on a small mesh serve larger ones.
+
+    python simplex-benchmarks_amd/train_smoother.py --n 16 --rhs 4096 --degree 2
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=16, help="hexes per cube edge of the training mesh")
+    ap.add_argument("--rhs", type=int, default=4096, help="random-field right-hand sides")
+    ap.add_argument("--pairs", type=int, default=12, help="CG iterations (pairs) per right-hand side")
+    ap.add_argument("--steps", type=int, default=2, help="Adam steps per pair")
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--degree", type=int, default=2, help="Chebyshev degree of the V-cycle")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--init-seed", type=int, default=3)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    import simplexmf as smf
+
+    out = args.out or os.path.join(HERE, "weights", f"learned_smoother_p2_cheb{args.degree}.bin")
+    mesh = smf.Mesh.kuhn_cube(args.n, perturb=0.2, seed=7)
+    A = smf.Operator(mesh, 2, kind="cg", precision="f64")
+    mg = smf.Multigrid(A, smoother_degree=args.degree)
+    ls = smf.LearnedSmoother(A, seed=args.init_seed)
+    t0 = time.time()
+    done, stats = 0, []
+    while done < args.rhs:  # chunks, so that progress shows
+        k = min(256, args.rhs - done)
+        st = ls.train(mg, n_rhs=k, pairs_per_rhs=args.pairs, steps_per_pair=args.steps, lr=args.lr,
+                      seed=args.seed + done, loss=0)
+        done += k
+        stats.append(st)
+        print(f"rhs {done}/{args.rhs}: energy left {st['first_loss']:.3f} -> {st['last_loss']:.3f} "
+              f"({time.time() - t0:.0f} s)", flush=True)
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    with open(out, "wb") as f:
+        f.write(ls.save())
+    meta = {"n": args.n, "rhs": args.rhs, "pairs_per_rhs": args.pairs, "steps_per_pair": args.steps,
+            "lr": args.lr, "chebyshev_degree": args.degree, "seed": args.seed, "init_seed": args.init_seed,
+            "loss": "energy fraction left after the learned step",
+            "first_loss": stats[0]["first_loss"], "last_loss": stats[-1]["last_loss"],
+            "seconds": round(time.time() - t0, 1)}
+    with open(out.replace(".bin", ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta))
+
+
+if __name__ == "__main__":
+    main()
