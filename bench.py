@@ -55,7 +55,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--mode", default="vmult", choices=["vmult", "solve"])
-    ap.add_argument("--solves", type=int, default=5, help="repeated solves in --mode solve")
+    ap.add_argument("--solves", type=int, default=5, help="repeated solves of the solver comparison")
+    ap.add_argument("--solve-n", type=int, default=48, help="mesh of the solver comparison (0: skip it)")
     return ap.parse_args()
 
 
@@ -87,43 +88,104 @@ def cpu_baseline(op, x, seconds):
                       f"{(op.info['degree'] + 2) ** 3} points per cell"}
 
 
-def solve_mode(args):
-    """Mixed-precision hybrid-MG-preconditioned CG (FP64 outer CG, FP32
-    V-cycle) on the CG Poisson problem; metrics of application_hp.cc:1954-1988."""
-    import simplexmf as smf
-
-    t0 = time.time()
-    mesh = smf.Mesh.kuhn_cube(args.n, 0.2, args.seed)
-    op = smf.Operator(mesh, args.degree, kind="cg", precision="f64")
-    mg = smf.Multigrid(op)
-    b = smf.DeviceVector.from_numpy(smf.assemble_rhs(op, "grf", seed=args.seed, n_features=256,
-                                                     corr_length=0.1))
-    log(f"setup {time.time() - t0:.1f}s levels {mg.info()}")
-    x = op.initialize_dof_vector()
-    stream = smf.Stream()
-    st = smf.solve_cg(op, x, b, mg, rel_tol=1e-10, stream=stream)  # warm-up
+def _time_solves(smf, fn, solves):
+    fn()  # warm-up
     times = []
-    for _ in range(args.solves):
+    for _ in range(solves):
         smf.synchronize()
         t = time.perf_counter()
-        st = smf.solve_cg(op, x, b, mg, rel_tol=1e-10, stream=stream)
+        st = fn()
         smf.synchronize()
         times.append(time.perf_counter() - t)
+    return st, times
+
+
+def _rates(st, tmin, dofs):
     n = st["iterations"]
     rho = (st["final_residual"] / st["initial_residual"]) ** (1.0 / n)
     n10 = -10 * np.log(10) / np.log(rho)
-    tmin = min(times)
     t10 = tmin * n10 / n
+    return {"iterations": n, "rho": rho, "n10": n10, "t10_s": t10, "E10_dofs_per_s": dofs / t10}
+
+
+def solve_compare(smf, n, degree, seed, solves, learned_degree=3):
+    """Time to 1e-10 of mixed-precision hybrid-MG CG (FP64 outer CG, FP32
+    V-cycle, Chebyshev(5)-Jacobi: the reference's MultigridPreconditioner +
+    SolverCG, application_ph.cc:1754-2026, metrics of application_hp.cc:
+    1954-1988) against flexible CG with the learned post-smoothing step
+    behind a Chebyshev(learned_degree) V-cycle, on a Gaussian-random-field
+    right-hand side.  Errors against an FP64 reference solve to 1e-13."""
+    t0 = time.time()
+    mesh = smf.Mesh.kuhn_cube(n, 0.2, seed)
+    op = smf.Operator(mesh, degree, kind="cg", precision="f64")
+    mg = smf.Multigrid(op)
+    b = smf.DeviceVector.from_numpy(smf.assemble_rhs(op, "grf", seed=seed, n_features=256, corr_length=0.1))
+    x = op.initialize_dof_vector()
+    stream = smf.Stream()
+    smf.solve_cg(op, x, b, mg, rel_tol=1e-13, max_iter=300, stream=stream)
+    xref = x.numpy()
+
+    def errors(xv):
+        e = smf.DeviceVector.from_numpy(xv.numpy() - xref)
+        ae = op.initialize_dof_vector()
+        op.vmult(ae, e)
+        scale = np.max(np.abs(xref))
+        return {"max_rel": float(np.max(np.abs(e.numpy())) / scale),
+                "energy": float(np.sqrt(max(np.dot(e.numpy(), ae.numpy()), 0.0)))}
+
+    out = {"workload": f"MG-CG Poisson P{degree}, {n}^3 x 6 perturbed Kuhn tets, FP64 CG + FP32 V-cycle, "
+                       f"GRF right-hand side (l=0.1), stop at 1e-10", "dofs": op.m(),
+           "levels": mg.info()["n_dofs"], "solves": solves}
+    st, times = _time_solves(smf, lambda: smf.solve_cg(op, x, b, mg, rel_tol=1e-10, stream=stream), solves)
+    plain = {"time_s": min(times), "mean_s": float(np.mean(times)), **_rates(st, min(times), op.m()),
+             "error": errors(x), "smoother": "Chebyshev(5)-Jacobi"}
+    out["plain"] = plain
+    log(f"plain MG-CG {n}^3: {plain['iterations']} it, {plain['time_s'] * 1e3:.2f} ms")
+    if degree == 2:
+        mgl = smf.Multigrid(op, smoother_degree=learned_degree)
+        ls = smf.LearnedSmoother(op, seed=0)
+        wpath = os.path.join(ROOT, "simplex-benchmarks_amd", "weights",
+                             f"learned_smoother_p2_cheb{learned_degree}.bin")
+        if os.path.exists(wpath):
+            with open(wpath, "rb") as f:
+                ls.load(f.read())
+            origin = os.path.relpath(wpath, ROOT)
+        else:
+            ls.train(mgl, n_rhs=256, pairs_per_rhs=12, steps_per_pair=2, loss=0)
+            origin = "trained in this run on the bench mesh (256 right-hand sides)"
+        st, times = _time_solves(smf, lambda: smf.solve_fcg_learned(op, x, b, mgl, ls, 1e-10, 300, stream=stream),
+                                 solves)
+        learned = {"time_s": min(times), "mean_s": float(np.mean(times)), **_rates(st, min(times), op.m()),
+                   "error": errors(x), "fallback": st["fallback"], "learned_steps": st["learned_steps"],
+                   "mean_omega": st["mean_omega"], "smoother": f"Chebyshev({learned_degree})-Jacobi + learned step",
+                   "weights": origin}
+        st, times = _time_solves(smf, lambda: smf.solve_cg(op, x, b, mgl, rel_tol=1e-10, stream=stream), solves)
+        ablation = {"time_s": min(times), **_rates(st, min(times), op.m()),
+                    "smoother": f"Chebyshev({learned_degree})-Jacobi, no learned step"}
+        out["learned"] = learned
+        out["plain_same_vcycle"] = ablation
+        out["speedup_learned_vs_plain"] = plain["time_s"] / learned["time_s"]
+        out["error_ratio_energy"] = learned["error"]["energy"] / max(plain["error"]["energy"], 1e-300)
+        log(f"learned FCG {n}^3: {learned['iterations']} it, {learned['time_s'] * 1e3:.2f} ms, "
+            f"speedup {out['speedup_learned_vs_plain']:.3f}")
+    log(f"solve comparison {time.time() - t0:.1f}s")
+    return out
+
+
+def solve_mode(args):
+    """One JSON line for the solver comparison alone (value: plain MG-CG
+    seconds to 1e-10)."""
+    import simplexmf as smf
+
+    r = solve_compare(smf, args.solve_n, args.degree, args.seed, args.solves)
+    p = r["plain"]
     out = {
-        "metric": METRIC, "value": round(tmin, 6), "unit": "s to 1e-10 (min over solves)",
-        "n_gpus": 1, "steps": args.solves, "warmup": 1, "ms_per_step": round(tmin * 1e3, 3),
+        "metric": METRIC, "value": round(p["time_s"], 6), "unit": "s to 1e-10 (min over solves)",
+        "n_gpus": 1, "steps": args.solves, "warmup": 1, "ms_per_step": round(p["time_s"] * 1e3, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
         "data": "synthetic: seeded perturbed Kuhn mesh, Gaussian random field RHS (l=0.1)",
-        "config": {"workload": f"MG-CG Poisson P{args.degree}, {args.n}^3 x 6 perturbed Kuhn tets, "
-                               f"FP64 CG + FP32 V-cycle, stop 1e-10",
-                   "dofs": op.m(), "levels": mg.info()["n_dofs"]},
-        "iterations": n, "rho": rho, "n10": n10, "t10_s": t10, "E10_dofs_per_s": op.m() / t10,
-        "mean_s": float(np.mean(times)),
+        "config": {"workload": r["workload"], "dofs": r["dofs"], "levels": r["levels"]},
+        "solve": r,
     }
     print(json.dumps(out), flush=True)
 
@@ -275,6 +337,8 @@ def main():
         "cpu_baseline": base,
         "variants": variants,
     }
+    if args.solve_n > 0:
+        out["solve"] = solve_compare(smf, args.solve_n, 2, args.seed, args.solves)
     print(json.dumps(out), flush=True)
 
 

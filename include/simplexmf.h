@@ -200,6 +200,9 @@ typedef struct mf_train_options {
   uint64_t seed;
   int32_t loss;            /* 0: energy of the remaining error |e' - w c|_A^2/|e'|_A^2 at the
                               optimal w (the gain inside FCG); 1: mean squared e'_K/(s_K kappa_K) */
+  int32_t trajectory;      /* iterates the pairs come from: 0 plain multigrid PCG, 1 flexible CG
+                              with the current learned step (errors still against the plain
+                              FP64 multigrid solution) */
 } mf_train_options;
 
 typedef struct mf_train_stats {

@@ -31,6 +31,7 @@ void mf_ls_default_train_options(mf_train_options* opt) {
   opt->corr_length = d.corr_length;
   opt->seed = d.seed;
   opt->loss = d.loss;
+  opt->trajectory = d.trajectory;
 }
 
 void mf_ls_default_fcg_options(mf_fcg_options* opt) {
@@ -157,6 +158,7 @@ int mf_ls_train(mf_ls_t h, mf_mg_t mg, const mf_train_options* opt, mf_train_sta
       o.corr_length = opt->corr_length;
       o.seed = opt->seed;
       o.loss = opt->loss;
+      o.trajectory = opt->trajectory;
     }
     if (o.n_features < 1 || !(o.corr_length > 0)) throw std::runtime_error("invalid random field parameters");
     const TrainStats st = ls_train(get(h), *mg->mg, o, static_cast<hipStream_t>(stream));

@@ -101,10 +101,20 @@ std::unique_ptr<LearnedSmoother> make_learned_smoother(const Operator& op, uint6
     }
     kappa[k] = cnt ? (float)(s / cnt) : 0.0f;
   }
-  ls->d_cell_dofs.upload(cd);
+  {
+    // component-major copies for coalesced per-cell loads: [i][cell]
+    std::vector<int32_t> cdt(nc * kMlpOut);
+    std::vector<float> geot(nc * 10);
+    for (int64_t k = 0; k < nc; ++k)
+      for (int i = 0; i < kMlpOut; ++i) {
+        cdt[(size_t)i * nc + k] = cd[k * kMlpOut + i];
+        geot[(size_t)i * nc + k] = geo[k * 10 + i];
+      }
+    ls->d_cell_dofs.upload(cdt);
+    ls->d_geo.upload(geot);
+  }
   ls->d_slot_ptr.upload(ptr);
   ls->d_slot.upload(slot);
-  ls->d_geo.upload(geo);
   ls->d_kappa.upload(kappa);
   ls->d_x.alloc(nc * kMlpIn * sizeof(float));
   ls->d_t.alloc(nc * kMlpOut * sizeof(float));
@@ -261,11 +271,11 @@ TrainStats ls_train(LearnedSmoother& ls, Multigrid& mg, const TrainOptions& opt,
   const Operator& A = *ls.op;
   if (mg.n_fine != A.m()) throw std::runtime_error("multigrid does not match the smoother's operator");
   if (opt.n_rhs < 1 || opt.pairs_per_rhs < 1 || opt.steps_per_pair < 1 || !(opt.lr > 0) ||
-      (opt.loss != 0 && opt.loss != 1))
+      (opt.loss != 0 && opt.loss != 1) || (opt.trajectory != 0 && opt.trajectory != 1))
     throw std::runtime_error("invalid training options");
   const int64_t n = A.m(), nc = ls.n_cells;
-  DevBuf b, xs, x, r, z, p, q, rp, e;
-  for (DevBuf* v : {&b, &xs, &x, &r, &z, &p, &q, &rp, &e}) v->alloc(n * sizeof(double));
+  DevBuf b, xs, x, r, z, p, q, rp, e, az, c, ac;
+  for (DevBuf* v : {&b, &xs, &x, &r, &z, &p, &q, &rp, &e, &az, &c, &ac}) v->alloc(n * sizeof(double));
   TrainStats st;
   std::vector<double> bh;
   std::vector<float> th(nc * kMlpOut);
@@ -280,17 +290,18 @@ TrainStats ls_train(LearnedSmoother& ls, Multigrid& mg, const TrainOptions& opt,
     SMF_HIP(hipMemcpyAsync(b.p, bh.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
     // reference solution by plain preconditioned CG (FP64 outer iteration)
     solve_cg(A, &mg, xs.as<double>(), b.as<double>(), 1e-12, 100, false, s);
-    // replay plain PCG from x = 0; each preconditioner application gives
+    // replay CG from x = 0 (plain PCG, or flexible CG with the learned step
+    // for opt.trajectory == 1); each preconditioner application gives
     // r' = r - A V(r) and e' = (x* - x) - V(r)
     const double bn = host_dot(ls.d_partial, ls.d_scalar, n, b.as<double>(), b.as<double>(), s);
     blas_fill<double>(n, 0.0, x.as<double>(), s);
     copy_d(r.as<double>(), b.as<double>(), n, s);
-    double rz = 0;
+    double rz = 0, alpha = 0;
     for (int it = 0; it < opt.pairs_per_rhs; ++it) {
       mg_vmult(mg, z.p, r.p, s);
-      launch_cg(A, 0, q.p, z.p, s);
+      launch_cg(A, 0, az.p, z.p, s);
       copy_d(rp.as<double>(), r.as<double>(), n, s);
-      blas_axpby<double>(n, -1.0, q.as<double>(), 1.0, rp.as<double>(), s);
+      blas_axpby<double>(n, -1.0, az.as<double>(), 1.0, rp.as<double>(), s);
       copy_d(e.as<double>(), xs.as<double>(), n, s);
       blas_axpby<double>(n, -1.0, x.as<double>(), 1.0, e.as<double>(), s);
       blas_axpby<double>(n, -1.0, z.as<double>(), 1.0, e.as<double>(), s);
@@ -326,14 +337,30 @@ TrainStats ls_train(LearnedSmoother& ls, Multigrid& mg, const TrainOptions& opt,
         }
       }
       st.pairs++;
-      // PCG step
+      if (opt.trajectory == 1) {
+        // the learned step as the solver applies it: z += omega c
+        ls_apply(ls, c.as<double>(), rp.as<double>(), s);
+        launch_cg(A, 0, ac.p, c.p, s);
+        double d[2];
+        host_dot2(ls.d_partial, ls.d_scalar, n, c.as<double>(), rp.as<double>(), c.as<double>(), ac.as<double>(), d,
+                  s);
+        const double omega = d[0] / d[1];
+        if (d[1] > 0 && std::isfinite(omega)) blas_axpby<double>(n, omega, c.as<double>(), 1.0, z.as<double>(), s);
+      }
+      // (F)CG step
       const double rz_new = host_dot(ls.d_partial, ls.d_scalar, n, r.as<double>(), z.as<double>(), s);
-      if (it == 0) copy_d(p.as<double>(), z.as<double>(), n, s);
-      else blas_axpby<double>(n, 1.0, z.as<double>(), rz_new / rz, p.as<double>(), s);
+      if (it == 0) {
+        copy_d(p.as<double>(), z.as<double>(), n, s);
+      } else {
+        const double beta = opt.trajectory == 1
+                                ? -alpha * host_dot(ls.d_partial, ls.d_scalar, n, z.as<double>(), q.as<double>(), s) / rz
+                                : rz_new / rz;
+        blas_axpby<double>(n, 1.0, z.as<double>(), beta, p.as<double>(), s);
+      }
       rz = rz_new;
       launch_cg(A, 0, q.p, p.p, s);
       const double pq = host_dot(ls.d_partial, ls.d_scalar, n, p.as<double>(), q.as<double>(), s);
-      const double alpha = rz / pq;
+      alpha = rz / pq;
       blas_axpby<double>(n, alpha, p.as<double>(), 1.0, x.as<double>(), s);
       blas_axpby<double>(n, -alpha, q.as<double>(), 1.0, r.as<double>(), s);
       const double rn = host_dot(ls.d_partial, ls.d_scalar, n, r.as<double>(), r.as<double>(), s);

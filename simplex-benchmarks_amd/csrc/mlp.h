@@ -34,8 +34,8 @@ struct LearnedSmoother {
   double h = 1.0;
   DevBuf d_params, d_m, d_v;     // FP32 master weights and Adam moments
   DevBuf d_frag;                 // BF16 MFMA fragments (kFrags x 64 lanes x 4 bf16)
-  DevBuf d_cell_dofs;            // n_cells x 10 (int32), -1 for Dirichlet DoFs
-  DevBuf d_geo;                  // n_cells x 10 features (FP32)
+  DevBuf d_cell_dofs;            // 10 x n_cells (int32, component-major), -1 for Dirichlet DoFs
+  DevBuf d_geo;                  // 10 x n_cells geometry features (FP32, component-major)
   DevBuf d_kappa;                // n_cells (FP32)
   DevBuf d_slot_ptr, d_slot;     // DoF -> (cell*10+i) slots, CSR
   DevBuf d_x, d_t, d_scale;      // per-cell features, targets, residual norms
@@ -87,6 +87,7 @@ struct TrainOptions {
   int n_features = 256;
   double corr_length = 0.1;
   uint64_t seed = 1;
+  int trajectory = 0;        // iterates the pairs come from: 0 plain PCG, 1 FCG with the learned step
 };
 struct TrainStats {
   int pairs = 0, steps = 0;

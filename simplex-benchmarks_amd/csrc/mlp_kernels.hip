@@ -28,6 +28,18 @@ __device__ inline short to_bf16(float f) {
   return (short)(u >> 16);
 }
 
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+// four floats -> four bf16 (round to nearest even, v_cvt_pk_bf16_f32)
+__device__ inline bf16x4 pack_bf16(float a0, float a1, float a2, float a3) {
+  const bf16x2_t lo = __builtin_convertvector(f32x2{a0, a1}, bf16x2_t);
+  const bf16x2_t hi = __builtin_convertvector(f32x2{a2, a3}, bf16x2_t);
+  const u32x2 w = {__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)};
+  return __builtin_bit_cast(bf16x4, w);
+}
+
 __device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
 __device__ inline float gelu_grad(float x) {
   return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.3989422804014327f * expf(-0.5f * x * x);
@@ -43,18 +55,18 @@ __global__ void ls_features_kernel(int64_t ncells, const int32_t* __restrict__ c
   if (c >= ncells) return;
   double rv[kMlpOut], ss = 0.0;
   for (int i = 0; i < kMlpOut; ++i) {
-    const int32_t d = cdofs[c * kMlpOut + i];
+    const int32_t d = cdofs[i * ncells + c];
     rv[i] = d >= 0 ? r[d] : 0.0;
     ss += rv[i] * rv[i];
   }
   const double sn = sqrt(ss) + 1e-300;
   for (int i = 0; i < kMlpOut; ++i) x[c * kMlpIn + i] = (float)(rv[i] / sn);
-  for (int i = 0; i < 10; ++i) x[c * kMlpIn + kMlpOut + i] = geo[c * 10 + i];
+  for (int i = 0; i < 10; ++i) x[c * kMlpIn + kMlpOut + i] = geo[i * ncells + c];
   if (scale) scale[c] = (float)sn;
   if (t) {
     const double den = sn * (double)kappa[c];
     for (int i = 0; i < kMlpOut; ++i) {
-      const int32_t d = cdofs[c * kMlpOut + i];
+      const int32_t d = cdofs[i * ncells + c];
       t[c * kMlpOut + i] = d >= 0 ? (float)(e[d] / den) : 0.0f;
     }
   }
@@ -118,7 +130,7 @@ __global__ __launch_bounds__(64) void ls_infer_kernel(InferArgs g) {
     if (FUSED) {
       double rv[kMlpOut], sq = 0.0;
       int32_t dd[kMlpOut];
-      for (int i = 0; i < kMlpOut; ++i) dd[i] = act ? g.cdofs[cell * kMlpOut + i] : -1;
+      for (int i = 0; i < kMlpOut; ++i) dd[i] = act ? g.cdofs[i * ncells + cell] : -1;
       for (int i = 0; i < kMlpOut; ++i) {
         rv[i] = dd[i] >= 0 ? g.r[dd[i]] - (g.az ? g.az[dd[i]] : 0.0) : 0.0;
         sq += rv[i] * rv[i];
@@ -126,7 +138,7 @@ __global__ __launch_bounds__(64) void ls_infer_kernel(InferArgs g) {
       const double sn = sqrt(sq) + 1e-300;
       const double inv = 1.0 / sn;
       for (int i = 0; i < kMlpOut; ++i) xs[l][i] = (float)(rv[i] * inv);
-      for (int i = 0; i < 10; ++i) xs[l][kMlpOut + i] = act ? g.geo[cell * 10 + i] : 0.0f;
+      for (int i = 0; i < 10; ++i) xs[l][kMlpOut + i] = act ? g.geo[i * ncells + cell] : 0.0f;
       ss[l] = act ? (float)sn * g.kappa[cell] : 0.0f;
       if (act) {
         if (g.x_out)
@@ -144,11 +156,9 @@ __global__ __launch_bounds__(64) void ls_infer_kernel(InferArgs g) {
       f32x4 acc[4];
       for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0, 0, 0, 0};
       for (int ks = 0; ks < 2; ++ks) {
-        bf16x4 a;
-        for (int j = 0; j < 4; ++j) {
-          const int k = ks * 16 + kq + j;
-          a[j] = to_bf16(k < kMlpIn ? xs[r0 + row][k] : 0.0f);
-        }
+        const int k = ks * 16 + kq;  // k..k+3 all below or all above kMlpIn (20 = 5 x 4)
+        const float* xr = xs[r0 + row];
+        const bf16x4 a = k < kMlpIn ? pack_bf16(xr[k], xr[k + 1], xr[k + 2], xr[k + 3]) : bf16x4{0, 0, 0, 0};
         for (int nb = 0; nb < 4; ++nb)
           acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B1[ks * 4 + nb], acc[nb], 0, 0, 0);
       }
@@ -158,8 +168,8 @@ __global__ __launch_bounds__(64) void ls_infer_kernel(InferArgs g) {
       // layer 2
       for (int nb = 0; nb < 4; ++nb) acc[nb] = f32x4{0, 0, 0, 0};
       for (int ks = 0; ks < 4; ++ks) {
-        bf16x4 a;
-        for (int j = 0; j < 4; ++j) a[j] = to_bf16(hb[row][ks * 16 + kq + j]);
+        const float* hr = &hb[row][ks * 16 + kq];
+        const bf16x4 a = pack_bf16(hr[0], hr[1], hr[2], hr[3]);
         for (int nb = 0; nb < 4; ++nb)
           acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B2[ks * 4 + nb], acc[nb], 0, 0, 0);
       }
@@ -170,8 +180,8 @@ __global__ __launch_bounds__(64) void ls_infer_kernel(InferArgs g) {
       // layer 3: N = 10 padded to 16
       f32x4 o = f32x4{0, 0, 0, 0};
       for (int ks = 0; ks < 4; ++ks) {
-        bf16x4 a;
-        for (int j = 0; j < 4; ++j) a[j] = to_bf16(hb[row][ks * 16 + kq + j]);
+        const float* hr = &hb[row][ks * 16 + kq];
+        const bf16x4 a = pack_bf16(hr[0], hr[1], hr[2], hr[3]);
         o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B3[ks], o, 0, 0, 0);
       }
       __syncthreads();
@@ -192,7 +202,16 @@ __global__ void ls_assemble_kernel(int64_t ndofs, const int64_t* __restrict__ pt
   if (d >= ndofs) return;
   const int32_t k0 = (int32_t)ptr[d], k1 = (int32_t)ptr[d + 1];
   float s = 0.0f;
-  for (int32_t k = k0; k < k1; ++k) s += corr[slot[k]];
+  int32_t k = k0;
+  for (; k + 4 <= k1; k += 4) {  // four independent gathers in flight
+    const int32_t a = slot[k], b = slot[k + 1], c = slot[k + 2], e = slot[k + 3];
+    const float va = corr[a], vb = corr[b], vc = corr[c], ve = corr[e];
+    s += va;
+    s += vb;
+    s += vc;
+    s += ve;
+  }
+  for (; k < k1; ++k) s += corr[slot[k]];
   out[d] = k1 > k0 ? (double)(s * __builtin_amdgcn_rcpf((float)(k1 - k0))) : 0.0;
 }
 
@@ -357,7 +376,7 @@ __global__ void ls_dy_kernel(int64_t ncells, const int32_t* __restrict__ cdofs, 
   if (c >= ncells) return;
   const double f = coef * (double)scale[c] * (double)kappa[c];
   for (int i = 0; i < kMlpOut; ++i) {
-    const int32_t d = cdofs[c * kMlpOut + i];
+    const int32_t d = cdofs[i * ncells + c];
     dy[c * kMlpOut + i] = d >= 0 ? (float)(f * gc[d] / (double)(ptr[d + 1] - ptr[d])) : 0.0f;
   }
 }

@@ -163,7 +163,7 @@ class SolveStats(C.Structure):
 class TrainOptions(C.Structure):
     _fields_ = [("n_rhs", C.c_int32), ("pairs_per_rhs", C.c_int32), ("steps_per_pair", C.c_int32),
                 ("lr", C.c_double), ("n_features", C.c_int32), ("corr_length", C.c_double),
-                ("seed", C.c_uint64), ("loss", C.c_int32)]
+                ("seed", C.c_uint64), ("loss", C.c_int32), ("trajectory", C.c_int32)]
 
 
 class TrainStats(C.Structure):

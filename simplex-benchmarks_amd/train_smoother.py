@@ -30,6 +30,9 @@ def main():
     ap.add_argument("--degree", type=int, default=2, help="Chebyshev degree of the V-cycle")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--init-seed", type=int, default=3)
+    ap.add_argument("--trajectory", type=int, default=0,
+                    help="0: pairs along plain MG-CG iterates; 1: along flexible CG with the learned step")
+    ap.add_argument("--warm", default=None, help="state blob to continue from")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     import simplexmf as smf
@@ -39,12 +42,15 @@ def main():
     A = smf.Operator(mesh, 2, kind="cg", precision="f64")
     mg = smf.Multigrid(A, smoother_degree=args.degree)
     ls = smf.LearnedSmoother(A, seed=args.init_seed)
+    if args.warm:
+        with open(args.warm, "rb") as f:
+            ls.load(f.read())
     t0 = time.time()
     done, stats = 0, []
     while done < args.rhs:  # chunks, so that progress shows
         k = min(256, args.rhs - done)
         st = ls.train(mg, n_rhs=k, pairs_per_rhs=args.pairs, steps_per_pair=args.steps, lr=args.lr,
-                      seed=args.seed + done, loss=0)
+                      seed=args.seed + done, loss=0, trajectory=args.trajectory)
         done += k
         stats.append(st)
         print(f"rhs {done}/{args.rhs}: energy left {st['first_loss']:.3f} -> {st['last_loss']:.3f} "
@@ -53,7 +59,7 @@ def main():
     with open(out, "wb") as f:
         f.write(ls.save())
     meta = {"n": args.n, "rhs": args.rhs, "pairs_per_rhs": args.pairs, "steps_per_pair": args.steps,
-            "lr": args.lr, "chebyshev_degree": args.degree, "seed": args.seed, "init_seed": args.init_seed,
+            "lr": args.lr, "chebyshev_degree": args.degree, "trajectory": args.trajectory, "warm": args.warm, "seed": args.seed, "init_seed": args.init_seed,
             "loss": "energy fraction left after the learned step",
             "first_loss": stats[0]["first_loss"], "last_loss": stats[-1]["last_loss"],
             "seconds": round(time.time() - t0, 1)}
