@@ -55,6 +55,10 @@ typedef struct mf_options {
   int32_t host_only;        /* 1: build the host plan only (no device memory); device entry points fail */
   int32_t accumulate;       /* CG LDS accumulation: 0 auto, 1 LDS atomics, 2 read-add-write per colour */
   int32_t dg_kernel;        /* DG kernel: 0 auto, 1 LDS tables, 2 compile-time tables + neighbour in face-3 frame */
+  int32_t boundary;         /* CG Poisson: 0 Dirichlet rows = identity (default); 1 Nitsche: no constrained
+                               DoFs, boundary faces add int(2 sigma u v - du/dn v - u dv/dn) with
+                               sigma = (p+1)(p+3)/3 tau_K (the CG level under a DG operator in the hybrid
+                               multigrid, DG/CG c-transfer, application_phc.cc:1424-1456, 1798-1832) */
 } mf_options;
 
 typedef struct mf_info {
@@ -145,6 +149,8 @@ typedef struct mf_mg_info {
   int64_t n_dofs[MF_MAX_LEVELS];
   double lambda_max[MF_MAX_LEVELS];     /* Chebyshev upper bound (1.2 x estimate) */
   int64_t vcycles;                      /* V-cycles applied so far */
+  int32_t level_kind[MF_MAX_LEVELS];    /* MF_CG_POISSON or MF_DG_SIPG */
+  int32_t nitsche[MF_MAX_LEVELS];       /* 1: CG level with the Nitsche boundary */
 } mf_mg_info;
 
 typedef struct mf_solve_stats {
@@ -154,7 +160,10 @@ typedef struct mf_solve_stats {
 } mf_solve_stats;
 
 void mf_mg_default_options(mf_mg_options* opt);
-/* Builds the hierarchy below a CG Poisson operator (needs kuhn_n > 0 for h-levels). */
+/* Builds the hierarchy below a scalar CG Poisson operator (P_k..P_1, then h)
+ * or a DG SIPG operator (DG P_k, c-level CG P_k with the Nitsche boundary,
+ * then p and h on CG Nitsche levels, application_phc.cc:3416-3552); h-levels
+ * need kuhn_n > 0. */
 int mf_mg_create(mf_op_t fine, const mf_mg_options* opt, mf_mg_t* out);
 /* dst = V-cycle(src) in the fine operator's precision (PreconditionMG::vmult) */
 int mf_mg_vmult(mf_mg_t mg, void* dst, const void* src, void* stream);

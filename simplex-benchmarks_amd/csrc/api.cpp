@@ -56,6 +56,13 @@ void upload_cg(Operator& op) {
     op.d_metric.upload(convert<float>(pl.metric));
     op.d_shape.upload(convert<float>(pl.shape));
   }
+  if (pl.nitsche) {
+    op.d_bnd_rows.upload(pl.bnd_rows);
+    op.d_bnd_ptr.upload(pl.bnd_ptr);
+    op.d_bnd_col.upload(pl.bnd_col);
+    if (op.precision == MF_F64) op.d_bnd_val.upload(pl.bnd_val);
+    else op.d_bnd_val.upload(convert<float>(pl.bnd_val));
+  }
   op.on_device = true;
 }
 
@@ -142,6 +149,16 @@ void host_apply_cg_t(const Operator& op, int mode, T* dst, const T* src) {
   }
   for (int64_t i = pl.n_free; i < pl.n_dofs; ++i) dstc[i] = mode == 0 ? srcc[i] : T(1);
   }
+  if (pl.nitsche)
+    for (size_t r = 0; r < pl.bnd_rows.size(); ++r) {
+      const int32_t row = pl.bnd_rows[r];
+      T s = T(0);
+      for (int32_t k = pl.bnd_ptr[r]; k < pl.bnd_ptr[r + 1]; ++k) {
+        if (mode == 0) s += T(pl.bnd_val[k]) * src[pl.bnd_col[k]];
+        else if (pl.bnd_col[k] == row) s += T(pl.bnd_val[k]);
+      }
+      dst[row] += s;
+    }
 }
 }  // namespace
 
@@ -180,6 +197,7 @@ void mf_default_options(mf_options* opt) {
   opt->host_only = 0;
   opt->accumulate = 0;
   opt->dg_kernel = 0;
+  opt->boundary = 0;
 }
 
 int mf_mesh_kuhn_cube(int32_t n, double perturb, uint64_t seed, double* vertices, int32_t* cells) {
@@ -198,6 +216,9 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
     *out = nullptr;
     if (degree < 1 || degree > 3) throw std::runtime_error("degree must be 1, 2 or 3");
     if (opt && (opt->accumulate < 0 || opt->accumulate > 2)) throw std::runtime_error("accumulate must be 0, 1 or 2");
+    if (opt && (opt->boundary < 0 || opt->boundary > 1)) throw std::runtime_error("boundary must be 0 or 1");
+    if (opt && opt->boundary == 1 && kind != MF_CG_POISSON)
+      throw std::runtime_error("Nitsche boundary (boundary = 1) is available for the scalar CG operator");
     if (precision != MF_F64 && precision != MF_F32) throw std::runtime_error("precision must be MF_F64 or MF_F32");
     if (kind != MF_CG_POISSON && kind != MF_CG_VECTOR && kind != MF_DG_SIPG && kind != MF_DG_VECTOR &&
         kind != MF_DG_HELMHOLTZ)
@@ -431,6 +452,8 @@ int mf_mg_get_info(mf_mg_t mg, mf_mg_info* info) {
       info->kuhn_n[l] = L[l].kuhn_n;
       info->n_dofs[l] = L[l].op->m();
       info->lambda_max[l] = L[l].lambda_max;
+      info->level_kind[l] = L[l].op->kind;
+      info->nitsche[l] = !L[l].op->is_dg() && L[l].op->cg.nitsche;
     }
     info->vcycles = mg->mg->vcycles;
   });
@@ -458,10 +481,16 @@ int mf_mg_level_dofs(mf_mg_t mg, int32_t level, double* xyz, uint8_t* constraine
     if (!mg) throw std::runtime_error("null argument");
     auto& L = mg->mg->levels;
     if (level < 0 || level >= (int)L.size()) throw std::runtime_error("level out of range");
-    const CGPlan& pl = L[level].op->cg;
-    if (xyz) std::memcpy(xyz, pl.dof_points.data(), pl.dof_points.size() * sizeof(double));
-    if (constrained)
-      for (int64_t i = 0; i < pl.n_dofs; ++i) constrained[i] = i >= pl.n_free ? 1 : 0;
+    const Operator& op = *L[level].op;
+    if (op.is_dg()) {
+      if (xyz) std::memcpy(xyz, op.dg.dof_points.data(), op.dg.dof_points.size() * sizeof(double));
+      if (constrained) std::memset(constrained, 0, op.dg.n_dofs);
+    } else {
+      const CGPlan& pl = op.cg;
+      if (xyz) std::memcpy(xyz, pl.dof_points.data(), pl.dof_points.size() * sizeof(double));
+      if (constrained)
+        for (int64_t i = 0; i < pl.n_dofs; ++i) constrained[i] = i >= pl.n_free ? 1 : 0;
+    }
   });
 }
 
@@ -474,7 +503,8 @@ int mf_solve_cg(mf_op_t A, mf_mg_t mg, void* x, const void* b, double rel_tol, i
   MF_TRY({
     if (!A || !x || !b) throw std::runtime_error("null argument");
     require_device(A->op);
-    if (A->op.kind != MF_CG_POISSON) throw std::runtime_error("solver needs a CG Poisson operator");
+    if (A->op.kind != MF_CG_POISSON && A->op.kind != MF_DG_SIPG)
+      throw std::runtime_error("solver needs a scalar CG Poisson or DG SIPG operator");
     if (mg && mg->mg->n_fine != A->op.m()) throw std::runtime_error("multigrid does not match the operator");
     SolveStats st = solve_cg(A->op, mg ? mg->mg.get() : nullptr, static_cast<double*>(x),
                              static_cast<const double*>(b), rel_tol, max_iter, flexible != 0,
@@ -504,7 +534,7 @@ int mf_assemble_rhs(mf_op_t op, int32_t kind, uint64_t seed, int32_t n_features,
 int mf_exact_solution(mf_op_t op, double* u) {
   MF_TRY({
     if (!op || !u) throw std::runtime_error("null argument");
-    if (op->op.kind != MF_CG_POISSON) throw std::runtime_error("CG operators only");
+    if (op->op.kind != MF_CG_POISSON && op->op.kind != MF_DG_SIPG) throw std::runtime_error("scalar CG and DG operators only");
     std::vector<double> v;
     exact_solution(op->op, v);
     std::memcpy(u, v.data(), v.size() * sizeof(double));

@@ -177,7 +177,8 @@ int mf_solve_fcg_learned(mf_op_t A, mf_mg_t mg, mf_ls_t h, void* x, const void* 
   MF_TRY({
     if (!A || !mg || !x || !b) throw std::runtime_error("null argument");
     require_device(A->op);
-    if (A->op.kind != MF_CG_POISSON) throw std::runtime_error("solver needs a CG Poisson operator");
+    if (A->op.kind != MF_CG_POISSON && A->op.kind != MF_DG_SIPG)
+      throw std::runtime_error("solver needs a scalar CG Poisson or DG SIPG operator");
     if (mg->mg->n_fine != A->op.m()) throw std::runtime_error("multigrid does not match the operator");
     FcgOptions o;
     if (opt) {

@@ -272,6 +272,34 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
   SMF_HIP(hipGetLastError());
 }
 
+// Nitsche boundary rows: dst[row] += sum_k val[k] src[col[k]] (MODE 0) or
+// dst[row] += B[row][row] (MODE 1).  One thread per boundary row: every row
+// has one writer, so the sums are deterministic.
+template <typename T, int MODE>
+__global__ void cg_boundary_kernel(int64_t nrows, const int32_t* __restrict__ rows, const int32_t* __restrict__ ptr,
+                                   const int32_t* __restrict__ col, const T* __restrict__ val,
+                                   const T* __restrict__ src, T* __restrict__ dst) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int32_t row = rows[r];
+  T s = T(0);
+  for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
+    if (MODE == 0) s += val[k] * src[col[k]];
+    else if (col[k] == row) s += val[k];
+  }
+  dst[row] += s;
+}
+
+template <typename T, int MODE>
+void launch_boundary_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
+  const int64_t nr = (int64_t)op.cg.bnd_rows.size();
+  if (nr > 0)
+    hipLaunchKernelGGL((cg_boundary_kernel<T, MODE>), dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
+                       op.d_bnd_rows.as<int32_t>(), op.d_bnd_ptr.as<int32_t>(), op.d_bnd_col.as<int32_t>(),
+                       op.d_bnd_val.as<T>(), src, dst);
+  SMF_HIP(hipGetLastError());
+}
+
 template <typename T, int NC>
 void launch_cg_nc(const Operator& op, int mode, T* dst, const T* src, hipStream_t s) {
   switch (op.cg.p * 2 + mode) {
@@ -289,6 +317,10 @@ template <typename T>
 void launch_cg_prec(const Operator& op, int mode, T* dst, const T* src, hipStream_t s) {
   if (op.ncomp == 3) launch_cg_nc<T, 3>(op, mode, dst, src, s);
   else launch_cg_nc<T, 1>(op, mode, dst, src, s);
+  if (op.cg.nitsche) {
+    if (mode == 0) launch_boundary_t<T, 0>(op, dst, src, s);
+    else launch_boundary_t<T, 1>(op, dst, src, s);
+  }
 }
 
 }  // namespace
@@ -298,6 +330,13 @@ void launch_cg(const Operator& op, int mode, void* dst, const void* src, hipStre
     launch_cg_prec<double>(op, mode, static_cast<double*>(dst), static_cast<const double*>(src), s);
   else
     launch_cg_prec<float>(op, mode, static_cast<float*>(dst), static_cast<const float*>(src), s);
+}
+
+void launch_dg(const Operator& op, int mode, void* dst, const void* src, hipStream_t s);
+
+void launch_op(const Operator& op, int mode, void* dst, const void* src, hipStream_t s) {
+  if (op.is_dg()) launch_dg(op, mode, dst, src, s);
+  else launch_cg(op, mode, dst, src, s);
 }
 
 int cg_quadrature_degree(int p) { return p == 1 ? 1 : (p == 2 ? 2 : 4); }

@@ -53,7 +53,8 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   if (p >= 2) ei = build_edges(mesh);
   const int64_t n_raw = nv + ei.n_edges * (p - 1) + (p == 3 ? fi.n_faces : 0);
   std::vector<uint8_t> raw_con(n_raw, 0);
-  for (int64_t f = 0; f < fi.n_faces; ++f) {
+  plan.nitsche = opt.boundary == 1;
+  for (int64_t f = 0; f < fi.n_faces && !plan.nitsche; ++f) {
     if (fi.face_cell[f][1] >= 0) continue;
     const int64_t c = fi.face_cell[f][0];
     const int lf = fi.face_lf[f][0];
@@ -265,6 +266,115 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
       for (int b = 0; b < 3; ++b) plan.jit[(size_t)(3 * a + b) * nc + k] = g.Jinv[b][a];
     plan.detj[k] = g.det;
   }
+  if (plan.nitsche) build_nitsche_boundary(mesh, plan);
+}
+
+namespace {
+const double kNref[4][3] = {{1, 1, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};  // outward, |.| = 2 area / ref area
+}
+
+std::vector<double> cell_tau(const Mesh& mesh) {
+  const FaceInfo fi = build_faces(mesh);
+  std::vector<double> surf(mesh.nc, 0.0);
+  for (int64_t f = 0; f < fi.n_faces; ++f) {
+    const bool bdry = fi.face_cell[f][1] < 0;
+    for (int side = 0; side < (bdry ? 1 : 2); ++side) {
+      const int64_t c = fi.face_cell[f][side];
+      const int lf = fi.face_lf[f][side];
+      const CellGeom g = cell_geometry(mesh, c);
+      double nn = 0;
+      for (int a = 0; a < 3; ++a) {
+        double v = 0;
+        for (int b = 0; b < 3; ++b) v += g.Jinv[b][a] * kNref[lf][b];
+        nn += v * v;
+      }
+      surf[c] += (bdry ? 1.0 : 0.5) * 0.5 * std::fabs(g.det) * std::sqrt(nn);
+    }
+  }
+  std::vector<double> tau(mesh.nc);
+  for (int64_t c = 0; c < mesh.nc; ++c) tau[c] = surf[c] / (std::fabs(cell_geometry(mesh, c).det) / 6.0);
+  return tau;
+}
+
+// B_ij = sum over boundary faces F of the cell of
+//   int_F 2 sigma phi_j phi_i - (n.grad phi_j) phi_i - phi_j (n.grad phi_i),
+// sigma = (p+1)(p+3)/3 tau_K: the boundary part of the SIPG form for
+// continuous functions (DG/cells_batched/application.cc:2688-2701 with the
+// mirror principle; as the CG level of application_phc.cc:1798-1832).
+void build_nitsche_boundary(const Mesh& mesh, CGPlan& plan) {
+  const int p = plan.p, nl = plan.n_loc;
+  const Lagrange fe(p);
+  const Quadrature qf = tri_rule(2 * p);
+  double wsum = 0;
+  for (double w : qf.w) wsum += w;
+  const std::vector<double> tau = cell_tau(mesh);
+  const double factor = (p + 1.0) * (p + 3.0) / 3.0;
+  std::vector<int64_t> inv(plan.n_cells);
+  for (int64_t k = 0; k < plan.n_cells; ++k) inv[plan.cell_perm[k]] = k;
+  const FaceInfo fi = build_faces(mesh);
+  struct Trip {
+    int32_t r, c;
+    double v;
+  };
+  std::vector<Trip> trips;
+  std::vector<double> val(nl), grad(3 * nl), dn(nl);
+  for (int64_t f = 0; f < fi.n_faces; ++f) {
+    if (fi.face_cell[f][1] >= 0) continue;
+    const int64_t c = fi.face_cell[f][0];
+    const int lf = fi.face_lf[f][0];
+    const CellGeom g = cell_geometry(mesh, c);
+    double n[3], nn = 0;
+    for (int a = 0; a < 3; ++a) {
+      n[a] = 0;
+      for (int b = 0; b < 3; ++b) n[a] += g.Jinv[b][a] * kNref[lf][b];
+      nn += n[a] * n[a];
+    }
+    const double area = 0.5 * std::fabs(g.det) * std::sqrt(nn);
+    for (int a = 0; a < 3; ++a) n[a] /= std::sqrt(nn);
+    const double sigma = factor * tau[c];
+    const int64_t k = inv[c];
+    std::vector<double> B(nl * nl, 0.0);
+    for (int q = 0; q < qf.size(); ++q) {
+      const auto x = face_to_cell(lf, qf.pts[q][0], qf.pts[q][1]);
+      fe.eval(x.data(), val.data(), grad.data());
+      for (int i = 0; i < nl; ++i) {
+        double s = 0;
+        for (int a = 0; a < 3; ++a) {
+          double ga = 0;  // physical gradient component a = sum_b Jinv[b][a] dphi/dxhat_b
+          for (int b = 0; b < 3; ++b) ga += g.Jinv[b][a] * grad[3 * i + b];
+          s += n[a] * ga;
+        }
+        dn[i] = s;
+      }
+      const double ds = qf.w[q] * area / wsum;
+      for (int i = 0; i < nl; ++i)
+        for (int j = 0; j < nl; ++j)
+          B[i * nl + j] += ds * (2.0 * sigma * val[j] * val[i] - dn[j] * val[i] - val[j] * dn[i]);
+    }
+    for (int i = 0; i < nl; ++i)
+      for (int j = 0; j < nl; ++j)
+        trips.push_back({(int32_t)plan.cell_dofs[k * nl + i], (int32_t)plan.cell_dofs[k * nl + j], B[i * nl + j]});
+  }
+  // stable: contributions to one entry are summed in boundary-face order
+  std::stable_sort(trips.begin(), trips.end(),
+                   [](const Trip& a, const Trip& b) { return a.r < b.r || (a.r == b.r && a.c < b.c); });
+  plan.bnd_rows.clear();
+  plan.bnd_ptr.assign(1, 0);
+  plan.bnd_col.clear();
+  plan.bnd_val.clear();
+  for (size_t t = 0; t < trips.size(); ++t) {
+    if (!plan.bnd_col.empty() && plan.bnd_rows.back() == trips[t].r && plan.bnd_col.back() == trips[t].c) {
+      plan.bnd_val.back() += trips[t].v;
+      continue;
+    }
+    if (plan.bnd_rows.empty() || plan.bnd_rows.back() != trips[t].r) {
+      if (!plan.bnd_rows.empty()) plan.bnd_ptr.push_back((int32_t)plan.bnd_col.size());
+      plan.bnd_rows.push_back(trips[t].r);
+    }
+    plan.bnd_col.push_back(trips[t].c);
+    plan.bnd_val.push_back(trips[t].v);
+  }
+  if (!plan.bnd_rows.empty()) plan.bnd_ptr.push_back((int32_t)plan.bnd_col.size());
 }
 
 }  // namespace smf

@@ -57,6 +57,20 @@ std::unique_ptr<Operator> make_cg_operator(const Mesh& mesh, int degree, int pre
   return op;
 }
 
+std::unique_ptr<Operator> make_dg_operator(const Mesh& mesh, int degree, int precision, const mf_options& opt) {
+  auto op = std::make_unique<Operator>();
+  op->kind = MF_DG_SIPG;
+  op->precision = precision;
+  op->degree = degree;
+  op->ncomp = 1;
+  op->opt = opt;
+  op->opt.host_only = 0;
+  op->mesh = mesh;
+  build_dg_plan(op->mesh, degree, 1, op->opt, false, op->dg);
+  upload_dg(*op);
+  return op;
+}
+
 std::vector<int64_t> inverse_perm(const std::vector<int64_t>& perm) {
   std::vector<int64_t> inv(perm.size());
   for (size_t k = 0; k < perm.size(); ++k) inv[perm[k]] = (int64_t)k;
@@ -160,6 +174,49 @@ Csr build_prolongation(const Operator& fine, const Operator& coarse, bool same_m
   return P;
 }
 
+// c-transfer: injection of the CG P_k space into the DG P_k space on the
+// same mesh (DG row (k, i) takes the CG DoF at the same support point), as
+// the DG/CG MGTwoLevelTransfer of application_phc.cc:3416-3552.
+Csr build_c_prolongation(const Operator& dg, const Operator& cg) {
+  const DGPlan& dp = dg.dg;
+  const CGPlan& cp = cg.cg;
+  if (dp.p != cp.p || dp.n_cells != cp.n_cells) throw std::runtime_error("c-transfer needs equal degree and mesh");
+  const int nl = dp.n_loc;
+  const std::vector<int64_t> cinv = inverse_perm(cp.cell_perm);
+  Csr P;
+  P.nrows = dp.n_dofs;
+  P.ncols = cp.n_dofs;
+  P.ptr.resize(P.nrows + 1);
+  P.col.resize(P.nrows);
+  P.val.assign(P.nrows, 1.0);
+  for (int64_t k = 0; k < dp.n_cells; ++k) {
+    const int64_t kc = cinv[dp.cell_perm[k]];
+    double h2 = 0;  // squared cell size for the matching tolerance
+    for (int a = 0; a < 3; ++a) {
+      const double d = dp.dof_points[3 * (k * nl + 1) + a] - dp.dof_points[3 * (k * nl) + a];
+      h2 += d * d;
+    }
+    for (int i = 0; i < nl; ++i) {
+      const double* X = &dp.dof_points[3 * (k * nl + i)];
+      int best = -1;
+      double bd = 1e300;
+      for (int j = 0; j < nl; ++j) {
+        const double* Y = &cp.dof_points[3 * cp.cell_dofs[kc * nl + j]];
+        const double d = (X[0] - Y[0]) * (X[0] - Y[0]) + (X[1] - Y[1]) * (X[1] - Y[1]) + (X[2] - Y[2]) * (X[2] - Y[2]);
+        if (d < bd) {
+          bd = d;
+          best = j;
+        }
+      }
+      if (!(bd <= 1e-16 * h2)) throw std::runtime_error("c-transfer: DG node without a CG node");
+      P.col[k * nl + i] = (int32_t)cp.cell_dofs[kc * nl + best];
+      P.ptr[k * nl + i + 1] = k * nl + i + 1;
+    }
+  }
+  P.ptr[0] = 0;
+  return P;
+}
+
 Csr transpose(const Csr& A) {
   Csr T;
   T.nrows = A.ncols;
@@ -210,6 +267,9 @@ std::vector<double> dense_inverse(const Operator& op) {
     }
   }
   for (int64_t i = pl.n_free; i < n; ++i) A[i * n + i] = 1.0;
+  if (pl.nitsche)
+    for (size_t r = 0; r < pl.bnd_rows.size(); ++r)
+      for (int32_t k = pl.bnd_ptr[r]; k < pl.bnd_ptr[r + 1]; ++k) A[pl.bnd_rows[r] * n + pl.bnd_col[k]] += pl.bnd_val[k];
   // Cholesky A = L L^T (in place, lower)
   for (int64_t j = 0; j < n; ++j) {
     double s = A[j * n + j];
@@ -275,7 +335,7 @@ T* P_(DevBuf& b) {
 
 template <typename T>
 void level_vmult(const MGLevel& L, T* dst, const T* src, hipStream_t s) {
-  launch_cg(*L.op, 0, dst, src, s);
+  launch_op(*L.op, 0, dst, src, s);
 }
 
 template <typename T>
@@ -388,19 +448,33 @@ void vcycle(Multigrid& mg, int l, hipStream_t s) {
 }  // namespace
 
 std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions& opt) {
-  if (fine.kind != MF_CG_POISSON) throw std::runtime_error("multigrid needs a scalar CG Poisson fine operator");
+  const bool dg = fine.kind == MF_DG_SIPG;
+  if (fine.kind != MF_CG_POISSON && !dg)
+    throw std::runtime_error("multigrid needs a scalar CG Poisson or DG SIPG fine operator");
+  if (fine.kind == MF_CG_POISSON && fine.cg.nitsche)
+    throw std::runtime_error("multigrid below a Nitsche CG operator: build it from the DG operator");
   auto mg = std::make_unique<Multigrid>();
   mg->opt = opt;
   mg->fine_precision = fine.precision;
   mg->n_fine = fine.m();
   const int prec = opt.level_precision;
   mf_options lopt = fine.opt;
-  // levels, finest first: P_k..P_1 on the fine mesh, then P1 on coarser Kuhn meshes
+  // DG: the CG levels carry the Nitsche boundary (no strong constraints),
+  // application_phc.cc:3519-3525
+  if (dg) lopt.boundary = 1;
+  // levels, finest first: [DG P_k, c-level CG P_k,] P_k-1..P_1 on the fine
+  // mesh, then P1 on coarser Kuhn meshes
   std::vector<std::unique_ptr<Operator>> ops;
   std::vector<int> kn;
+  if (dg) {
+    ops.push_back(make_dg_operator(fine.mesh, fine.degree, prec, fine.opt));
+    kn.push_back(fine.mesh.kuhn_n);
+    if (ops[0]->dg.cell_perm != fine.dg.cell_perm) throw std::runtime_error("level numbering differs from the fine operator");
+  }
   ops.push_back(make_cg_operator(fine.mesh, fine.degree, prec, lopt));
   kn.push_back(fine.mesh.kuhn_n);
-  if (ops[0]->cg.cell_dofs != fine.cg.cell_dofs) throw std::runtime_error("level numbering differs from the fine operator");
+  if (!dg && ops[0]->cg.cell_dofs != fine.cg.cell_dofs)
+    throw std::runtime_error("level numbering differs from the fine operator");
   for (int p = fine.degree - 1; p >= 1; --p) {
     ops.push_back(make_cg_operator(fine.mesh, p, prec, lopt));
     kn.push_back(fine.mesh.kuhn_n);
@@ -432,11 +506,12 @@ std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions
     MGLevel& L = mg->levels[l];
     const int64_t m = L.op->m();
     for (DevBuf* b : {&L.inv_diag, &L.x, &L.b, &L.r, &L.t, &L.d}) b->alloc(m * w);
-    launch_cg(*L.op, 1, L.inv_diag.p, nullptr, s);
+    launch_op(*L.op, 1, L.inv_diag.p, nullptr, s);
     launch_invert_diagonal(prec, m, L.inv_diag.p, s);
     if (l > 0) {
       const bool same = L.op->mesh.nc == mg->levels[l - 1].op->mesh.nc && L.kuhn_n == mg->levels[l - 1].kuhn_n;
-      L.P = build_prolongation(*L.op, *mg->levels[l - 1].op, same);
+      if (L.op->is_dg()) L.P = build_c_prolongation(*L.op, *mg->levels[l - 1].op);
+      else L.P = build_prolongation(*L.op, *mg->levels[l - 1].op, same);
       L.R = transpose(L.P);
       L.P.upload(prec);
       L.R.upload(prec);
@@ -527,7 +602,7 @@ SolveStats solve_cg(const Operator& A, Multigrid* mg, double* x, const double* b
   SMF_HIP(hipMemcpyAsync(p.p, z.p, n * 8, hipMemcpyDeviceToDevice, s));
   double rz = dot(r.as<double>(), z.as<double>());
   for (int it = 1; it <= max_iter; ++it) {
-    launch_cg(A, 0, q.p, p.p, s);
+    launch_op(A, 0, q.p, p.p, s);
     const double pq = dot(p.as<double>(), q.as<double>());
     const double alpha = rz / pq;
     blas_axpby<double>(n, alpha, p.as<double>(), 1.0, x, s);
@@ -553,21 +628,24 @@ SolveStats solve_cg(const Operator& A, Multigrid* mg, double* x, const double* b
 
 void assemble_rhs(const Operator& op, int kind, uint64_t seed, int n_features, double corr_length,
                   std::vector<double>& b) {
-  if (op.kind != MF_CG_POISSON) throw std::runtime_error("right-hand side assembly is for CG operators");
+  const bool dg = op.kind == MF_DG_SIPG;
+  if (op.kind != MF_CG_POISSON && !dg) throw std::runtime_error("right-hand side assembly is for scalar CG and DG operators");
   const CGPlan& pl = op.cg;
   const Mesh& m = op.mesh;
-  const int nl = pl.n_loc;
-  // per cell x0 and J (row-major), internal order
-  std::vector<double> geo(12 * pl.n_cells);
-  for (int64_t k = 0; k < pl.n_cells; ++k) {
-    const int32_t* v = &m.cells[4 * pl.cell_perm[k]];
+  const int nl = dg ? op.dg.n_loc : pl.n_loc;
+  const int64_t ncell = dg ? op.dg.n_cells : pl.n_cells;
+  const int deg = dg ? op.dg.p : pl.p;
+  // per cell x0 and J (row-major), internal order, in the operator's local vertex order
+  std::vector<double> geo(12 * ncell);
+  for (int64_t k = 0; k < ncell; ++k) {
+    const int32_t* v = dg ? &op.dg.cell_verts[4 * k] : &m.cells[4 * pl.cell_perm[k]];
     for (int a = 0; a < 3; ++a) {
       geo[12 * k + a] = m.x[3 * v[0] + a];
       for (int c = 0; c < 3; ++c) geo[12 * k + 3 + 3 * a + c] = m.x[3 * v[c + 1] + a] - m.x[3 * v[0] + a];
     }
   }
   const Quadrature q = tet_rule(6);
-  const Lagrange fe(pl.p);
+  const Lagrange fe(deg);
   std::vector<double> qtab(q.size() * (4 + nl)), val(nl);
   for (int k = 0; k < q.size(); ++k) {
     fe.eval(q.pts[k].data(), val.data(), nullptr);
@@ -594,11 +672,15 @@ void assemble_rhs(const Operator& op, int kind, uint64_t seed, int n_features, d
   dgeo.upload(geo);
   dq.upload(qtab);
   dfeat.upload(feat);
-  dout.alloc(pl.n_cells * nl * sizeof(double));
-  launch_rhs_cells(pl.n_cells, nl, q.size(), dgeo.as<double>(), dq.as<double>(), kind, kind == 1 ? n_features : 0,
+  dout.alloc(ncell * nl * sizeof(double));
+  launch_rhs_cells(ncell, nl, q.size(), dgeo.as<double>(), dq.as<double>(), kind, kind == 1 ? n_features : 0,
                    dfeat.as<double>(), dout.as<double>(), nullptr);
-  std::vector<double> local(pl.n_cells * nl);
+  std::vector<double> local(ncell * nl);
   SMF_HIP(hipMemcpy(local.data(), dout.p, local.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (dg) {  // cell-contiguous DoFs; homogeneous boundary data adds no Nitsche term
+    b = local;
+    return;
+  }
   b.assign(pl.n_dofs, 0.0);
   for (int64_t k = 0; k < pl.n_cells; ++k)
     for (int i = 0; i < nl; ++i) {
@@ -608,6 +690,16 @@ void assemble_rhs(const Operator& op, int kind, uint64_t seed, int n_features, d
 }
 
 void exact_solution(const Operator& op, std::vector<double>& u) {
+  const double a0 = 3.0 * M_PI;
+  if (op.kind == MF_DG_SIPG) {
+    const DGPlan& dp = op.dg;
+    u.resize(dp.n_dofs);
+    for (int64_t d = 0; d < dp.n_dofs; ++d) {
+      const double* X = &dp.dof_points[3 * d];
+      u[d] = std::sin(a0 * X[0]) * std::sin(a0 * X[1]) * std::sin(a0 * X[2]);
+    }
+    return;
+  }
   const CGPlan& pl = op.cg;
   u.assign(pl.n_dofs, 0.0);
   const double a = 3.0 * M_PI;

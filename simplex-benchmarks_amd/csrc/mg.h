@@ -5,7 +5,9 @@
 // PreconditionChebyshev<..., DiagonalMatrix> smoothers (degree 5, smoothing
 // range 20, 10 eigenvalue CG iterations), a coarse solver and SolverCG to
 // 1e-10 ||b||.  The coarse solver here is an FP64 dense inverse instead of one
-// Trilinos ML V-cycle (application_hp.cc:1684-1731).
+// Trilinos ML V-cycle (application_hp.cc:1684-1731).  Under a DG SIPG fine
+// operator the hierarchy is DG P_k -> (c) CG P_k -> (p) ... P_1 -> (h), the
+// CG levels with the Nitsche boundary (application_phc.cc:3416-3552).
 #pragma once
 #include <memory>
 #include <vector>

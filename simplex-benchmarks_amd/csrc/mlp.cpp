@@ -43,25 +43,27 @@ uint16_t bf16_bits(float f) {
 }  // namespace
 
 std::unique_ptr<LearnedSmoother> make_learned_smoother(const Operator& op, uint64_t seed) {
-  if (op.kind != MF_CG_POISSON || op.degree != 2 || op.ncomp != 1)
-    throw std::runtime_error("the learned smoother is defined for the scalar CG P2 operator");
+  const bool dg = op.kind == MF_DG_SIPG;
+  if (!((op.kind == MF_CG_POISSON || dg) && op.degree == 2 && op.ncomp == 1))
+    throw std::runtime_error("the learned smoother is defined for the scalar P2 CG and DG operators");
   if (op.precision != MF_F64) throw std::runtime_error("the learned smoother works on the FP64 fine operator");
   if (!op.on_device) throw std::runtime_error("operator was created host_only");
-  const CGPlan& pl = op.cg;
   auto ls = std::make_unique<LearnedSmoother>();
   ls->op = &op;
-  const int64_t nc = pl.n_cells, nd = pl.n_dofs;
+  const int64_t nc = dg ? op.dg.n_cells : op.cg.n_cells, nd = op.m();
   if (nc * kMlpOut >= (int64_t)1 << 31) throw std::runtime_error("too many cells for the learned smoother's int32 slots");
   ls->n_cells = nc;
   ls->n_dofs = nd;
   // cell DoFs (-1 = Dirichlet) and DoF -> slot lists in ascending slot order
+  // (DG: DoFs are cell-contiguous, one slot each)
   std::vector<int32_t> cd(nc * kMlpOut);
   std::vector<int64_t> ptr(nd + 1, 0);
   for (int64_t k = 0; k < nc; ++k)
     for (int i = 0; i < kMlpOut; ++i) {
-      const int64_t d = pl.cell_dofs[k * kMlpOut + i];
-      cd[k * kMlpOut + i] = d < pl.n_free ? (int32_t)d : -1;
-      if (d < pl.n_free) ptr[d + 1]++;
+      const int64_t d = dg ? k * kMlpOut + i : op.cg.cell_dofs[k * kMlpOut + i];
+      const bool free = dg || d < op.cg.n_free;
+      cd[k * kMlpOut + i] = free ? (int32_t)d : -1;
+      if (free) ptr[d + 1]++;
     }
   for (int64_t d = 0; d < nd; ++d) ptr[d + 1] += ptr[d];
   std::vector<int32_t> slot(ptr[nd]);
@@ -71,21 +73,23 @@ std::unique_ptr<LearnedSmoother> make_learned_smoother(const Operator& op, uint6
       for (int i = 0; i < kMlpOut; ++i)
         if (cd[k * kMlpOut + i] >= 0) slot[fill[cd[k * kMlpOut + i]]++] = (int32_t)(k * kMlpOut + i);
   }
-  // geometry features
+  // geometry features: J^-T (component-major 9 x nc) and |det J| of either plan
+  const double* jit = dg ? op.dg.geo.data() : op.cg.jit.data();
+  auto adet = [&](int64_t k) { return dg ? op.dg.geo[(size_t)9 * nc + k] : std::fabs(op.cg.detj[k]); };
   double vol = 0;
-  for (int64_t k = 0; k < nc; ++k) vol += std::fabs(pl.detj[k]);
+  for (int64_t k = 0; k < nc; ++k) vol += adet(k);
   ls->h = std::cbrt(vol / (double)nc);
   std::vector<float> geo(nc * 10);
   for (int64_t k = 0; k < nc; ++k) {
-    for (int e = 0; e < 9; ++e) geo[k * 10 + e] = (float)(ls->h * pl.jit[(size_t)e * nc + k]);
-    geo[k * 10 + 9] = (float)(std::fabs(pl.detj[k]) / (ls->h * ls->h * ls->h));
+    for (int e = 0; e < 9; ++e) geo[k * 10 + e] = (float)(ls->h * jit[(size_t)e * nc + k]);
+    geo[k * 10 + 9] = (float)(adet(k) / (ls->h * ls->h * ls->h));
   }
   // kappa_K = mean inverse diagonal over the cell's free DoFs
   std::vector<double> diag(nd);
   {
     DevBuf dd;
     dd.alloc(nd * sizeof(double));
-    launch_cg(op, 1, dd.p, nullptr, nullptr);
+    launch_op(op, 1, dd.p, nullptr, nullptr);
     SMF_HIP(hipMemcpy(diag.data(), dd.p, nd * sizeof(double), hipMemcpyDeviceToHost));
   }
   std::vector<float> kappa(nc, 0.0f);
@@ -238,7 +242,7 @@ double ls_energy_gradient(LearnedSmoother& ls, const double* rp, const double* e
   c.alloc(n * sizeof(double));
   ac.alloc(n * sizeof(double));
   ls_apply(ls, c.as<double>(), rp, s);  // forward as at inference (BF16 MFMA); features and scales kept
-  launch_cg(A, 0, ac.p, c.p, s);
+  launch_op(A, 0, ac.p, c.p, s);
   double d[2];
   host_dot2(ls.d_partial, ls.d_scalar, n, c.as<double>(), rp, c.as<double>(), ac.as<double>(), d, s);
   const double E = host_dot(ls.d_partial, ls.d_scalar, n, e, rp, s);  // |e'|_A^2 = e'.A e' = e'.r'
@@ -299,7 +303,7 @@ TrainStats ls_train(LearnedSmoother& ls, Multigrid& mg, const TrainOptions& opt,
     double rz = 0, alpha = 0;
     for (int it = 0; it < opt.pairs_per_rhs; ++it) {
       mg_vmult(mg, z.p, r.p, s);
-      launch_cg(A, 0, az.p, z.p, s);
+      launch_op(A, 0, az.p, z.p, s);
       copy_d(rp.as<double>(), r.as<double>(), n, s);
       blas_axpby<double>(n, -1.0, az.as<double>(), 1.0, rp.as<double>(), s);
       copy_d(e.as<double>(), xs.as<double>(), n, s);
@@ -340,7 +344,7 @@ TrainStats ls_train(LearnedSmoother& ls, Multigrid& mg, const TrainOptions& opt,
       if (opt.trajectory == 1) {
         // the learned step as the solver applies it: z += omega c
         ls_apply(ls, c.as<double>(), rp.as<double>(), s);
-        launch_cg(A, 0, ac.p, c.p, s);
+        launch_op(A, 0, ac.p, c.p, s);
         double d[2];
         host_dot2(ls.d_partial, ls.d_scalar, n, c.as<double>(), rp.as<double>(), c.as<double>(), ac.as<double>(), d,
                   s);
@@ -358,7 +362,7 @@ TrainStats ls_train(LearnedSmoother& ls, Multigrid& mg, const TrainOptions& opt,
         blas_axpby<double>(n, 1.0, z.as<double>(), beta, p.as<double>(), s);
       }
       rz = rz_new;
-      launch_cg(A, 0, q.p, p.p, s);
+      launch_op(A, 0, q.p, p.p, s);
       const double pq = host_dot(ls.d_partial, ls.d_scalar, n, p.as<double>(), q.as<double>(), s);
       alpha = rz / pq;
       blas_axpby<double>(n, alpha, p.as<double>(), 1.0, x.as<double>(), s);
@@ -408,14 +412,14 @@ FcgStats solve_fcg_learned(const Operator& A, Multigrid& mg, LearnedSmoother* ls
   auto precond = [&]() -> bool {
     mg_vmult(mg, z.p, r.p, s);
     if (!learned) return false;
-    launch_cg(A, 0, az.p, z.p, s);
+    launch_op(A, 0, az.p, z.p, s);
     // c = L(r - az): features formed inside the inference kernel
     ls_launch_infer_fused(ls->n_cells, ls->d_cell_dofs.as<int32_t>(), ls->d_geo.as<float>(),
                           ls->d_kappa.as<float>(), r.as<double>(), az.as<double>(), nullptr, nullptr, ls->d_frag.p,
                           ls->d_params.as<float>(), ls->d_corr.as<float>(), s);
     ls_launch_assemble(ls->n_dofs, ls->d_slot_ptr.as<int64_t>(), ls->d_slot.as<int32_t>(), ls->d_corr.as<float>(),
                        c.as<double>(), s);
-    launch_cg(A, 0, ac.p, c.p, s);
+    launch_op(A, 0, ac.p, c.p, s);
     ls_launch_omega(n, ls->d_part2.as<double>(), lsc, c.as<double>(), r.as<double>(), az.as<double>(),
                     ac.as<double>(), z.as<double>(), az.as<double>(), s);
     SMF_HIP(hipMemcpyAsync(ls->h_lsc, lsc, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -435,7 +439,7 @@ FcgStats solve_fcg_learned(const Operator& A, Multigrid& mg, LearnedSmoother* ls
   bool have_az = precond();
   copy_d(p.as<double>(), z.as<double>(), n, s);
   if (have_az) copy_d(q.as<double>(), az.as<double>(), n, s);
-  else launch_cg(A, 0, q.p, p.p, s);
+  else launch_op(A, 0, q.p, p.p, s);
   double rz = host_dot(partial, scalar, n, r.as<double>(), z.as<double>(), s);
   collect(0);
   double rprev = st.base.r0;
@@ -463,7 +467,7 @@ FcgStats solve_fcg_learned(const Operator& A, Multigrid& mg, LearnedSmoother* ls
     const double beta = -alpha * d[1] / rz;
     blas_axpby<double>(n, 1.0, z.as<double>(), beta, p.as<double>(), s);
     if (have_az) blas_axpby<double>(n, 1.0, az.as<double>(), beta, q.as<double>(), s);
-    else launch_cg(A, 0, q.p, p.p, s);
+    else launch_op(A, 0, q.p, p.p, s);
     rz = d[0];
   }
   SMF_HIP(hipStreamSynchronize(s));

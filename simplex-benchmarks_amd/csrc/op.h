@@ -91,6 +91,11 @@ struct CGPlan {
   std::vector<double> detj;          // n_cells
   int64_t n_ghost_total = 0;
   std::vector<double> shape;         // G (n_loc x n_q x 3) then w_q G
+  // Nitsche boundary (weak Dirichlet): sparse rows of the boundary-face terms,
+  // row bnd_rows[r] has entries [bnd_ptr[r], bnd_ptr[r+1]) (columns ascending)
+  bool nitsche = false;
+  std::vector<int32_t> bnd_rows, bnd_ptr, bnd_col;
+  std::vector<double> bnd_val;
 };
 
 // Host-side plan of a discontinuous-Galerkin (SIPG) operator.  Cell local
@@ -136,6 +141,7 @@ struct Operator {
   // device data
   DevBuf d_cell_start, d_own_start, d_ghost_start, d_ghost_idx, d_ccol, d_fix_ptr,
       d_fix_src, d_lidx, d_metric, d_shape, d_partial;
+  DevBuf d_bnd_rows, d_bnd_ptr, d_bnd_col, d_bnd_val;
   DevBuf d_dg_geo, d_dg_nbr, d_dg_nface, d_dg_cell_tab, d_dg_face_tab;
   bool on_device = false;
   int64_t m() const { return is_dg() ? dg.n_dofs : cg.n_dofs * ncomp; }
@@ -143,6 +149,12 @@ struct Operator {
 };
 
 void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan);
+// boundary-face matrix of the Nitsche CG operator (plan.nitsche)
+void build_nitsche_boundary(const Mesh& mesh, CGPlan& plan);
+// SIPG penalty tau_K = (sum_bdry |F| + 1/2 sum_int |F|) / |K| per input cell
+std::vector<double> cell_tau(const Mesh& mesh);
+// any operator: dispatch to the CG or DG launcher
+void launch_op(const Operator& op, int mode, void* dst, const void* src, hipStream_t s);
 void make_cg_shape(CGPlan& plan);
 // host execution of the kernels' schedule (self-check of the tables)
 void host_apply_cg(const Operator& op, int mode, void* dst, const void* src);

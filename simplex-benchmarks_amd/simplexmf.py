@@ -38,7 +38,7 @@ class Options(C.Structure):
     _fields_ = [("reorder", C.c_int32), ("patch_cells", C.c_int32),
                 ("mass_coefficient", C.c_double), ("laplace_coefficient", C.c_double),
                 ("penalty_kind", C.c_int32), ("host_only", C.c_int32), ("accumulate", C.c_int32),
-                ("dg_kernel", C.c_int32)]
+                ("dg_kernel", C.c_int32), ("boundary", C.c_int32)]
 
 
 class Info(C.Structure):
@@ -148,7 +148,8 @@ MF_MAX_LEVELS = 16
 class MGInfo(C.Structure):
     _fields_ = [("n_levels", C.c_int32), ("degree", C.c_int32 * MF_MAX_LEVELS),
                 ("kuhn_n", C.c_int32 * MF_MAX_LEVELS), ("n_dofs", C.c_int64 * MF_MAX_LEVELS),
-                ("lambda_max", C.c_double * MF_MAX_LEVELS), ("vcycles", C.c_int64)]
+                ("lambda_max", C.c_double * MF_MAX_LEVELS), ("vcycles", C.c_int64),
+                ("level_kind", C.c_int32 * MF_MAX_LEVELS), ("nitsche", C.c_int32 * MF_MAX_LEVELS)]
 
 
 class SolveStats(C.Structure):
@@ -355,7 +356,7 @@ class Operator:
 
     def __init__(self, mesh: Mesh, degree: int, kind="cg", precision="f64", reorder=True,
                  patch_cells=0, mass_coefficient=1.0, laplace_coefficient=1.0,
-                 penalty_kind=0, host_only=False, accumulate=0, dg_kernel=0):
+                 penalty_kind=0, host_only=False, accumulate=0, dg_kernel=0, boundary="dirichlet"):
         L = lib()
         opt = Options()
         L.mf_default_options(C.byref(opt))
@@ -367,6 +368,7 @@ class Operator:
         opt.host_only = 1 if host_only else 0
         opt.accumulate = accumulate
         opt.dg_kernel = dg_kernel
+        opt.boundary = {"dirichlet": 0, "nitsche": 1}[boundary] if isinstance(boundary, str) else int(boundary)
         self.mesh = mesh
         self.kind = KINDS[kind] if isinstance(kind, str) else int(kind)
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
@@ -493,7 +495,7 @@ class Multigrid:
         n = inf.n_levels
         return {"n_levels": n, "degree": list(inf.degree[:n]), "kuhn_n": list(inf.kuhn_n[:n]),
                 "n_dofs": list(inf.n_dofs[:n]), "lambda_max": list(inf.lambda_max[:n]),
-                "vcycles": inf.vcycles}
+                "vcycles": inf.vcycles, "level_kind": list(inf.level_kind[:n]), "nitsche": list(inf.nitsche[:n])}
 
     def vmult(self, dst, src, stream=None):
         check(lib().mf_mg_vmult(self.handle, dst.ptr, src.ptr, stream.handle if stream else None))
