@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--mode", default="vmult", choices=["vmult", "solve"])
     ap.add_argument("--solves", type=int, default=5, help="repeated solves of the solver comparison")
     ap.add_argument("--solve-n", type=int, default=48, help="mesh of the solver comparison (0: skip it)")
+    ap.add_argument("--solve-kind", default="cg", choices=["cg", "dg"], help="operator of --mode solve")
     return ap.parse_args()
 
 
@@ -108,16 +109,18 @@ def _rates(st, tmin, dofs):
     return {"iterations": n, "rho": rho, "n10": n10, "t10_s": t10, "E10_dofs_per_s": dofs / t10}
 
 
-def solve_compare(smf, n, degree, seed, solves, learned_degree=3):
+def solve_compare(smf, n, degree, seed, solves, learned_degree=3, kind="cg"):
     """Time to 1e-10 of mixed-precision hybrid-MG CG (FP64 outer CG, FP32
     V-cycle, Chebyshev(5)-Jacobi: the reference's MultigridPreconditioner +
     SolverCG, application_ph.cc:1754-2026, metrics of application_hp.cc:
     1954-1988) against flexible CG with the learned post-smoothing step
     behind a Chebyshev(learned_degree) V-cycle, on a Gaussian-random-field
-    right-hand side.  Errors against an FP64 reference solve to 1e-13."""
+    right-hand side.  Errors against an FP64 reference solve to 1e-13.
+    kind "dg": DG SIPG with the DG -> CG (Nitsche) -> P1 -> h hierarchy
+    (application_phc.cc:3416-3552)."""
     t0 = time.time()
     mesh = smf.Mesh.kuhn_cube(n, 0.2, seed)
-    op = smf.Operator(mesh, degree, kind="cg", precision="f64")
+    op = smf.Operator(mesh, degree, kind=kind, precision="f64")
     mg = smf.Multigrid(op)
     b = smf.DeviceVector.from_numpy(smf.assemble_rhs(op, "grf", seed=seed, n_features=256, corr_length=0.1))
     x = op.initialize_dof_vector()
@@ -133,19 +136,21 @@ def solve_compare(smf, n, degree, seed, solves, learned_degree=3):
         return {"max_rel": float(np.max(np.abs(e.numpy())) / scale),
                 "energy": float(np.sqrt(max(np.dot(e.numpy(), ae.numpy()), 0.0)))}
 
-    out = {"workload": f"MG-CG Poisson P{degree}, {n}^3 x 6 perturbed Kuhn tets, FP64 CG + FP32 V-cycle, "
+    name = "MG-CG Poisson" if kind == "cg" else "DG SIPG hybrid h/p/c-MG-CG"
+    out = {"workload": f"{name} P{degree}, {n}^3 x 6 perturbed Kuhn tets, FP64 CG + FP32 V-cycle, "
                        f"GRF right-hand side (l=0.1), stop at 1e-10", "dofs": op.m(),
            "levels": mg.info()["n_dofs"], "solves": solves}
     st, times = _time_solves(smf, lambda: smf.solve_cg(op, x, b, mg, rel_tol=1e-10, stream=stream), solves)
     plain = {"time_s": min(times), "mean_s": float(np.mean(times)), **_rates(st, min(times), op.m()),
              "error": errors(x), "smoother": "Chebyshev(5)-Jacobi"}
     out["plain"] = plain
-    log(f"plain MG-CG {n}^3: {plain['iterations']} it, {plain['time_s'] * 1e3:.2f} ms")
+    log(f"plain {kind} MG-CG {n}^3: {plain['iterations']} it, {plain['time_s'] * 1e3:.2f} ms")
     if degree == 2:
         mgl = smf.Multigrid(op, smoother_degree=learned_degree)
         ls = smf.LearnedSmoother(op, seed=0)
+        tag = "" if kind == "cg" else "_dg"
         wpath = os.path.join(ROOT, "simplex-benchmarks_amd", "weights",
-                             f"learned_smoother_p2_cheb{learned_degree}.bin")
+                             f"learned_smoother_p2{tag}_cheb{learned_degree}.bin")
         if os.path.exists(wpath):
             with open(wpath, "rb") as f:
                 ls.load(f.read())
@@ -166,7 +171,7 @@ def solve_compare(smf, n, degree, seed, solves, learned_degree=3):
         out["plain_same_vcycle"] = ablation
         out["speedup_learned_vs_plain"] = plain["time_s"] / learned["time_s"]
         out["error_ratio_energy"] = learned["error"]["energy"] / max(plain["error"]["energy"], 1e-300)
-        log(f"learned FCG {n}^3: {learned['iterations']} it, {learned['time_s'] * 1e3:.2f} ms, "
+        log(f"learned {kind} FCG {n}^3: {learned['iterations']} it, {learned['time_s'] * 1e3:.2f} ms, "
             f"speedup {out['speedup_learned_vs_plain']:.3f}")
     log(f"solve comparison {time.time() - t0:.1f}s")
     return out
@@ -177,7 +182,7 @@ def solve_mode(args):
     seconds to 1e-10)."""
     import simplexmf as smf
 
-    r = solve_compare(smf, args.solve_n, args.degree, args.seed, args.solves)
+    r = solve_compare(smf, args.solve_n, args.degree, args.seed, args.solves, kind=args.solve_kind)
     p = r["plain"]
     out = {
         "metric": METRIC, "value": round(p["time_s"], 6), "unit": "s to 1e-10 (min over solves)",
@@ -339,6 +344,7 @@ def main():
     }
     if args.solve_n > 0:
         out["solve"] = solve_compare(smf, args.solve_n, 2, args.seed, args.solves)
+        out["solve_dg"] = solve_compare(smf, args.solve_n, 2, args.seed, args.solves, kind="dg")
     print(json.dumps(out), flush=True)
 
 

@@ -28,6 +28,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2, help="Adam steps per pair")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--degree", type=int, default=2, help="Chebyshev degree of the V-cycle")
+    ap.add_argument("--kind", default="cg", choices=["cg", "dg"], help="P2 operator: CG Poisson or DG SIPG")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--init-seed", type=int, default=3)
     ap.add_argument("--trajectory", type=int, default=0,
@@ -37,9 +38,10 @@ def main():
     args = ap.parse_args()
     import simplexmf as smf
 
-    out = args.out or os.path.join(HERE, "weights", f"learned_smoother_p2_cheb{args.degree}.bin")
+    tag = "" if args.kind == "cg" else "_dg"
+    out = args.out or os.path.join(HERE, "weights", f"learned_smoother_p2{tag}_cheb{args.degree}.bin")
     mesh = smf.Mesh.kuhn_cube(args.n, perturb=0.2, seed=7)
-    A = smf.Operator(mesh, 2, kind="cg", precision="f64")
+    A = smf.Operator(mesh, 2, kind=args.kind, precision="f64")
     mg = smf.Multigrid(A, smoother_degree=args.degree)
     ls = smf.LearnedSmoother(A, seed=args.init_seed)
     if args.warm:
@@ -58,7 +60,7 @@ def main():
     os.makedirs(os.path.dirname(out), exist_ok=True)
     with open(out, "wb") as f:
         f.write(ls.save())
-    meta = {"n": args.n, "rhs": args.rhs, "pairs_per_rhs": args.pairs, "steps_per_pair": args.steps,
+    meta = {"kind": args.kind, "n": args.n, "rhs": args.rhs, "pairs_per_rhs": args.pairs, "steps_per_pair": args.steps,
             "lr": args.lr, "chebyshev_degree": args.degree, "trajectory": args.trajectory, "warm": args.warm, "seed": args.seed, "init_seed": args.init_seed,
             "loss": "energy fraction left after the learned step",
             "first_loss": stats[0]["first_loss"], "last_loss": stats[-1]["last_loss"],
