@@ -1,7 +1,8 @@
 // Continuous-Galerkin cell kernels for gfx950.
 //
-// One wavefront (64 lanes) processes one patch of cells of the hierarchical
-// reordering, in rounds of 64 cells (one cell per lane):
+// One workgroup of W wavefronts (W = 1..4, 64 W lanes) processes one patch of
+// cells of the hierarchical reordering, in rounds of 64 W cells (one cell per
+// lane):
 //   1. the patch's DoF values are staged in LDS (owned DoFs are a contiguous
 //      global range -> coalesced loads; ghost DoFs through an index list);
 //   2. each lane evaluates its cell: gradients at the quadrature points
@@ -9,8 +10,8 @@
 //      quadrature-point operator w_q |det J| J^-1 J^-T and the transposed
 //      interpolation (integration), in registers with FP64/FP32 FMA;
 //   3. the round's cell vectors are added into an LDS accumulator colour by
-//      colour (cells of one colour share no DoF; LDS executes one wave's
-//      instructions in order), so every DoF is summed in cell order;
+//      colour, with a workgroup barrier after each colour (cells of one
+//      colour share no DoF), so every DoF is summed in cell order;
 //   4. owned DoFs are stored; ghost DoFs (owned by another patch) are written
 //      as partial sums that a fix-up kernel adds to the owner's value in patch
 //      order.  No floating-point atomics reach global memory: the result is
@@ -25,6 +26,7 @@
 
 #include <algorithm>
 #include <stdexcept>
+#include <string>
 
 #include "op.h"
 #include "shape_tables.h"
@@ -34,7 +36,7 @@ namespace smf {
 namespace {
 
 constexpr int kWave = 64;
-constexpr unsigned short kInvalid = 0xFFFF;
+constexpr int kMaxThreads = 4 * kWave;
 
 template <int P>
 struct CGShape {
@@ -50,7 +52,7 @@ struct CGShape {
 // CG/components_batched/application.cc:1024-1216) share the index tables and
 // the geometry of a cell; component c of scalar DoF d is row c*cstride + d.
 template <int P, typename T, int MODE, bool ATOMIC, int NC>
-__global__ __launch_bounds__(kWave) void cg_wave_kernel(
+__global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ own_start,
     const int32_t* __restrict__ ghost_start, const int32_t* __restrict__ ghost_idx,
     const uint8_t* __restrict__ ccol, const uint16_t* __restrict__ lidx, int64_t ncells_total,
@@ -59,7 +61,8 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
   constexpr int NL = CGShape<P>::NL, NQ = CGShape<P>::NQ;
   extern __shared__ unsigned char smem_raw[];
   T* sbase = reinterpret_cast<T*>(smem_raw);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x;  // cell slot within a round
+  const int nt = blockDim.x;     // 64 W
   const int patch = (int)xcd_block(blockIdx.x, gridDim.x);
   const int c0 = cell_start[patch], nc = cell_start[patch + 1] - c0;
   const int o0 = own_start[patch], no = own_start[patch + 1] - o0;
@@ -75,7 +78,7 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
     T* su = sbase + cc * lstride;
     T* sy = su + nloc + 1;
     const T* sc = src + cc * cstride;
-    for (int j = lane; j <= nloc; j += kWave) {
+    for (int j = lane; j <= nloc; j += nt) {
       if (MODE == 0) su[j] = j < no ? sc[o0 + j] : (j < nloc ? sc[ghost_idx[g0 + j - no]] : T(0));
       sy[j] = T(0);
     }
@@ -95,7 +98,7 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
     col_n = ccol[cell];
   };
   if (nc > 0) load_round(0);
-  for (int r0 = 0; r0 < nc; r0 += kWave) {
+  for (int r0 = 0; r0 < nc; r0 += nt) {
     const int lc = r0 + lane;
     const bool active = lc < nc;
     uint16_t li[NL];
@@ -103,7 +106,7 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
     for (int i = 0; i < NL; ++i) li[i] = min((int)li_n[i], nloc);
     const T m0 = m_n[0], m1 = m_n[1], m2 = m_n[2], m3 = m_n[3], m4 = m_n[4], m5 = m_n[5];
     const int mycol = col_n;
-    if (r0 + kWave < nc) load_round(r0 + kWave);
+    if (r0 + nt < nc) load_round(r0 + nt);
 #pragma unroll 1
     for (int cc = 0; cc < NC; ++cc) {
     T* su = sbase + cc * lstride;
@@ -186,9 +189,10 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
         }
       }
     }
-    // 3. colour-ordered accumulation (colours ascend with the cell index)
-    const int clo = __builtin_amdgcn_readfirstlane(mycol);
-    const int last = min(r0 + kWave, nc) - 1;
+    // 3. colour-ordered accumulation (colours ascend with the cell index);
+    //    the colour range of the round is uniform over the workgroup
+    const int clo = ccol[c0 + r0];
+    const int last = min(r0 + nt, nc) - 1;
     const int chi = ccol[c0 + last];
     for (int c = clo; c <= chi; ++c) {
       if (active && mycol == c) {
@@ -205,6 +209,7 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
           for (int i = 0; i < NL; ++i) sy[li[i]] = v[i] + y[i];
         }
       }
+      if (nt > kWave) __syncthreads();  // one wave alone keeps its LDS order
     }
     }  // components
   }
@@ -215,7 +220,7 @@ __global__ __launch_bounds__(kWave) void cg_wave_kernel(
     const T* sy = sbase + cc * lstride + nloc + 1;
     T* dc = dst + cc * cstride;
     T* pc = partial + cc * pstride;
-    for (int j = lane; j < nloc; j += kWave) {
+    for (int j = lane; j < nloc; j += nt) {
       if (j < no)
         dc[o0 + j] = sy[j];
       else
@@ -252,10 +257,13 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
   const size_t shmem = NC * 2 * ((size_t)pl.max_local + 1) * sizeof(T);
   const bool atomic = op.opt.accumulate == 1 || (op.opt.accumulate == 0 && sizeof(T) == 8);
   auto kern = atomic ? cg_wave_kernel<P, T, MODE, true, NC> : cg_wave_kernel<P, T, MODE, false, NC>;
+  if (shmem > 160 * 1024)
+    throw std::runtime_error("CG patch needs " + std::to_string(shmem) + " bytes of LDS (max 160 KiB): use smaller patch_cells");
   if (shmem > 65536)
     SMF_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  if (pl.waves < 1 || pl.waves > kMaxThreads / kWave) throw std::runtime_error("CG plan: 1 to 4 waves per patch");
   if (pl.n_patches > 0)
-    hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_patches), dim3(kWave), shmem, s,
+    hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_patches), dim3(kWave * pl.waves), shmem, s,
                        op.d_cell_start.as<int32_t>(), op.d_own_start.as<int32_t>(),
                        op.d_ghost_start.as<int32_t>(), op.d_ghost_idx.as<int32_t>(),
                        op.d_ccol.as<uint8_t>(), op.d_lidx.as<uint16_t>(), pl.n_cells,

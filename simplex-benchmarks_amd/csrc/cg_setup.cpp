@@ -36,15 +36,19 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   plan.n_cells = mesh.nc;
   const int64_t nc = mesh.nc, nv = mesh.nv;
 
-  // default patch size per degree, halved (down to 64 cells) until there
-  // are >= 2048 patches, so that small operators (multigrid levels) still
-  // give every SIMD of the 256 CUs a wave
+  // cells per wave by degree (P1 1024, P2 512, P3 128); a patch is one
+  // workgroup of up to 4 waves.  The default patch is 4 waves' worth of
+  // cells (1 for the 3-component operator, whose LDS is 3x larger), halved
+  // (down to 64 cells) until there are >= 1024 patches, so that small
+  // operators (multigrid levels) still fill the 256 CUs
+  const int per_wave = p == 1 ? 1024 : (p == 2 ? 512 : 128);
   int patch = opt.patch_cells;
   if (patch <= 0) {
-    patch = p == 1 ? 1024 : (p == 2 ? 512 : 128);
-    while (patch > 64 && mesh.nc / patch < 2048) patch /= 2;
+    patch = (plan.ncomp == 3 ? 1 : 4) * per_wave;
+    while (patch > 64 && mesh.nc / patch < 1024) patch /= 2;
   }
   if (patch > 4096) throw std::runtime_error("patch_cells must be <= 4096");
+  plan.waves = std::min(4, std::max(1, (patch + per_wave - 1) / per_wave));
   plan.max_patch_cells = 0;
 
   // ---- entities and raw DoFs (entity-based numbering) ----

@@ -81,6 +81,7 @@ struct CGPlan {
   // [own_start[p], own_start[p+1]); ghost DoFs ghost_idx[ghost_start[p]..)
   int64_t n_patches = 0;
   int max_icolors = 0, max_local = 0, max_patch_cells = 0;
+  int waves = 1;                     // wavefronts per patch workgroup (1..4)
   std::vector<int32_t> cell_start, own_start, ghost_start, ghost_idx;
   std::vector<uint8_t> ccol;         // per cell: colour within its patch (cells sorted by colour)
   // interface fix-up: DoF n_int+t = sum of partial[fix_src[k]], k in [fix_ptr[t], fix_ptr[t+1])
