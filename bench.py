@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--precision", default="f64")
     ap.add_argument("--no-reorder", action="store_true")
     ap.add_argument("--patch-cells", type=int, default=0)
+    ap.add_argument("--cells-per-wave", type=int, default=0, help="CG: cells per wave of a patch workgroup")
     ap.add_argument("--accumulate", type=int, default=0, help="CG LDS accumulation: 0 auto, 1 atomics, 2 read-add-write")
     ap.add_argument("--dg-kernel", type=int, default=0, help="DG kernel: 0 auto, 1 LDS tables, 2 compile-time tables")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="0 to skip the CPU leg")
@@ -195,7 +196,7 @@ def solve_mode(args):
     print(json.dumps(out), flush=True)
 
 
-def measure(smf, kind, n, degree, precision, steps, warmup, seed, reorder=True, patch_cells=0,
+def measure(smf, kind, n, degree, precision, steps, warmup, seed, reorder=True, patch_cells=0, cells_per_wave=0,
             accumulate=0, keep=False, dg_kernel=0):
     """Event-timed vmults of one operator on `stream`; returns the numbers of
     one bench line (and the operator and input when keep=True)."""
@@ -203,6 +204,7 @@ def measure(smf, kind, n, degree, precision, steps, warmup, seed, reorder=True, 
     mesh = smf.Mesh.kuhn_cube(n, 0.2, seed)
     op = smf.Operator(mesh, degree, kind=kind, precision=precision, reorder=reorder,
                       patch_cells=patch_cells, accumulate=accumulate, dg_kernel=dg_kernel, mass_coefficient=1.0,
+                      cells_per_wave=cells_per_wave,
                       laplace_coefficient=1.0)
     info = op.info
     log(f"setup {kind} p{degree} {precision} n{n}: {time.time() - t0:.1f}s")
@@ -282,6 +284,7 @@ def main():
         del a, b
     head = measure(smf, args.kind, args.n, args.degree, args.precision, args.steps, args.warmup,
                    args.seed, reorder=not args.no_reorder, patch_cells=args.patch_cells,
+                   cells_per_wave=args.cells_per_wave,
                    accumulate=args.accumulate, keep=args.cpu_baseline and args.kind == "cg",
                    dg_kernel=args.dg_kernel)
     info = head["info"]

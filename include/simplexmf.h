@@ -59,6 +59,8 @@ typedef struct mf_options {
                                DoFs, boundary faces add int(2 sigma u v - du/dn v - u dv/dn) with
                                sigma = (p+1)(p+3)/3 tau_K (the CG level under a DG operator in the hybrid
                                multigrid, DG/CG c-transfer, application_phc.cc:1424-1456, 1798-1832) */
+  int32_t cells_per_wave;   /* CG: cells per wavefront of a patch workgroup (0: 1024 P1, 512 P2, 128 P3);
+                               waves per patch = patch_cells / cells_per_wave, at most 4 */
 } mf_options;
 
 typedef struct mf_info {

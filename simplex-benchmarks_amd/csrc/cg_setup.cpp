@@ -41,7 +41,7 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   // cells (1 for the 3-component operator, whose LDS is 3x larger), halved
   // (down to 64 cells) until there are >= 1024 patches, so that small
   // operators (multigrid levels) still fill the 256 CUs
-  const int per_wave = p == 1 ? 1024 : (p == 2 ? 512 : 128);
+  const int per_wave = opt.cells_per_wave > 0 ? opt.cells_per_wave : (p == 1 ? 1024 : (p == 2 ? 512 : 128));
   int patch = opt.patch_cells;
   if (patch <= 0) {
     patch = (plan.ncomp == 3 ? 1 : 4) * per_wave;

@@ -38,7 +38,7 @@ class Options(C.Structure):
     _fields_ = [("reorder", C.c_int32), ("patch_cells", C.c_int32),
                 ("mass_coefficient", C.c_double), ("laplace_coefficient", C.c_double),
                 ("penalty_kind", C.c_int32), ("host_only", C.c_int32), ("accumulate", C.c_int32),
-                ("dg_kernel", C.c_int32), ("boundary", C.c_int32)]
+                ("dg_kernel", C.c_int32), ("boundary", C.c_int32), ("cells_per_wave", C.c_int32)]
 
 
 class Info(C.Structure):
@@ -356,7 +356,8 @@ class Operator:
 
     def __init__(self, mesh: Mesh, degree: int, kind="cg", precision="f64", reorder=True,
                  patch_cells=0, mass_coefficient=1.0, laplace_coefficient=1.0,
-                 penalty_kind=0, host_only=False, accumulate=0, dg_kernel=0, boundary="dirichlet"):
+                 penalty_kind=0, host_only=False, accumulate=0, dg_kernel=0, boundary="dirichlet",
+                 cells_per_wave=0):
         L = lib()
         opt = Options()
         L.mf_default_options(C.byref(opt))
@@ -369,6 +370,7 @@ class Operator:
         opt.accumulate = accumulate
         opt.dg_kernel = dg_kernel
         opt.boundary = {"dirichlet": 0, "nitsche": 1}[boundary] if isinstance(boundary, str) else int(boundary)
+        opt.cells_per_wave = cells_per_wave
         self.mesh = mesh
         self.kind = KINDS[kind] if isinstance(kind, str) else int(kind)
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
