@@ -54,7 +54,8 @@ typedef struct mf_options {
   int32_t penalty_kind;     /* 0: (p+1)(p+3)/3 max(tau-,tau+) (reference); 1: (p+1)(p+2)/h_face */
   int32_t host_only;        /* 1: build the host plan only (no device memory); device entry points fail */
   int32_t accumulate;       /* CG LDS accumulation: 0 auto, 1 LDS atomics, 2 read-add-write per colour */
-  int32_t dg_kernel;        /* DG kernel: 0 auto, 1 LDS tables, 2 compile-time tables + neighbour in face-3 frame */
+  int32_t dg_kernel;        /* DG kernel: 0 auto, 1 LDS tables, 2 compile-time tables + neighbour in face-3 frame,
+                               3 four lanes per cell (one face each) */
   int32_t boundary;         /* CG Poisson: 0 Dirichlet rows = identity (default); 1 Nitsche: no constrained
                                DoFs, boundary faces add int(2 sigma u v - du/dn v - u dv/dn) with
                                sigma = (p+1)(p+3)/3 tau_K (the CG level under a DG operator in the hybrid

@@ -392,5 +392,239 @@ SMF_HD void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, con
   }
 }
 
+// Share of cell-rule point q for one lane of the four-lane-per-cell form:
+// the lane works in the frame where its face is face 3, and takes the points
+// whose largest barycentric coordinate belongs to vertex 3 (ties split
+// evenly).  The rules are symmetric under vertex permutations, so over the
+// four lanes every point of the rule counts exactly once.
+template <int P, bool MASS>
+struct DGCellShare {
+  double s[DGCellRule<P, MASS>::NQ];
+};
+template <int P, bool MASS>
+constexpr DGCellShare<P, MASS> make_cell_share() {
+  DGCellShare<P, MASS> r{};
+  for (int q = 0; q < DGCellRule<P, MASS>::NQ; ++q) {
+    const double* x = DGCellRule<P, MASS>::pts[q];
+    const double l[4] = {1.0 - x[0] - x[1] - x[2], x[0], x[1], x[2]};
+    double m = l[0];
+    for (int a = 1; a < 4; ++a) m = l[a] > m ? l[a] : m;
+    int cnt = 0;
+    for (int a = 0; a < 4; ++a) cnt += (m - l[a] < 1e-12) ? 1 : 0;
+    r.s[q] = (m - l[3] < 1e-12) ? 1.0 / cnt : 0.0;
+  }
+  return r;
+}
+
+// Lane f of cell k, component c: the cell term's share (above) and the SIPG
+// flux of face f, evaluated with the cell's nodes renumbered so that face f
+// is face 3 (the neighbour's frame of apply_cell), so every table is the
+// compile-time face-3 table.  yrow[i] (original node order) receives the
+// lane's contribution to row i (MODE 0) or to the diagonal (MODE 1); the four
+// lanes' rows add up to apply_cell's result.
+template <int P, typename T, int NC, bool MASS, int MODE>
+SMF_HD void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __restrict__ geo,
+                            const int32_t* __restrict__ nbr, const uint32_t* __restrict__ nface, T factor, T mass,
+                            T lap, const T* __restrict__ src, T* __restrict__ yrow) {
+  constexpr DGCellTab<P, MASS> CT = make_cell_tab<P, MASS>();
+  constexpr DGCellShare<P, MASS> CS = make_cell_share<P, MASS>();
+  constexpr DGFaceTab<P> FT = make_face_tab<P>();
+  constexpr int NL = nloc(P), NQ = DGCellTab<P, MASS>::NQ, NQF = DGFaceTab<P>::NQF;
+  constexpr int NFN = DGFaceTab<P>::NFN, NFG = DGFaceTab<P>::NFG;
+  using std::sqrt;
+
+  // barycentric gradients grad lambda_a: a = 1..3 are the columns of J^-T
+  T gl[4][3];
+  for (int d = 0; d < 3; ++d) {
+    gl[1][d] = geo[(3 * d + 0) * ncells + k];
+    gl[2][d] = geo[(3 * d + 1) * ncells + k];
+    gl[3][d] = geo[(3 * d + 2) * ncells + k];
+    gl[0][d] = -(gl[1][d] + gl[2][d] + gl[3][d]);
+  }
+  const T adet = geo[9 * ncells + k], tau = geo[10 * ncells + k];
+  // own face f: J^-T nhat_f = -grad lambda_f; outward unit normal, area
+  T gf[3];
+  for (int d = 0; d < 3; ++d) gf[d] = select4<T>(f, gl[0][d], gl[1][d], gl[2][d], gl[3][d]);
+  const T len = sqrt(gf[0] * gf[0] + gf[1] * gf[1] + gf[2] * gf[2]);
+  const T il = T(1) / len;
+  const T n0 = -gf[0] * il, n1 = -gf[1] * il, n2 = -gf[2] * il;
+  const T area2 = adet * len * lap;
+  T bo[4];
+  for (int a = 0; a < 4; ++a) bo[a] = gl[a][0] * n0 + gl[a][1] * n1 + gl[a][2] * n2;
+  // lane frame: new vertex j <- old vertex sigma[f][j]; reference direction
+  // d (1..3) is new vertex d
+  T nJ[3], G[3][3];
+  for (int d = 0; d < 3; ++d) {
+    nJ[d] = select4<T>(f, bo[FT.sigma[0][d + 1]], bo[FT.sigma[1][d + 1]], bo[FT.sigma[2][d + 1]],
+                       bo[FT.sigma[3][d + 1]]);
+    for (int e = 0; e < 3; ++e)
+      G[d][e] = select4<T>(f, gl[FT.sigma[0][d + 1]][e], gl[FT.sigma[1][d + 1]][e], gl[FT.sigma[2][d + 1]][e],
+                           gl[FT.sigma[3][d + 1]][e]);
+  }
+  const T M0 = adet * (G[0][0] * G[0][0] + G[0][1] * G[0][1] + G[0][2] * G[0][2]);
+  const T M1 = adet * (G[0][0] * G[1][0] + G[0][1] * G[1][1] + G[0][2] * G[1][2]);
+  const T M2 = adet * (G[0][0] * G[2][0] + G[0][1] * G[2][1] + G[0][2] * G[2][2]);
+  const T M3 = adet * (G[1][0] * G[1][0] + G[1][1] * G[1][1] + G[1][2] * G[1][2]);
+  const T M4 = adet * (G[1][0] * G[2][0] + G[1][1] * G[2][1] + G[1][2] * G[2][2]);
+  const T M5 = adet * (G[2][0] * G[2][0] + G[2][1] * G[2][1] + G[2][2] * G[2][2]);
+  int pf[NL];
+  for (int i = 0; i < NL; ++i) pf[i] = select4<int>(f, FT.perm[0][i], FT.perm[1][i], FT.perm[2][i], FT.perm[3][i]);
+
+  const int64_t base = (k * NC + c) * NL;
+  T u[NL], y[NL];
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    u[i] = MODE == 0 ? src[base + pf[i]] : T(0);
+    y[i] = T(0);
+  }
+  // ---- the lane's share of the cell integral ----
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    if (CS.s[q] == 0.0) continue;
+    const T w = T(CT.w[q] * CS.s[q]);
+    if (MODE == 0) {
+      T gx = T(0), gy = T(0), gz = T(0), uv = T(0);
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        gx += T(CT.g[q][i][0]) * u[i];
+        gy += T(CT.g[q][i][1]) * u[i];
+        gz += T(CT.g[q][i][2]) * u[i];
+        if (MASS) uv += T(CT.v[q][i]) * u[i];
+      }
+      const T wl = w * lap;
+      const T tx = wl * (M0 * gx + M1 * gy + M2 * gz);
+      const T ty = wl * (M1 * gx + M3 * gy + M4 * gz);
+      const T tz = wl * (M2 * gx + M4 * gy + M5 * gz);
+      const T tv = MASS ? w * mass * adet * uv : T(0);
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        y[i] += T(CT.g[q][i][0]) * tx + T(CT.g[q][i][1]) * ty + T(CT.g[q][i][2]) * tz;
+        if (MASS) y[i] += T(CT.v[q][i]) * tv;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        const T gx = T(CT.g[q][i][0]), gy = T(CT.g[q][i][1]), gz = T(CT.g[q][i][2]);
+        y[i] += w * lap * (gx * (M0 * gx + M1 * gy + M2 * gz) + gy * (M1 * gx + M3 * gy + M4 * gz) +
+                           gz * (M2 * gx + M4 * gy + M5 * gz));
+        if (MASS) y[i] += w * mass * adet * T(CT.v[q][i]) * T(CT.v[q][i]);
+      }
+    }
+  }
+  // ---- face f (= face 3 of the lane frame) ----
+  const int32_t kb = nbr[f * ncells + k];
+  const bool interior = kb >= 0;
+  const T taub = geo[10 * ncells + (interior ? kb : k)];
+  const T sigma = factor * (interior ? (tau > taub ? tau : taub) : tau);
+  if (MODE == 0) {
+    T gn[NFG];
+#pragma unroll
+    for (int j = 0; j < NFG; ++j) {
+      T sx = T(0), sy = T(0), sz = T(0);
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        sx += T(FT.dg[3][j][i][0]) * u[i];
+        sy += T(FT.dg[3][j][i][1]) * u[i];
+        sz += T(FT.dg[3][j][i][2]) * u[i];
+      }
+      gn[j] = sx * nJ[0] + sy * nJ[1] + sz * nJ[2];
+    }
+    T ub[NL], gb[NFG];
+    if (interior) {
+      const int fb = (nface[k] >> (8 * f)) & 3;
+      const int64_t nbase = ((int64_t)kb * NC + c) * NL;
+#pragma unroll
+      for (int i = 0; i < NL; ++i)
+        ub[i] = src[nbase + select4<int>(fb, FT.perm[0][i], FT.perm[1][i], FT.perm[2][i], FT.perm[3][i])];
+      T bb[4];
+      {
+        T jb[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) jb[t] = geo[t * ncells + kb];
+        bb[1] = jb[0] * n0 + jb[3] * n1 + jb[6] * n2;
+        bb[2] = jb[1] * n0 + jb[4] * n1 + jb[7] * n2;
+        bb[3] = jb[2] * n0 + jb[5] * n1 + jb[8] * n2;
+        bb[0] = -(bb[1] + bb[2] + bb[3]);
+      }
+      T nb[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        nb[d] = select4<T>(fb, bb[FT.sigma[0][d + 1]], bb[FT.sigma[1][d + 1]], bb[FT.sigma[2][d + 1]],
+                           bb[FT.sigma[3][d + 1]]);
+#pragma unroll
+      for (int j = 0; j < NFG; ++j) {
+        T sx = T(0), sy = T(0), sz = T(0);
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          sx += T(FT.dg[3][j][i][0]) * ub[i];
+          sy += T(FT.dg[3][j][i][1]) * ub[i];
+          sz += T(FT.dg[3][j][i][2]) * ub[i];
+        }
+        gb[j] = sx * nb[0] + sy * nb[1] + sz * nb[2];
+      }
+    }
+    T tgj[NFG];
+#pragma unroll
+    for (int j = 0; j < NFG; ++j) tgj[j] = T(0);
+#pragma unroll
+    for (int q = 0; q < NQF; ++q) {
+      T uv = T(0), dn = T(0);
+#pragma unroll
+      for (int kk = 0; kk < NFN; ++kk) uv += T(FT.v[3][q][kk]) * u[FT.fn[3][kk]];
+#pragma unroll
+      for (int j = 0; j < NFG; ++j) dn += T(FT.psi[3][q][j]) * gn[j];
+      T up = -uv, dnp = dn;
+      if (interior) {
+        up = T(0);
+        dnp = T(0);
+#pragma unroll
+        for (int kk = 0; kk < NFN; ++kk) up += T(FT.v[3][q][kk]) * ub[FT.fn[3][kk]];
+#pragma unroll
+        for (int j = 0; j < NFG; ++j) dnp += T(FT.psi[3][q][j]) * gb[j];
+      }
+      const T jxw = T(FT.w[q]) * area2;
+      const T jump = uv - up, avg = T(0.5) * (dn + dnp);
+      const T tv = (sigma * jump - avg) * jxw, tg = T(-0.5) * jump * jxw;
+#pragma unroll
+      for (int kk = 0; kk < NFN; ++kk) y[FT.fn[3][kk]] += T(FT.v[3][q][kk]) * tv;
+#pragma unroll
+      for (int j = 0; j < NFG; ++j) tgj[j] += T(FT.psi[3][q][j]) * tg;
+    }
+#pragma unroll
+    for (int j = 0; j < NFG; ++j) {
+      const T ax = tgj[j] * nJ[0], ay = tgj[j] * nJ[1], az = tgj[j] * nJ[2];
+#pragma unroll
+      for (int i = 0; i < NL; ++i)
+        y[i] += T(FT.dg[3][j][i][0]) * ax + T(FT.dg[3][j][i][1]) * ay + T(FT.dg[3][j][i][2]) * az;
+    }
+  } else {
+    const T s = interior ? T(1) : T(2);
+    T dnode[NFG][NL];
+#pragma unroll
+    for (int j = 0; j < NFG; ++j)
+#pragma unroll
+      for (int i = 0; i < NL; ++i)
+        dnode[j][i] = T(FT.dg[3][j][i][0]) * nJ[0] + T(FT.dg[3][j][i][1]) * nJ[1] + T(FT.dg[3][j][i][2]) * nJ[2];
+#pragma unroll
+    for (int q = 0; q < NQF; ++q) {
+      const T jxw = T(FT.w[q]) * area2;
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        T vi = T(0);
+#pragma unroll
+        for (int kk = 0; kk < NFN; ++kk)
+          if (FT.fn[3][kk] == i) vi = T(FT.v[3][q][kk]);
+        T di = T(0);
+#pragma unroll
+        for (int j = 0; j < NFG; ++j) di += T(FT.psi[3][q][j]) * dnode[j][i];
+        const T jump = s * vi, avg = T(0.5) * s * di;
+        y[i] += jxw * ((sigma * jump - avg) * vi - T(0.5) * jump * di);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NL; ++i) yrow[pf[i]] = y[i];
+}
+
 }  // namespace dgt
 }  // namespace smf

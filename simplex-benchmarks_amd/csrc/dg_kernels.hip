@@ -217,9 +217,49 @@ __global__ __launch_bounds__(kThreads) void dg_cell_kernel_v2(int64_t ncells, co
   dgt::apply_cell<P, T, NC, MASS, MODE>(k, ncells, geo, nbr, nface, factor, mass, lap, src, dst);
 }
 
+// Four lanes per cell, lane f = face f (dgt::apply_cell_lane): a quarter of
+// the registers of the one-lane form, so four times the waves in flight to
+// cover the neighbour gathers.  The lanes' rows meet in LDS and are summed
+// in lane order (deterministic); each lane then stores every fourth row.
+constexpr int kQuadCells = 64;  // cells per 256-thread block
+template <int P, typename T, int NC, bool MASS, int MODE>
+__global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells, const T* __restrict__ geo,
+                                                                  const int32_t* __restrict__ nbr,
+                                                                  const uint32_t* __restrict__ nface, T factor,
+                                                                  T mass, T lap, const T* __restrict__ src,
+                                                                  T* __restrict__ dst) {
+  constexpr int NL = dgt::nloc(P);
+  __shared__ T ybuf[kQuadCells][4][NL + 1];
+  const int cl = threadIdx.x >> 2, f = threadIdx.x & 3;
+  const int64_t k0 = xcd_block(blockIdx.x, gridDim.x) * kQuadCells;
+  const int64_t k = k0 + cl;
+  const bool active = k < ncells;
+  const int64_t kc = active ? k : ncells - 1;  // every lane reaches the barriers
+#pragma unroll 1
+  for (int c = 0; c < NC; ++c) {
+    dgt::apply_cell_lane<P, T, NC, MASS, MODE>(kc, f, c, ncells, geo, nbr, nface, factor, mass, lap, src,
+                                               ybuf[cl][f]);
+    __syncthreads();
+    if (active)
+      for (int i = f; i < NL; i += 4)
+        dst[(k * NC + c) * NL + i] = ((ybuf[cl][0][i] + ybuf[cl][1][i]) + ybuf[cl][2][i]) + ybuf[cl][3][i];
+    __syncthreads();
+  }
+}
+
 template <int P, typename T, int NC, bool MASS, int MODE>
 void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
   const DGPlan& pl = op.dg;
+  // auto: four lanes per cell from P2 up (registers, not bandwidth, limit those)
+  if (op.opt.dg_kernel == 3 || (op.opt.dg_kernel == 0 && P >= 2)) {
+    if (pl.n_cells > 0)
+      hipLaunchKernelGGL((dg_quad_kernel<P, T, NC, MASS, MODE>),
+                         dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells)), dim3(4 * kQuadCells), 0, s,
+                         pl.n_cells, op.d_dg_geo.as<T>(), op.d_dg_nbr.as<int32_t>(), op.d_dg_nface.as<uint32_t>(),
+                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst);
+    SMF_HIP(hipGetLastError());
+    return;
+  }
   // compile-time-table kernel except P3 FP64, which exceeds the register file
   const bool v2 = op.opt.dg_kernel == 2 || (op.opt.dg_kernel == 0 && (P <= 2 || sizeof(T) == 4));
   if (v2) {

@@ -173,6 +173,24 @@ template <int P, typename T, int NC, bool MASS>
 void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
   const DGPlan& pl = op.dg;
   std::vector<T> geo(pl.geo.begin(), pl.geo.end());
+  if (op.opt.dg_kernel == 3) {
+    // four lanes per cell (one face each), summed in lane order
+    constexpr int NL = dgt::nloc(P);
+    T yr[4][NL];
+    for (int64_t k = 0; k < pl.n_cells; ++k)
+      for (int c = 0; c < NC; ++c) {
+        for (int f = 0; f < 4; ++f) {
+          if (mode == 0)
+            dgt::apply_cell_lane<P, T, NC, MASS, 0>(k, f, c, pl.n_cells, geo.data(), pl.nbr.data(), pl.nface.data(),
+                                                    (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, yr[f]);
+          else
+            dgt::apply_cell_lane<P, T, NC, MASS, 1>(k, f, c, pl.n_cells, geo.data(), pl.nbr.data(), pl.nface.data(),
+                                                    (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, yr[f]);
+        }
+        for (int i = 0; i < NL; ++i) dst[(k * NC + c) * NL + i] = ((yr[0][i] + yr[1][i]) + yr[2][i]) + yr[3][i];
+      }
+    return;
+  }
   for (int64_t k = 0; k < pl.n_cells; ++k) {
     if (mode == 0)
       dgt::apply_cell<P, T, NC, MASS, 0>(k, pl.n_cells, geo.data(), pl.nbr.data(), pl.nface.data(),

@@ -9,12 +9,15 @@ from conftest import rel_err
 TOL = {"f64": 1e-13, "f32": 3e-6}
 
 
+@pytest.mark.parametrize("dg_kernel", [2, 3])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 @pytest.mark.parametrize("kind", ["dg", "dg_vector", "dg_helmholtz"])
 @pytest.mark.parametrize("p", [1, 2, 3])
-def test_host_dg_matches_oracle(smf, orc, p, kind, precision):
+def test_host_dg_matches_oracle(smf, orc, p, kind, precision, dg_kernel):
+    """dg_kernel 2: one lane per cell; 3: four lanes per cell (one face each,
+    the lane's share of the cell rule), summed in lane order."""
     op = smf.Operator(smf.Mesh.kuhn_cube(3, 0.2, 2), p, kind=kind, precision=precision,
-                      host_only=True, mass_coefficient=1.0)
+                      host_only=True, mass_coefficient=1.0, dg_kernel=dg_kernel)
     pr = orc.DGOracleProblem.from_operator(op, mass=1.0 if kind == "dg_helmholtz" else 0.0)
     x = np.random.default_rng(4).uniform(-1, 1, op.m())
     assert rel_err(op.host_apply(x).astype(float), pr.vmult(x)) < TOL[precision]
