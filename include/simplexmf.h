@@ -51,7 +51,8 @@ typedef struct mf_options {
   int32_t patch_cells;      /* cells per patch (0: default for the degree) */
   double mass_coefficient;  /* Helmholtz mass coefficient (default 1) */
   double laplace_coefficient; /* Helmholtz Laplace coefficient (default 1) */
-  int32_t penalty_kind;     /* 0: (p+1)(p+3)/3 max(tau-,tau+) (reference); 1: (p+1)(p+2)/h_face */
+  int32_t penalty_kind;     /* 0: (p+1)(p+3)/3 max(tau-,tau+) (reference); 1: (p+1)(p+2)/h_face with
+                               h_face = |K|/|F| per side, the larger penalty of the two sides */
   int32_t host_only;        /* 1: build the host plan only (no device memory); device entry points fail */
   int32_t accumulate;       /* CG LDS accumulation: 0 auto, 1 LDS atomics, 2 read-add-write per colour */
   int32_t dg_kernel;        /* DG kernel: 0 auto, 1 LDS tables, 2 compile-time tables + neighbour in face-3 frame,

@@ -69,6 +69,8 @@ int orc_dg_vmult_omp(const orc_mesh* m, int p, const int64_t* cell_dofs, int64_t
                      double mass, double laplace, const double* src, double* dst);
 int orc_dg_diagonal(const orc_mesh* m, int p, const int64_t* cell_dofs, int64_t n_dofs, int ncomp,
                     double mass, double laplace, double* diag);
+/* penalty variant for the DG functions: 0 reference (default), 1 (p+1)(p+2)/h_F, h_F = |K|/|F| */
+int orc_dg_set_penalty_kind(int kind);
 /* per-cell penalty tau_K (n_cells) */
 int orc_dg_tau(const orc_mesh* m, double* tau);
 

@@ -23,6 +23,16 @@
 
 static int nloc_of(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
 
+/* penalty variant: 0 = (p+1)(p+3)/3 max(tau-, tau+) (reference,
+ * DG/cells_batched/application.cc:1193-1245); 1 = (p+1)(p+2)/h_F with
+ * h_F = |K|/|F| on each side, the larger value over the two sides */
+static int g_penalty_kind = 0;
+int orc_dg_set_penalty_kind(int kind) {
+  if (kind != 0 && kind != 1) return -1;
+  g_penalty_kind = kind;
+  return 0;
+}
+
 typedef struct geo {
   double x0[3], J[3][3], Jinv[3][3], det;
 } geo;
@@ -250,7 +260,13 @@ static int dg_apply(const dgctx* c, const double* src, double* dst, double* diag
       if (s < 0)
         for (int d = 0; d < 3; ++d) n[d] = -n[d];
     }
-    const double sigma = interior ? c->factor * fmax(c->tau[cm], c->tau[cp]) : c->factor * c->tau[cm];
+    double sigma;
+    if (g_penalty_kind == 1) {
+      const double hm = fabs(gm.det) / 6.0 / area, hp = interior ? fabs(gp.det) / 6.0 / area : hm;
+      sigma = (c->p + 1.0) * (c->p + 2.0) * fmax(1.0 / hm, 1.0 / hp);
+    } else {
+      sigma = interior ? c->factor * fmax(c->tau[cm], c->tau[cp]) : c->factor * c->tau[cm];
+    }
     const int64_t* dm = c->cd + cm * nl;
     const int64_t* dp = interior ? c->cd + cp * nl : NULL;
     for (int q = 0; q < nqf; ++q) {

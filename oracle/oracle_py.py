@@ -28,7 +28,7 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         for name in ["orc_cg_vmult", "orc_cg_diagonal", "orc_cg_matrix_vmult", "orc_cg_dense",
-                     "orc_dg_vmult", "orc_dg_diagonal", "orc_dg_tau", "orc_dg_vmult_omp"]:
+                     "orc_dg_vmult", "orc_dg_diagonal", "orc_dg_tau", "orc_dg_vmult_omp", "orc_dg_set_penalty_kind"]:
             getattr(_lib, name).restype = C.c_int
         _lib.orc_basis.argtypes = [C.c_int, vp, vp, vp]
         _lib.orc_tet_rule.argtypes = [C.c_int, vp, vp]
@@ -111,7 +111,8 @@ class DGOracleProblem:
     """DG SIPG operator (optionally vector / Helmholtz) on the operator's
     internal cells; rows (k*ncomp + c)*n_loc + i."""
 
-    def __init__(self, vertices, cells, degree, ncomp=1, mass=0.0, laplace=1.0):
+    def __init__(self, vertices, cells, degree, ncomp=1, mass=0.0, laplace=1.0, penalty_kind=0):
+        self.penalty_kind = int(penalty_kind)
         self.vertices = np.ascontiguousarray(vertices, np.float64)
         self.cells = np.ascontiguousarray(cells, np.int32)
         self.degree = int(degree)
@@ -133,9 +134,9 @@ class DGOracleProblem:
         self._m = OrcMesh(self.vertices.shape[0], self.vertices.ctypes.data, nc, self.cells.ctypes.data)
 
     @staticmethod
-    def from_operator(op, mass=0.0, laplace=1.0):
+    def from_operator(op, mass=0.0, laplace=1.0, penalty_kind=0):
         return DGOracleProblem(op.mesh.vertices, op.cell_vertices(), op.info["degree"],
-                               op.info["n_components"], mass, laplace)
+                               op.info["n_components"], mass, laplace, penalty_kind)
 
     def _to_orc(self, x):
         y = np.empty(x.size)
@@ -150,6 +151,7 @@ class DGOracleProblem:
     def vmult(self, src):
         s = self._to_orc(np.asarray(src, np.float64))
         d = np.empty_like(s)
+        lib().orc_dg_set_penalty_kind(self.penalty_kind)
         rc = lib().orc_dg_vmult(C.byref(self._m), self.degree, C.c_void_p(self.cell_dofs.ctypes.data),
                                 C.c_int64(self.n_scalar), self.ncomp, C.c_double(self.mass),
                                 C.c_double(self.laplace), C.c_void_p(s.ctypes.data), C.c_void_p(d.ctypes.data))
@@ -160,6 +162,7 @@ class DGOracleProblem:
     def vmult_omp(self, src):
         s = self._to_orc(np.asarray(src, np.float64))
         d = np.empty_like(s)
+        lib().orc_dg_set_penalty_kind(self.penalty_kind)
         rc = lib().orc_dg_vmult_omp(C.byref(self._m), self.degree, C.c_void_p(self.cell_dofs.ctypes.data),
                                     C.c_int64(self.n_scalar), self.ncomp, C.c_double(self.mass),
                                     C.c_double(self.laplace), C.c_void_p(s.ctypes.data),
@@ -170,6 +173,7 @@ class DGOracleProblem:
 
     def diagonal(self):
         d = np.empty(self.n_scalar * self.ncomp)
+        lib().orc_dg_set_penalty_kind(self.penalty_kind)
         rc = lib().orc_dg_diagonal(C.byref(self._m), self.degree, C.c_void_p(self.cell_dofs.ctypes.data),
                                    C.c_int64(self.n_scalar), self.ncomp, C.c_double(self.mass),
                                    C.c_double(self.laplace), C.c_void_p(d.ctypes.data))

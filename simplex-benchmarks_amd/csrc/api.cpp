@@ -218,6 +218,7 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
     if (opt && (opt->accumulate < 0 || opt->accumulate > 2)) throw std::runtime_error("accumulate must be 0, 1 or 2");
     if (opt && (opt->boundary < 0 || opt->boundary > 1)) throw std::runtime_error("boundary must be 0 or 1");
     if (opt && (opt->dg_kernel < 0 || opt->dg_kernel > 3)) throw std::runtime_error("dg_kernel must be 0..3");
+    if (opt && (opt->penalty_kind < 0 || opt->penalty_kind > 1)) throw std::runtime_error("penalty_kind must be 0 or 1");
     if (opt && opt->boundary == 1 && kind != MF_CG_POISSON)
       throw std::runtime_error("Nitsche boundary (boundary = 1) is available for the scalar CG operator");
     if (precision != MF_F64 && precision != MF_F32) throw std::runtime_error("precision must be MF_F64 or MF_F32");
@@ -273,9 +274,11 @@ int mf_destroy(mf_op_t op) {
 
 int mf_vmult(mf_op_t h, void* dst, const void* src, void* stream) {
   MF_TRY({
-    if (!h || !dst || !src) throw std::runtime_error("null argument");
-    if (dst == src) throw std::runtime_error("dst and src must not alias");
+    if (!h) throw std::runtime_error("null argument");
+    if (dst && dst == src) throw std::runtime_error("dst and src must not alias");
     require_device(h->op);
+    if (h->op.m() == 0) return 0;  // empty mesh: nothing to do (vectors may be null)
+    if (!dst || !src) throw std::runtime_error("null argument");
     if (h->op.is_dg()) launch_dg(h->op, 0, dst, src, static_cast<hipStream_t>(stream));
     else launch_cg(h->op, 0, dst, src, static_cast<hipStream_t>(stream));
   });
@@ -287,8 +290,10 @@ int mf_tvmult(mf_op_t h, void* dst, const void* src, void* stream) {
 
 int mf_compute_diagonal(mf_op_t h, void* diag, void* stream) {
   MF_TRY({
-    if (!h || !diag) throw std::runtime_error("null argument");
+    if (!h) throw std::runtime_error("null argument");
     require_device(h->op);
+    if (h->op.m() == 0) return 0;
+    if (!diag) throw std::runtime_error("null argument");
     if (h->op.is_dg()) launch_dg(h->op, 1, diag, nullptr, static_cast<hipStream_t>(stream));
     else launch_cg(h->op, 1, diag, nullptr, static_cast<hipStream_t>(stream));
   });
@@ -296,8 +301,10 @@ int mf_compute_diagonal(mf_op_t h, void* diag, void* stream) {
 
 int mf_compute_inverse_diagonal(mf_op_t h, void* inv, void* stream) {
   MF_TRY({
-    if (!h || !inv) throw std::runtime_error("null argument");
+    if (!h) throw std::runtime_error("null argument");
     require_device(h->op);
+    if (h->op.m() == 0) return 0;
+    if (!inv) throw std::runtime_error("null argument");
     const hipStream_t s = static_cast<hipStream_t>(stream);
     if (h->op.is_dg()) launch_dg(h->op, 1, inv, nullptr, s);
     else launch_cg(h->op, 1, inv, nullptr, s);

@@ -177,8 +177,7 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
       const int64_t r = raw_of(perm[k], i);
       if (number[r] < 0) number[r] = next++;
     }
-  plan.n_dofs = next;
-  if (next != n_raw) throw std::runtime_error("DoF numbering left unreached entities");
+  plan.n_dofs = next;  // entities no cell touches (unused vertices) get no DoF
   if (plan.n_dofs > INT32_MAX) throw std::runtime_error("too many DoFs for 32-bit indices");
 
   // ---- global cell table, support points ----
@@ -270,7 +269,7 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
       for (int b = 0; b < 3; ++b) plan.jit[(size_t)(3 * a + b) * nc + k] = g.Jinv[b][a];
     plan.detj[k] = g.det;
   }
-  if (plan.nitsche) build_nitsche_boundary(mesh, plan);
+  if (plan.nitsche) build_nitsche_boundary(mesh, plan, opt.penalty_kind);
 }
 
 namespace {
@@ -302,10 +301,10 @@ std::vector<double> cell_tau(const Mesh& mesh) {
 
 // B_ij = sum over boundary faces F of the cell of
 //   int_F 2 sigma phi_j phi_i - (n.grad phi_j) phi_i - phi_j (n.grad phi_i),
-// sigma = (p+1)(p+3)/3 tau_K: the boundary part of the SIPG form for
+// sigma = (p+1)(p+3)/3 tau_K (or (p+1)(p+2)|F|/|K|): the boundary part of the SIPG form for
 // continuous functions (DG/cells_batched/application.cc:2688-2701 with the
 // mirror principle; as the CG level of application_phc.cc:1798-1832).
-void build_nitsche_boundary(const Mesh& mesh, CGPlan& plan) {
+void build_nitsche_boundary(const Mesh& mesh, CGPlan& plan, int penalty_kind) {
   const int p = plan.p, nl = plan.n_loc;
   const Lagrange fe(p);
   const Quadrature qf = tri_rule(2 * p);
@@ -335,7 +334,8 @@ void build_nitsche_boundary(const Mesh& mesh, CGPlan& plan) {
     }
     const double area = 0.5 * std::fabs(g.det) * std::sqrt(nn);
     for (int a = 0; a < 3; ++a) n[a] /= std::sqrt(nn);
-    const double sigma = factor * tau[c];
+    // penalty_kind 1: (p+1)(p+2) |F| / |K|
+    const double sigma = penalty_kind == 1 ? (p + 1.0) * (p + 2.0) * area / (std::fabs(g.det) / 6.0) : factor * tau[c];
     const int64_t k = inv[c];
     std::vector<double> B(nl * nl, 0.0);
     for (int q = 0; q < qf.size(); ++q) {

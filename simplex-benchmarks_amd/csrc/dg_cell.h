@@ -193,7 +193,7 @@ SMF_HD inline T select4(int f, T a0, T a1, T a2, T a3) {
 template <int P, typename T, int NC, bool MASS, int MODE>
 SMF_HD void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, const int32_t* __restrict__ nbr,
                        const uint32_t* __restrict__ nface, T factor, T mass, T lap, const T* __restrict__ src,
-                       T* __restrict__ dst) {
+                       T* __restrict__ dst, int pkind = 0) {
   constexpr DGCellTab<P, MASS> CT = make_cell_tab<P, MASS>();
   constexpr DGFaceTab<P> FT = make_face_tab<P>();
   constexpr int NL = nloc(P), NQ = DGCellTab<P, MASS>::NQ, NQF = DGFaceTab<P>::NQF;
@@ -274,8 +274,14 @@ SMF_HD void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, con
       const int32_t kb = nbr[f * ncells + k];
       const bool interior = kb >= 0;
       const int64_t kbs = interior ? kb : k;
-      const T taub = geo[10 * ncells + kbs];
-      const T sigma = factor * (interior ? (tau > taub ? tau : taub) : tau);
+      T sigma;
+      if (pkind == 1) {  // (p+1)(p+2) |F| / min |K|: factor * max(len, len |det J| / |det J+|)
+        const T lb = len * adet / geo[9 * ncells + kbs];
+        sigma = factor * (len > lb ? len : lb);
+      } else {
+        const T taub = geo[10 * ncells + kbs];
+        sigma = factor * (interior ? (tau > taub ? tau : taub) : tau);
+      }
 
       if (MODE == 0) {
         // own normal derivative at the P_{p-1} face nodes
@@ -425,7 +431,7 @@ constexpr DGCellShare<P, MASS> make_cell_share() {
 template <int P, typename T, int NC, bool MASS, int MODE>
 SMF_HD void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __restrict__ geo,
                             const int32_t* __restrict__ nbr, const uint32_t* __restrict__ nface, T factor, T mass,
-                            T lap, const T* __restrict__ src, T* __restrict__ yrow) {
+                            T lap, const T* __restrict__ src, T* __restrict__ yrow, int pkind = 0) {
   constexpr DGCellTab<P, MASS> CT = make_cell_tab<P, MASS>();
   constexpr DGCellShare<P, MASS> CS = make_cell_share<P, MASS>();
   constexpr DGFaceTab<P> FT = make_face_tab<P>();
@@ -514,8 +520,14 @@ SMF_HD void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __
   // ---- face f (= face 3 of the lane frame) ----
   const int32_t kb = nbr[f * ncells + k];
   const bool interior = kb >= 0;
-  const T taub = geo[10 * ncells + (interior ? kb : k)];
-  const T sigma = factor * (interior ? (tau > taub ? tau : taub) : tau);
+  T sigma;
+  if (pkind == 1) {  // (p+1)(p+2) |F| / min |K|: factor * max(len, len |det J| / |det J+|)
+    const T lb = len * adet / geo[9 * ncells + (interior ? kb : k)];
+    sigma = factor * (len > lb ? len : lb);
+  } else {
+    const T taub = geo[10 * ncells + (interior ? kb : k)];
+    sigma = factor * (interior ? (tau > taub ? tau : taub) : tau);
+  }
   if (MODE == 0) {
     T gn[NFG];
 #pragma unroll

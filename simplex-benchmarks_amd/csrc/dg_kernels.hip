@@ -211,10 +211,10 @@ __global__ __launch_bounds__(kThreads) void dg_cell_kernel_v2(int64_t ncells, co
                                                                const int32_t* __restrict__ nbr,
                                                                const uint32_t* __restrict__ nface, T factor, T mass,
                                                                T lap, const T* __restrict__ src,
-                                                               T* __restrict__ dst) {
+                                                               T* __restrict__ dst, int pkind) {
   const int64_t k = xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
   if (k >= ncells) return;
-  dgt::apply_cell<P, T, NC, MASS, MODE>(k, ncells, geo, nbr, nface, factor, mass, lap, src, dst);
+  dgt::apply_cell<P, T, NC, MASS, MODE>(k, ncells, geo, nbr, nface, factor, mass, lap, src, dst, pkind);
 }
 
 // Four lanes per cell, lane f = face f (dgt::apply_cell_lane): a quarter of
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells,
                                                                   const int32_t* __restrict__ nbr,
                                                                   const uint32_t* __restrict__ nface, T factor,
                                                                   T mass, T lap, const T* __restrict__ src,
-                                                                  T* __restrict__ dst) {
+                                                                  T* __restrict__ dst, int pkind) {
   constexpr int NL = dgt::nloc(P);
   __shared__ T ybuf[kQuadCells][4][NL + 1];
   const int cl = threadIdx.x >> 2, f = threadIdx.x & 3;
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells,
 #pragma unroll 1
   for (int c = 0; c < NC; ++c) {
     dgt::apply_cell_lane<P, T, NC, MASS, MODE>(kc, f, c, ncells, geo, nbr, nface, factor, mass, lap, src,
-                                               ybuf[cl][f]);
+                                               ybuf[cl][f], pkind);
     __syncthreads();
     if (active)
       for (int i = f; i < NL; i += 4)
@@ -256,7 +256,7 @@ void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
       hipLaunchKernelGGL((dg_quad_kernel<P, T, NC, MASS, MODE>),
                          dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells)), dim3(4 * kQuadCells), 0, s,
                          pl.n_cells, op.d_dg_geo.as<T>(), op.d_dg_nbr.as<int32_t>(), op.d_dg_nface.as<uint32_t>(),
-                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst);
+                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst, pl.penalty_kind);
     SMF_HIP(hipGetLastError());
     return;
   }
@@ -267,12 +267,13 @@ void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
       hipLaunchKernelGGL((dg_cell_kernel_v2<P, T, NC, MASS, MODE>),
                          dim3((unsigned)((pl.n_cells + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, pl.n_cells,
                          op.d_dg_geo.as<T>(), op.d_dg_nbr.as<int32_t>(), op.d_dg_nface.as<uint32_t>(),
-                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst);
+                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst, pl.penalty_kind);
     SMF_HIP(hipGetLastError());
     return;
   }
   using S = DGShape<P, MASS>;
   if (pl.n_q != S::NQ || pl.n_qf != S::NQF) throw std::runtime_error("DG table size mismatch");
+  if (pl.penalty_kind != 0) throw std::runtime_error("penalty_kind 1 needs dg_kernel 0, 2 or 3");
   const size_t shmem = (size_t)(S::NQ * S::NL * 4 + S::NQ + 4 * S::NQF * S::NL * 4 + S::NQF) * sizeof(T);
   auto kern = dg_cell_kernel<P, T, NC, MASS, MODE>;
   if (shmem > 65536)

@@ -35,7 +35,8 @@ void build_dg_plan(const Mesh& mesh, int p, int ncomp, const mf_options& opt, bo
   plan.ncomp = ncomp;
   plan.n_cells = nc;
   plan.n_dofs = nc * ncomp * nl;
-  plan.penalty_factor = (p + 1.0) * (p + 3.0) / 3.0;
+  plan.penalty_kind = opt.penalty_kind;
+  plan.penalty_factor = opt.penalty_kind == 1 ? 3.0 * (p + 1.0) * (p + 2.0) : (p + 1.0) * (p + 3.0) / 3.0;
   plan.mass = helmholtz ? opt.mass_coefficient : 0.0;
   plan.laplace = helmholtz ? opt.laplace_coefficient : 1.0;
 
@@ -182,10 +183,12 @@ void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
         for (int f = 0; f < 4; ++f) {
           if (mode == 0)
             dgt::apply_cell_lane<P, T, NC, MASS, 0>(k, f, c, pl.n_cells, geo.data(), pl.nbr.data(), pl.nface.data(),
-                                                    (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, yr[f]);
+                                                    (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, yr[f],
+                                                    pl.penalty_kind);
           else
             dgt::apply_cell_lane<P, T, NC, MASS, 1>(k, f, c, pl.n_cells, geo.data(), pl.nbr.data(), pl.nface.data(),
-                                                    (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, yr[f]);
+                                                    (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, yr[f],
+                                                    pl.penalty_kind);
         }
         for (int i = 0; i < NL; ++i) dst[(k * NC + c) * NL + i] = ((yr[0][i] + yr[1][i]) + yr[2][i]) + yr[3][i];
       }
@@ -194,10 +197,10 @@ void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
   for (int64_t k = 0; k < pl.n_cells; ++k) {
     if (mode == 0)
       dgt::apply_cell<P, T, NC, MASS, 0>(k, pl.n_cells, geo.data(), pl.nbr.data(), pl.nface.data(),
-                                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst);
+                                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst, pl.penalty_kind);
     else
       dgt::apply_cell<P, T, NC, MASS, 1>(k, pl.n_cells, geo.data(), pl.nbr.data(), pl.nface.data(),
-                                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst);
+                                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst, pl.penalty_kind);
   }
 }
 

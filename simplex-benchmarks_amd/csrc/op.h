@@ -117,6 +117,10 @@ struct DGPlan {
   // values [f][q][i], gradients [f][q][i][d], weights [q]
   std::vector<double> cell_tab, face_tab;
   double penalty_factor = 0, mass = 0, laplace = 1;
+  // 0: sigma = (p+1)(p+3)/3 max(tau-, tau+) (reference); 1: sigma = (p+1)(p+2)
+  // max(|F|/|K-|, |F|/|K+|) = penalty_factor max(len, len |det J-|/|det J+|)
+  // with len = |J^-T nhat_f| (|F|/|K| = 3 len), penalty_factor = 3 (p+1)(p+2)
+  int penalty_kind = 0;
 };
 
 #ifdef __HIPCC__
@@ -151,7 +155,7 @@ struct Operator {
 
 void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan);
 // boundary-face matrix of the Nitsche CG operator (plan.nitsche)
-void build_nitsche_boundary(const Mesh& mesh, CGPlan& plan);
+void build_nitsche_boundary(const Mesh& mesh, CGPlan& plan, int penalty_kind);
 // SIPG penalty tau_K = (sum_bdry |F| + 1/2 sum_int |F|) / |K| per input cell
 std::vector<double> cell_tau(const Mesh& mesh);
 // any operator: dispatch to the CG or DG launcher

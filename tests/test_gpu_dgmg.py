@@ -131,3 +131,26 @@ def test_dg_learned_smoother(dgmg):
     ls.set_params(np.zeros(smf.LS_N_PARAMS, np.float32))
     s0 = smf.solve_fcg_learned(A, xl, b, mg2, ls, 1e-10, 200)
     assert s0["converged"] and s0["fallback"] == 1
+
+
+@pytest.mark.parametrize("dg_kernel", [0, 2])
+def test_dg_penalty_variant_gpu(gpu, orc, dg_kernel):
+    smf = gpu
+    mesh = smf.Mesh.kuhn_cube(6, perturb=0.2, seed=4)
+    A = smf.Operator(mesh, 2, kind="dg", penalty_kind=1, dg_kernel=dg_kernel)
+    x = np.random.default_rng(1).uniform(-1, 1, A.m())
+    y = A.initialize_dof_vector()
+    A.vmult(y, smf.DeviceVector.from_numpy(x))
+    want = orc.DGOracleProblem.from_operator(A, penalty_kind=1).vmult(x)
+    assert np.max(np.abs(y.numpy() - want)) <= 1e-12 * np.max(np.abs(want))
+    mg = smf.Multigrid(A)  # the CG (Nitsche) levels inherit the penalty variant
+    b = smf.DeviceVector.from_numpy(smf.assemble_rhs(A, "grf", seed=3))
+    st = smf.solve_cg(A, A.initialize_dof_vector(), b, mg, rel_tol=1e-10, max_iter=100)
+    assert st["converged"] and st["iterations"] <= 25
+
+
+def test_penalty_variant_rejected_by_table_kernel(gpu):
+    smf = gpu
+    A = smf.Operator(smf.Mesh.kuhn_cube(2), 2, kind="dg", penalty_kind=1, dg_kernel=1)
+    with pytest.raises(smf.MFError, match="penalty_kind"):
+        A.vmult(A.initialize_dof_vector(), A.initialize_dof_vector())

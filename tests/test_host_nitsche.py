@@ -58,3 +58,16 @@ def test_nitsche_rejected_for_other_kinds(smf):
         smf.Operator(mesh, 2, kind="dg", boundary="nitsche", host_only=True)
     with pytest.raises(smf.MFError, match="Nitsche"):
         smf.Operator(mesh, 2, kind="cg_vector", boundary="nitsche", host_only=True)
+
+
+def test_nitsche_penalty_variant(smf, orc):
+    mesh = smf.Mesh.kuhn_cube(3, perturb=0.2, seed=5)
+    cg = smf.Operator(mesh, 2, kind="cg", boundary="nitsche", penalty_kind=1, host_only=True)
+    dg = smf.Operator(mesh, 2, kind="dg", host_only=True)
+    cols = cg_to_dg(cg, dg)
+    dgo = orc.DGOracleProblem.from_operator(dg, penalty_kind=1)
+    x = np.random.default_rng(9).uniform(-1, 1, cg.m())
+    want = np.zeros(cg.m())
+    np.add.at(want, cols, dgo.vmult(x[cols]))
+    y = cg.host_apply(x)
+    assert np.max(np.abs(y - want)) <= 1e-12 * np.max(np.abs(want))

@@ -31,3 +31,20 @@ def test_dg_sizes(smf):
     assert op.info["n_boundary_faces"] == 12 * n ** 2
     assert op.info["n_interior_faces"] == (4 * 6 * n ** 3 - 12 * n ** 2) // 2
     assert op.info["n_constrained"] == 0
+
+
+@pytest.mark.parametrize("dg_kernel", [2, 3])
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_host_dg_penalty_variant(smf, orc, p, dg_kernel):
+    """penalty_kind 1: sigma = (p+1)(p+2)/h_F with h_F = |K|/|F| per side (the
+    config's variant) against the oracle with the same penalty; it differs
+    from the reference penalty."""
+    m = smf.Mesh.kuhn_cube(3, 0.2, 2)
+    op = smf.Operator(m, p, kind="dg", host_only=True, penalty_kind=1, dg_kernel=dg_kernel)
+    ref = smf.Operator(m, p, kind="dg", host_only=True, dg_kernel=dg_kernel)
+    pr = orc.DGOracleProblem.from_operator(op, penalty_kind=1)
+    x = np.random.default_rng(5).uniform(-1, 1, op.m())
+    y = op.host_apply(x)
+    assert rel_err(y, pr.vmult(x)) < 1e-13
+    assert rel_err(op.host_apply(None, mode=1), pr.diagonal()) < 1e-13
+    assert rel_err(y, ref.host_apply(x)) > 1e-3
