@@ -62,6 +62,10 @@ void upload_cg(Operator& op) {
     op.d_bnd_col.upload(pl.bnd_col);
     if (op.precision == MF_F64) op.d_bnd_val.upload(pl.bnd_val);
     else op.d_bnd_val.upload(convert<float>(pl.bnd_val));
+    // dense boundary row sums for the fused epilogues: rows without boundary
+    // entries stay zero
+    op.d_bnd_dense.alloc(std::max<int64_t>(pl.n_dofs, 1) * op.wbytes());
+    SMF_HIP(hipMemset(op.d_bnd_dense.p, 0, op.d_bnd_dense.bytes));
   }
   op.on_device = true;
 }

@@ -26,6 +26,8 @@ void blas_csr(int64_t nrows, const int64_t* ptr, const int32_t* col, const T* va
 template <typename T>
 void blas_cheby_step(int64_t n, T c1, T c2, const T* dinv, const T* b, const T* ax, T* d, T* x, bool first,
                      hipStream_t s);
+template <typename T>
+void blas_cheby_zero(int64_t n, T c2, const T* dinv, const T* b, T* d, T* x, hipStream_t s);  // d = x = c2 Dinv b
 void blas_dense_gemv(int n, const double* M, const double* x, double* y, hipStream_t s);
 
 }  // namespace smf

@@ -103,6 +103,17 @@ __global__ void cheby_step_kernel(int64_t n, T c1, T c2, const T* __restrict__ d
   x[i] += di;
 }
 
+// first Chebyshev step from x = 0: d = c2 Dinv b, x = d
+template <typename T>
+__global__ void cheby_zero_kernel(int64_t n, T c2, const T* __restrict__ dinv, const T* __restrict__ b,
+                                  T* __restrict__ d, T* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T di = c2 * dinv[i] * b[i];
+  d[i] = di;
+  x[i] = di;
+}
+
 // y = M x for a dense row-major n x n matrix (FP64)
 __global__ void dense_gemv_kernel(int n, const double* __restrict__ M, const double* __restrict__ x,
                                   double* __restrict__ y) {
@@ -154,12 +165,17 @@ void blas_cheby_step(int64_t n, T c1, T c2, const T* dinv, const T* b, const T* 
     hipLaunchKernelGGL(cheby_step_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, c1, c2, dinv, b, ax, d, x,
                        first ? 1 : 0);
 }
+template <typename T>
+void blas_cheby_zero(int64_t n, T c2, const T* dinv, const T* b, T* d, T* x, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(cheby_zero_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, c2, dinv, b, d, x);
+}
 void blas_dense_gemv(int n, const double* M, const double* x, double* y, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(dense_gemv_kernel, dim3((n + 63) / 64), dim3(64), 0, s, n, M, x, y);
 }
 int blas_reduction_parts() { return kRedBlocks; }
 
 #define SMF_INST(T)                                                                                   \
+  template void blas_cheby_zero<T>(int64_t, T, const T*, const T*, T*, T*, hipStream_t);           \
   template void blas_axpby<T>(int64_t, double, const T*, double, T*, hipStream_t);                 \
   template void blas_axpby_t<T>(int64_t, T, const T*, T, T*, hipStream_t);                        \
   template void blas_scale<T>(int64_t, const T*, const T*, T*, hipStream_t);                       \

@@ -44,6 +44,19 @@ struct CGShape {
   static constexpr int NQ = P == 1 ? 1 : (P == 2 ? 4 : 11);
 };
 
+template <typename T>
+__device__ inline void apply_epilogue(const CgEpilogue<T>& ep, int64_t i, T ax) {
+  if (ep.bnd) ax += ep.bnd[i];
+  if (ep.kind == 1) {
+    const T r = ep.b[i] - ax;
+    const T di = ep.first ? ep.c2 * ep.dinv[i] * r : ep.c1 * ep.d[i] + ep.c2 * ep.dinv[i] * r;
+    ep.d[i] = di;
+    ep.x[i] += di;
+  } else {
+    ep.r[i] = ep.b[i] - ax;
+  }
+}
+
 // MODE 0: y = A_K u;  MODE 1: y_i = (A_K)_ii
 // shape: G[(q*NL + i)*3 + d] = d phi_i / d xhat_d at point q, then w[q].
 // ATOMIC: accumulate with ds_add (fast for f64 on gfx950) or with a plain
@@ -56,8 +69,8 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ own_start,
     const int32_t* __restrict__ ghost_start, const int32_t* __restrict__ ghost_idx,
     const uint8_t* __restrict__ ccol, const uint16_t* __restrict__ lidx, int64_t ncells_total,
-    const T* __restrict__ metric, const T* __restrict__ shape, const T* __restrict__ src,
-    T* __restrict__ dst, T* __restrict__ partial, int64_t cstride, int64_t pstride) {
+    const T* __restrict__ metric, const T* __restrict__ shape, const T* src,
+    T* __restrict__ dst, T* __restrict__ partial, int64_t cstride, int64_t pstride, CgEpilogue<T> ep) {
   constexpr int NL = CGShape<P>::NL, NQ = CGShape<P>::NQ;
   extern __shared__ unsigned char smem_raw[];
   T* sbase = reinterpret_cast<T*>(smem_raw);
@@ -221,10 +234,12 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     T* dc = dst + cc * cstride;
     T* pc = partial + cc * pstride;
     for (int j = lane; j < nloc; j += nt) {
-      if (j < no)
+      if (j >= no)
+        pc[g0 + j - no] = sy[j];
+      else if (ep.kind == 0)
         dc[o0 + j] = sy[j];
       else
-        pc[g0 + j - no] = sy[j];
+        apply_epilogue(ep, o0 + j, sy[j]);  // owned DoFs are read by this patch only
     }
   }
 }
@@ -235,8 +250,8 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
 template <typename T, int MODE>
 __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* __restrict__ fix_ptr,
                                 const int32_t* __restrict__ fix_src, const T* __restrict__ partial,
-                                int64_t n0, int64_t n1, const T* __restrict__ src,
-                                T* __restrict__ dst, int nc, int64_t cstride, int64_t pstride) {
+                                int64_t n0, int64_t n1, const T* src,
+                                T* __restrict__ dst, int nc, int64_t cstride, int64_t pstride, CgEpilogue<T> ep) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int c = 0; c < nc; ++c) {
     const T* pc = partial + c * pstride;
@@ -244,15 +259,20 @@ __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* _
       const int32_t k0 = fix_ptr[t], k1 = fix_ptr[t + 1];
       T s = pc[fix_src[k0]];
       for (int32_t k = k0 + 1; k < k1; ++k) s += pc[fix_src[k]];
-      dst[c * cstride + n_int + t] = s;
+      if (ep.kind == 0) dst[c * cstride + n_int + t] = s;
+      else apply_epilogue(ep, n_int + t, s);
     }
     const int64_t j = n0 + t;
-    if (j < n1) dst[c * cstride + j] = MODE == 0 ? src[c * cstride + j] : T(1);
+    if (j < n1) {
+      const T ax = MODE == 0 ? src[c * cstride + j] : T(1);
+      if (ep.kind == 0) dst[c * cstride + j] = ax;
+      else apply_epilogue(ep, j, ax);
+    }
   }
 }
 
 template <int P, typename T, int MODE, int NC>
-void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
+void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, CgEpilogue<T> ep = {}) {
   const CGPlan& pl = op.cg;
   const size_t shmem = NC * 2 * ((size_t)pl.max_local + 1) * sizeof(T);
   const bool atomic = op.opt.accumulate == 1 || (op.opt.accumulate == 0 && sizeof(T) == 8);
@@ -268,7 +288,7 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
                        op.d_ghost_start.as<int32_t>(), op.d_ghost_idx.as<int32_t>(),
                        op.d_ccol.as<uint8_t>(), op.d_lidx.as<uint16_t>(), pl.n_cells,
                        op.d_metric.as<T>(), op.d_shape.as<T>(), src, dst, op.d_partial.as<T>(), pl.n_dofs,
-                       std::max<int64_t>(pl.n_ghost_total, 1));
+                       std::max<int64_t>(pl.n_ghost_total, 1), ep);
   const int64_t nif = pl.n_free - pl.n_int;
   const int64_t ncon = pl.n_dofs - pl.n_free;
   const int64_t nthreads = std::max(nif, ncon);
@@ -276,14 +296,14 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
     hipLaunchKernelGGL((cg_fixup_kernel<T, MODE>), dim3((unsigned)((nthreads + 255) / 256)), dim3(256),
                        0, s, pl.n_int, nif, op.d_fix_ptr.as<int32_t>(), op.d_fix_src.as<int32_t>(),
                        op.d_partial.as<T>(), pl.n_free, pl.n_dofs, src, dst, NC, pl.n_dofs,
-                       std::max<int64_t>(pl.n_ghost_total, 1));
+                       std::max<int64_t>(pl.n_ghost_total, 1), ep);
   SMF_HIP(hipGetLastError());
 }
 
 // Nitsche boundary rows: dst[row] += sum_k val[k] src[col[k]] (MODE 0) or
 // dst[row] += B[row][row] (MODE 1).  One thread per boundary row: every row
 // has one writer, so the sums are deterministic.
-template <typename T, int MODE>
+template <typename T, int MODE, bool STORE = false>
 __global__ void cg_boundary_kernel(int64_t nrows, const int32_t* __restrict__ rows, const int32_t* __restrict__ ptr,
                                    const int32_t* __restrict__ col, const T* __restrict__ val,
                                    const T* __restrict__ src, T* __restrict__ dst) {
@@ -295,7 +315,8 @@ __global__ void cg_boundary_kernel(int64_t nrows, const int32_t* __restrict__ ro
     if (MODE == 0) s += val[k] * src[col[k]];
     else if (col[k] == row) s += val[k];
   }
-  dst[row] += s;
+  if (STORE) dst[row] = s;
+  else dst[row] += s;
 }
 
 template <typename T, int MODE>
@@ -339,6 +360,29 @@ void launch_cg(const Operator& op, int mode, void* dst, const void* src, hipStre
   else
     launch_cg_prec<float>(op, mode, static_cast<float*>(dst), static_cast<const float*>(src), s);
 }
+
+template <typename T>
+void launch_cg_epilogue(const Operator& op, const T* x, CgEpilogue<T> ep, hipStream_t s) {
+  if (op.is_dg() || op.ncomp != 1 || ep.kind == 0) throw std::runtime_error("CG epilogue: scalar CG, kind 1 or 2");
+  if ((op.precision == MF_F64) != (sizeof(T) == 8)) throw std::runtime_error("CG epilogue: precision mismatch");
+  if (op.cg.nitsche) {
+    // boundary row sums first, read by the epilogues
+    const int64_t nr = (int64_t)op.cg.bnd_rows.size();
+    if (nr > 0)
+      hipLaunchKernelGGL((cg_boundary_kernel<T, 0, true>), dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
+                         op.d_bnd_rows.as<int32_t>(), op.d_bnd_ptr.as<int32_t>(), op.d_bnd_col.as<int32_t>(),
+                         op.d_bnd_val.as<T>(), x, op.d_bnd_dense.as<T>());
+    ep.bnd = op.d_bnd_dense.as<T>();
+  }
+  switch (op.cg.p) {
+    case 1: launch_cg_t<1, T, 0, 1>(op, nullptr, x, s, ep); break;
+    case 2: launch_cg_t<2, T, 0, 1>(op, nullptr, x, s, ep); break;
+    case 3: launch_cg_t<3, T, 0, 1>(op, nullptr, x, s, ep); break;
+    default: throw std::runtime_error("unsupported CG degree");
+  }
+}
+template void launch_cg_epilogue<double>(const Operator&, const double*, CgEpilogue<double>, hipStream_t);
+template void launch_cg_epilogue<float>(const Operator&, const float*, CgEpilogue<float>, hipStream_t);
 
 void launch_dg(const Operator& op, int mode, void* dst, const void* src, hipStream_t s);
 

@@ -399,20 +399,36 @@ void chebyshev(Multigrid& mg, MGLevel& L, T* x, const T* b, bool zero_start, hip
   T* ax = L.t.as<T>();
   T* d = L.d.as<T>();
   const T* dinv = L.inv_diag.as<T>();
+  // CG levels apply the step inside the operator's kernels (no A x vector)
+  const bool fused = !L.op->is_dg();
+  CgEpilogue<T> ep;
+  ep.kind = 1;
+  ep.b = b;
+  ep.dinv = dinv;
+  ep.d = d;
+  ep.x = x;
   for (int k = 0; k < mg.opt.smoother_degree; ++k) {
+    T c1 = T(0), c2;
     if (k == 0) {
+      c2 = T(1.0 / theta);
       if (zero_start) {
-        blas_fill<T>(n, T(0), ax, s);
-        blas_fill<T>(n, T(0), x, s);
-      } else {
-        level_vmult<T>(L, ax, x, s);
+        blas_cheby_zero<T>(n, c2, dinv, b, d, x, s);
+        continue;
       }
-      blas_cheby_step<T>(n, T(0), T(1.0 / theta), dinv, b, ax, d, x, true, s);
     } else {
       const double rho_new = 1.0 / (2.0 * sigma - rho);
-      level_vmult<T>(L, ax, x, s);
-      blas_cheby_step<T>(n, T(rho_new * rho), T(2.0 * rho_new / delta), dinv, b, ax, d, x, false, s);
+      c1 = T(rho_new * rho);
+      c2 = T(2.0 * rho_new / delta);
       rho = rho_new;
+    }
+    if (fused) {
+      ep.first = k == 0;
+      ep.c1 = c1;
+      ep.c2 = c2;
+      launch_cg_epilogue<T>(*L.op, x, ep, s);
+    } else {
+      level_vmult<T>(L, ax, x, s);
+      blas_cheby_step<T>(n, c1, c2, dinv, b, ax, d, x, k == 0, s);
     }
   }
 }
@@ -434,8 +450,16 @@ void vcycle(Multigrid& mg, int l, hipStream_t s) {
   chebyshev<T>(mg, L, x, b, true, s);
   // r = b - A x ; b_c = R r
   T* r = L.r.as<T>();
-  level_vmult<T>(L, r, x, s);
-  blas_axpby_t<T>(n, T(1), b, T(-1), r, s);
+  if (!L.op->is_dg()) {
+    CgEpilogue<T> ep;
+    ep.kind = 2;
+    ep.b = b;
+    ep.r = r;
+    launch_cg_epilogue<T>(*L.op, x, ep, s);
+  } else {
+    level_vmult<T>(L, r, x, s);
+    blas_axpby_t<T>(n, T(1), b, T(-1), r, s);
+  }
   blas_csr<T>(L.R.nrows, L.R.d_ptr.as<int64_t>(), L.R.d_col.as<int32_t>(), L.R.d_val.as<T>(), r, C.b.as<T>(), false,
               s);
   vcycle<T>(mg, l - 1, s);

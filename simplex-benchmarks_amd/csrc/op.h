@@ -146,12 +146,32 @@ struct Operator {
   // device data
   DevBuf d_cell_start, d_own_start, d_ghost_start, d_ghost_idx, d_ccol, d_fix_ptr,
       d_fix_src, d_lidx, d_metric, d_shape, d_partial;
-  DevBuf d_bnd_rows, d_bnd_ptr, d_bnd_col, d_bnd_val;
+  DevBuf d_bnd_rows, d_bnd_ptr, d_bnd_col, d_bnd_val, d_bnd_dense;
   DevBuf d_dg_geo, d_dg_nbr, d_dg_nface, d_dg_cell_tab, d_dg_face_tab;
   bool on_device = false;
   int64_t m() const { return is_dg() ? dg.n_dofs : cg.n_dofs * ncomp; }
   size_t wbytes() const { return precision == MF_F64 ? 8 : 4; }
 };
+
+// What the CG kernels do with each row of A x (scalar operators, MODE 0):
+// kind 0 store it; kind 1 one Chebyshev step (d = c1 d + c2 Dinv (b - Ax),
+// or d = c2 Dinv (b - Ax) when first; x += d, x being the vmult's input,
+// updated in place); kind 2 the residual r = b - A x.  bnd: Nitsche boundary
+// row sums B x (dense), added to A x.
+template <typename T>
+struct CgEpilogue {
+  int kind = 0, first = 0;
+  T c1 = T(0), c2 = T(0);
+  const T* b = nullptr;
+  const T* dinv = nullptr;
+  T* d = nullptr;
+  T* x = nullptr;
+  T* r = nullptr;
+  const T* bnd = nullptr;
+};
+// A x with the epilogue (kind 1 or 2) applied row by row inside the kernels
+template <typename T>
+void launch_cg_epilogue(const Operator& op, const T* x, CgEpilogue<T> ep, hipStream_t s);
 
 void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan);
 // boundary-face matrix of the Nitsche CG operator (plan.nitsche)
