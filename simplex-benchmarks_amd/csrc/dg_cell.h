@@ -483,6 +483,22 @@ SMF_HD void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __
     u[i] = MODE == 0 ? src[base + pf[i]] : T(0);
     y[i] = T(0);
   }
+  // neighbour data issued before the cell term so that its latency overlaps
+  // it (a boundary face reads its own cell; those values are not used)
+  const int32_t kb = nbr[f * ncells + k];
+  const bool interior = kb >= 0;
+  const int64_t kn = interior ? kb : k;
+  const int fb = interior ? (int)((nface[k] >> (8 * f)) & 3) : 3;
+  T ub[NL], jb[9];
+  if (MODE == 0) {
+    const int64_t nbase = (kn * NC + c) * NL;
+#pragma unroll
+    for (int i = 0; i < NL; ++i)
+      ub[i] = src[nbase + select4<int>(fb, FT.perm[0][i], FT.perm[1][i], FT.perm[2][i], FT.perm[3][i])];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) jb[t] = geo[t * ncells + kn];
+  }
+  const T adetb = geo[9 * ncells + kn], taub = geo[10 * ncells + kn];
   // ---- the lane's share of the cell integral ----
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
@@ -518,14 +534,11 @@ SMF_HD void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __
     }
   }
   // ---- face f (= face 3 of the lane frame) ----
-  const int32_t kb = nbr[f * ncells + k];
-  const bool interior = kb >= 0;
   T sigma;
   if (pkind == 1) {  // (p+1)(p+2) |F| / min |K|: factor * max(len, len |det J| / |det J+|)
-    const T lb = len * adet / geo[9 * ncells + (interior ? kb : k)];
+    const T lb = len * adet / adetb;
     sigma = factor * (len > lb ? len : lb);
   } else {
-    const T taub = geo[10 * ncells + (interior ? kb : k)];
     sigma = factor * (interior ? (tau > taub ? tau : taub) : tau);
   }
   if (MODE == 0) {
@@ -541,18 +554,10 @@ SMF_HD void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __
       }
       gn[j] = sx * nJ[0] + sy * nJ[1] + sz * nJ[2];
     }
-    T ub[NL], gb[NFG];
+    T gb[NFG];
     if (interior) {
-      const int fb = (nface[k] >> (8 * f)) & 3;
-      const int64_t nbase = ((int64_t)kb * NC + c) * NL;
-#pragma unroll
-      for (int i = 0; i < NL; ++i)
-        ub[i] = src[nbase + select4<int>(fb, FT.perm[0][i], FT.perm[1][i], FT.perm[2][i], FT.perm[3][i])];
       T bb[4];
       {
-        T jb[9];
-#pragma unroll
-        for (int t = 0; t < 9; ++t) jb[t] = geo[t * ncells + kb];
         bb[1] = jb[0] * n0 + jb[3] * n1 + jb[6] * n2;
         bb[2] = jb[1] * n0 + jb[4] * n1 + jb[7] * n2;
         bb[3] = jb[2] * n0 + jb[5] * n1 + jb[8] * n2;
