@@ -114,14 +114,17 @@ __global__ void cheby_zero_kernel(int64_t n, T c2, const T* __restrict__ dinv, c
   x[i] = di;
 }
 
-// y = M x for a dense row-major n x n matrix (FP64)
-__global__ void dense_gemv_kernel(int n, const double* __restrict__ M, const double* __restrict__ x,
-                                  double* __restrict__ y) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
+// y = M x for a dense row-major n x n matrix (FP64): one wave per row, lanes
+// stride the columns (coalesced), fixed-order butterfly sum (deterministic)
+__global__ __launch_bounds__(256) void dense_gemv_kernel(int n, const double* __restrict__ M,
+                                                         const double* __restrict__ x, double* __restrict__ y) {
+  const int r = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (r >= n) return;  // whole waves exit together
+  const double* row = M + (int64_t)r * n;
   double s = 0.0;
-  for (int c = 0; c < n; ++c) s += M[(int64_t)r * n + c] * x[c];
-  y[r] = s;
+  for (int c = lane; c < n; c += 64) s += row[c] * x[c];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) y[r] = s;
 }
 
 }  // namespace
@@ -170,7 +173,7 @@ void blas_cheby_zero(int64_t n, T c2, const T* dinv, const T* b, T* d, T* x, hip
   if (n > 0) hipLaunchKernelGGL(cheby_zero_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, c2, dinv, b, d, x);
 }
 void blas_dense_gemv(int n, const double* M, const double* x, double* y, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(dense_gemv_kernel, dim3((n + 63) / 64), dim3(64), 0, s, n, M, x, y);
+  if (n > 0) hipLaunchKernelGGL(dense_gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, s, n, M, x, y);
 }
 int blas_reduction_parts() { return kRedBlocks; }
 

@@ -270,34 +270,45 @@ std::vector<double> dense_inverse(const Operator& op) {
   if (pl.nitsche)
     for (size_t r = 0; r < pl.bnd_rows.size(); ++r)
       for (int32_t k = pl.bnd_ptr[r]; k < pl.bnd_ptr[r + 1]; ++k) A[pl.bnd_rows[r] * n + pl.bnd_col[k]] += pl.bnd_val[k];
-  // Cholesky A = L L^T (in place, lower)
+  // Cholesky A = L L^T (in place, lower; rows of a column in parallel)
   for (int64_t j = 0; j < n; ++j) {
     double s = A[j * n + j];
     for (int64_t k = 0; k < j; ++k) s -= A[j * n + k] * A[j * n + k];
     if (!(s > 0)) throw std::runtime_error("coarse matrix not positive definite");
     const double l = std::sqrt(s);
     A[j * n + j] = l;
+#pragma omp parallel for schedule(static) if (n - j > 256)
     for (int64_t i = j + 1; i < n; ++i) {
       double t = A[i * n + j];
       for (int64_t k = 0; k < j; ++k) t -= A[i * n + k] * A[j * n + k];
       A[i * n + j] = t / l;
     }
   }
-  // inverse: solve L L^T X = I column by column
-  std::vector<double> inv(n * n);
-  std::vector<double> y(n);
-  for (int64_t c = 0; c < n; ++c) {
-    for (int64_t i = 0; i < n; ++i) {
-      double t = (i == c) ? 1.0 : 0.0;
-      for (int64_t k = 0; k < i; ++k) t -= A[i * n + k] * y[k];
-      y[i] = t / A[i * n + i];
-    }
-    for (int64_t i = n - 1; i >= 0; --i) {
-      double t = y[i];
-      for (int64_t k = i + 1; k < n; ++k) t -= A[k * n + i] * inv[k * n + c];
-      inv[i * n + c] = t / A[i * n + i];
+  // inverse: solve L L^T X = I, columns in parallel (results independent of the thread count)
+  std::vector<double> invt(n * n);  // invt[c * n + i] = inv[i][c]
+#pragma omp parallel
+  {
+    std::vector<double> y(n);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t c = 0; c < n; ++c) {
+      for (int64_t i = 0; i < n; ++i) {
+        double t = (i == c) ? 1.0 : 0.0;
+        for (int64_t k = 0; k < i; ++k) t -= A[i * n + k] * y[k];
+        y[i] = t / A[i * n + i];
+      }
+      double* xc = &invt[c * n];
+      for (int64_t i = n - 1; i >= 0; --i) {
+        double t = y[i];
+        for (int64_t k = i + 1; k < n; ++k) t -= A[k * n + i] * xc[k];
+        xc[i] = t / A[i * n + i];
+      }
     }
   }
+  // the inverse is symmetric: invt is its row-major form up to rounding;
+  // return it transposed back so that entry (i, c) is the solve's result
+  std::vector<double> inv(n * n);
+  for (int64_t c = 0; c < n; ++c)
+    for (int64_t i = 0; i < n; ++i) inv[i * n + c] = invt[c * n + i];
   return inv;
 }
 
@@ -503,9 +514,11 @@ std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions
     ops.push_back(make_cg_operator(fine.mesh, p, prec, lopt));
     kn.push_back(fine.mesh.kuhn_n);
   }
+  // h-coarsening stops at the first level the dense FP64 coarse solver
+  // takes (each level below costs a V-cycle a dozen latency-bound launches)
   int n = fine.mesh.kuhn_n;
   if (n > 0) {
-    while (n % 2 == 0 && n / 2 >= 2 && ops.back()->cg.n_free > 64) {
+    while (n % 2 == 0 && n / 2 >= 2 && ops.back()->m() > opt.max_coarse_dofs) {
       n /= 2;
       ops.push_back(make_cg_operator(coarse_kuhn(fine.mesh, n), 1, prec, lopt));
       kn.push_back(n);

@@ -49,7 +49,10 @@ def test_dg_hierarchy(dgmg):
     assert inf["degree"][-2:] == [2, 2] and inf["degree"][-3] == 1
     assert inf["n_dofs"][-1] == A.m() == 6 * 8 ** 3 * 10
     assert inf["n_dofs"][-2] == 17 ** 3  # CG P2 on 8^3 with every boundary node free
-    assert inf["kuhn_n"][0] < 8
+    # h-coarsening stops at the first level the dense coarse solver takes
+    assert inf["kuhn_n"][0] == 8 and inf["n_dofs"][0] == 9 ** 3
+    deep = smf.Multigrid(A, max_coarse_dofs=64).info()
+    assert deep["kuhn_n"][0] == 2 and deep["n_dofs"][-3:] == inf["n_dofs"][-3:]
 
 
 def test_c_transfer_is_injection(dgmg):

@@ -90,7 +90,7 @@ def test_cfg3_mg_cg_48cube(gpu, orc):
     op = smf.Operator(smf.Mesh.kuhn_cube(48, 0.2, 7), 2, kind="cg", precision="f64")
     assert op.m() == 912673 and op.info["n_constrained"] == 55298
     mg = smf.Multigrid(op)
-    assert mg.info()["n_dofs"] == [64, 343, 2197, 15625, 117649, 912673]
+    assert mg.info()["n_dofs"] == [2197, 15625, 117649, 912673]
     bh = smf.assemble_rhs(op, "grf", seed=7, n_features=256, corr_length=0.1)
     x = op.initialize_dof_vector()
     st = smf.solve_cg(op, x, smf.DeviceVector.from_numpy(bh), mg, rel_tol=1e-10)

@@ -23,8 +23,9 @@ def test_transfers_reproduce_lattice_interpolation(gpu):
     smf = gpu
     print("\ncase: MG transfers P2, n=8, unperturbed, f64 levels", flush=True)
     fine = smf.Operator(smf.Mesh.kuhn_cube(8, 0.0, 0), 2)
-    mg = smf.Multigrid(fine, level_precision="f64")
+    mg = smf.Multigrid(fine, level_precision="f64", max_coarse_dofs=64)  # keep the h-levels
     info = mg.info()
+    assert info["kuhn_n"][0] == 2
     assert info["degree"][-1] == 2 and info["degree"][-2] == 1
     rng = np.random.default_rng(0)
     for l in range(1, info["n_levels"]):
