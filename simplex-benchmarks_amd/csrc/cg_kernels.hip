@@ -28,6 +28,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "epilogue.h"
 #include "op.h"
 #include "shape_tables.h"
 
@@ -44,19 +45,6 @@ struct CGShape {
   static constexpr int NQ = P == 1 ? 1 : (P == 2 ? 4 : 11);
 };
 
-template <typename T>
-__device__ inline void apply_epilogue(const CgEpilogue<T>& ep, int64_t i, T ax) {
-  if (ep.bnd) ax += ep.bnd[i];
-  if (ep.kind == 1) {
-    const T r = ep.b[i] - ax;
-    const T di = ep.first ? ep.c2 * ep.dinv[i] * r : ep.c1 * ep.d[i] + ep.c2 * ep.dinv[i] * r;
-    ep.d[i] = di;
-    ep.x[i] += di;
-  } else {
-    ep.r[i] = ep.b[i] - ax;
-  }
-}
-
 // MODE 0: y = A_K u;  MODE 1: y_i = (A_K)_ii
 // shape: G[(q*NL + i)*3 + d] = d phi_i / d xhat_d at point q, then w[q].
 // ATOMIC: accumulate with ds_add (fast for f64 on gfx950) or with a plain
@@ -70,7 +58,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int32_t* __restrict__ ghost_start, const int32_t* __restrict__ ghost_idx,
     const uint8_t* __restrict__ ccol, const uint16_t* __restrict__ lidx, int64_t ncells_total,
     const T* __restrict__ metric, const T* __restrict__ shape, const T* src,
-    T* __restrict__ dst, T* __restrict__ partial, int64_t cstride, int64_t pstride, CgEpilogue<T> ep) {
+    T* __restrict__ dst, T* __restrict__ partial, int64_t cstride, int64_t pstride, RowEpilogue<T> ep) {
   constexpr int NL = CGShape<P>::NL, NQ = CGShape<P>::NQ;
   extern __shared__ unsigned char smem_raw[];
   T* sbase = reinterpret_cast<T*>(smem_raw);
@@ -251,7 +239,7 @@ template <typename T, int MODE>
 __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* __restrict__ fix_ptr,
                                 const int32_t* __restrict__ fix_src, const T* __restrict__ partial,
                                 int64_t n0, int64_t n1, const T* src,
-                                T* __restrict__ dst, int nc, int64_t cstride, int64_t pstride, CgEpilogue<T> ep) {
+                                T* __restrict__ dst, int nc, int64_t cstride, int64_t pstride, RowEpilogue<T> ep) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int c = 0; c < nc; ++c) {
     const T* pc = partial + c * pstride;
@@ -272,7 +260,7 @@ __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* _
 }
 
 template <int P, typename T, int MODE, int NC>
-void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, CgEpilogue<T> ep = {}) {
+void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpilogue<T> ep = {}) {
   const CGPlan& pl = op.cg;
   const size_t shmem = NC * 2 * ((size_t)pl.max_local + 1) * sizeof(T);
   const bool atomic = op.opt.accumulate == 1 || (op.opt.accumulate == 0 && sizeof(T) == 8);
@@ -301,32 +289,57 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, CgEpil
 }
 
 // Nitsche boundary rows: dst[row] += sum_k val[k] src[col[k]] (MODE 0) or
-// dst[row] += B[row][row] (MODE 1).  One thread per boundary row: every row
-// has one writer, so the sums are deterministic.
-template <typename T, int MODE, bool STORE = false>
-__global__ void cg_boundary_kernel(int64_t nrows, const int32_t* __restrict__ rows, const int32_t* __restrict__ ptr,
-                                   const int32_t* __restrict__ col, const T* __restrict__ val,
-                                   const T* __restrict__ src, T* __restrict__ dst) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  const int32_t row = rows[r];
+// dst[row] += B[row][row] (MODE 1).  G consecutive lanes share a row: lane s
+// sums entries s, s + G, ... (coalesced column/value reads), then a
+// fixed-order butterfly combines the G partial sums.  Every row has one
+// writer, so the result is deterministic.
+template <typename T, int MODE, bool STORE, int G>
+__global__ __launch_bounds__(256) void cg_boundary_kernel(int64_t nrows, const int32_t* __restrict__ rows,
+                                                          const int32_t* __restrict__ ptr,
+                                                          const int32_t* __restrict__ col,
+                                                          const T* __restrict__ val, const T* __restrict__ src,
+                                                          T* __restrict__ dst) {
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int sub = threadIdx.x % G;
+  const bool ok = r < nrows;  // no early exit: every lane takes part in the butterfly
   T s = T(0);
-  for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
-    if (MODE == 0) s += val[k] * src[col[k]];
-    else if (col[k] == row) s += val[k];
+  if (ok) {
+    const int32_t row = rows[r], k1 = ptr[r + 1];
+    for (int32_t k = ptr[r] + sub; k < k1; k += G) {
+      if (MODE == 0) s += val[k] * src[col[k]];
+      else if (col[k] == row) s += val[k];
+    }
   }
-  if (STORE) dst[row] = s;
-  else dst[row] += s;
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
+  if (ok && sub == 0) {
+    const int32_t row = rows[r];
+    if (STORE) dst[row] = s;
+    else dst[row] += s;
+  }
+}
+
+// lanes per row from the mean row length (about 4 entries per lane)
+template <typename T, int MODE, bool STORE>
+void launch_boundary_rows(const Operator& op, T* dst, const T* src, hipStream_t s) {
+  const int64_t nr = (int64_t)op.cg.bnd_rows.size();
+  if (nr == 0) return;
+  const int64_t mean = (int64_t)op.cg.bnd_col.size() / nr;
+  const int g = mean >= 48 ? 16 : (mean >= 20 ? 8 : 4);
+  const unsigned blocks = (unsigned)((nr * g + 255) / 256);
+  auto args = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, nr, op.d_bnd_rows.as<int32_t>(),
+                       op.d_bnd_ptr.as<int32_t>(), op.d_bnd_col.as<int32_t>(), op.d_bnd_val.as<T>(), src, dst);
+  };
+  if (g == 16) args(cg_boundary_kernel<T, MODE, STORE, 16>);
+  else if (g == 8) args(cg_boundary_kernel<T, MODE, STORE, 8>);
+  else args(cg_boundary_kernel<T, MODE, STORE, 4>);
+  SMF_HIP(hipGetLastError());
 }
 
 template <typename T, int MODE>
 void launch_boundary_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
-  const int64_t nr = (int64_t)op.cg.bnd_rows.size();
-  if (nr > 0)
-    hipLaunchKernelGGL((cg_boundary_kernel<T, MODE>), dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
-                       op.d_bnd_rows.as<int32_t>(), op.d_bnd_ptr.as<int32_t>(), op.d_bnd_col.as<int32_t>(),
-                       op.d_bnd_val.as<T>(), src, dst);
-  SMF_HIP(hipGetLastError());
+  launch_boundary_rows<T, MODE, false>(op, dst, src, s);
 }
 
 template <typename T, int NC>
@@ -362,16 +375,12 @@ void launch_cg(const Operator& op, int mode, void* dst, const void* src, hipStre
 }
 
 template <typename T>
-void launch_cg_epilogue(const Operator& op, const T* x, CgEpilogue<T> ep, hipStream_t s) {
+void launch_cg_epilogue(const Operator& op, const T* x, RowEpilogue<T> ep, hipStream_t s) {
   if (op.is_dg() || op.ncomp != 1 || ep.kind == 0) throw std::runtime_error("CG epilogue: scalar CG, kind 1 or 2");
   if ((op.precision == MF_F64) != (sizeof(T) == 8)) throw std::runtime_error("CG epilogue: precision mismatch");
   if (op.cg.nitsche) {
     // boundary row sums first, read by the epilogues
-    const int64_t nr = (int64_t)op.cg.bnd_rows.size();
-    if (nr > 0)
-      hipLaunchKernelGGL((cg_boundary_kernel<T, 0, true>), dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, nr,
-                         op.d_bnd_rows.as<int32_t>(), op.d_bnd_ptr.as<int32_t>(), op.d_bnd_col.as<int32_t>(),
-                         op.d_bnd_val.as<T>(), x, op.d_bnd_dense.as<T>());
+    launch_boundary_rows<T, 0, true>(op, op.d_bnd_dense.as<T>(), x, s);
     ep.bnd = op.d_bnd_dense.as<T>();
   }
   switch (op.cg.p) {
@@ -381,8 +390,8 @@ void launch_cg_epilogue(const Operator& op, const T* x, CgEpilogue<T> ep, hipStr
     default: throw std::runtime_error("unsupported CG degree");
   }
 }
-template void launch_cg_epilogue<double>(const Operator&, const double*, CgEpilogue<double>, hipStream_t);
-template void launch_cg_epilogue<float>(const Operator&, const float*, CgEpilogue<float>, hipStream_t);
+template void launch_cg_epilogue<double>(const Operator&, const double*, RowEpilogue<double>, hipStream_t);
+template void launch_cg_epilogue<float>(const Operator&, const float*, RowEpilogue<float>, hipStream_t);
 
 void launch_dg(const Operator& op, int mode, void* dst, const void* src, hipStream_t s);
 

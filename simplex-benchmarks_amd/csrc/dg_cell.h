@@ -24,8 +24,12 @@
 
 #ifdef __HIPCC__
 #define SMF_HD __host__ __device__
+// the cell bodies are inlined into every kernel that calls them (a shared
+// out-of-line copy costs the kernels occupancy)
+#define SMF_HDI __host__ __device__ __attribute__((always_inline))
 #else
 #define SMF_HD
+#define SMF_HDI
 #endif
 
 namespace smf {
@@ -191,7 +195,7 @@ SMF_HD inline T select4(int f, T a0, T a1, T a2, T a3) {
 // One cell k: MODE 0 dst rows of k = (A src) rows; MODE 1 = diagonal of A.
 // geo: SoA [11][ncells] = J^-T (row-major), |det J|, tau
 template <int P, typename T, int NC, bool MASS, int MODE>
-SMF_HD void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, const int32_t* __restrict__ nbr,
+SMF_HDI void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, const int32_t* __restrict__ nbr,
                        const uint32_t* __restrict__ nface, T factor, T mass, T lap, const T* __restrict__ src,
                        T* __restrict__ dst, int pkind = 0) {
   constexpr DGCellTab<P, MASS> CT = make_cell_tab<P, MASS>();
@@ -429,7 +433,7 @@ constexpr DGCellShare<P, MASS> make_cell_share() {
 // lane's contribution to row i (MODE 0) or to the diagonal (MODE 1); the four
 // lanes' rows add up to apply_cell's result.
 template <int P, typename T, int NC, bool MASS, int MODE>
-SMF_HD void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __restrict__ geo,
+SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __restrict__ geo,
                             const int32_t* __restrict__ nbr, const uint32_t* __restrict__ nface, T factor, T mass,
                             T lap, const T* __restrict__ src, T* __restrict__ yrow, int pkind = 0) {
   constexpr DGCellTab<P, MASS> CT = make_cell_tab<P, MASS>();

@@ -21,6 +21,7 @@
 #include <stdexcept>
 
 #include "dg_cell.h"
+#include "epilogue.h"
 #include "op.h"
 
 namespace smf {
@@ -247,6 +248,29 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells,
   }
 }
 
+// The four-lane kernel for scalar SIPG with a RowEpilogue on every row
+// instead of the store (the Chebyshev step or the residual of a multigrid
+// level).
+template <int P, typename T>
+__global__ __launch_bounds__(4 * kQuadCells) void dg_quad_ep_kernel(int64_t ncells, const T* __restrict__ geo,
+                                                                     const int32_t* __restrict__ nbr,
+                                                                     const uint32_t* __restrict__ nface, T factor,
+                                                                     const T* __restrict__ src, int pkind,
+                                                                     RowEpilogue<T> ep) {
+  constexpr int NL = dgt::nloc(P);
+  __shared__ T ybuf[kQuadCells][4][NL + 1];
+  const int cl = threadIdx.x >> 2, f = threadIdx.x & 3;
+  const int64_t k = xcd_block(blockIdx.x, gridDim.x) * kQuadCells + cl;
+  const bool active = k < ncells;
+  const int64_t kc = active ? k : ncells - 1;
+  dgt::apply_cell_lane<P, T, 1, false, 0>(kc, f, 0, ncells, geo, nbr, nface, factor, T(0), T(1), src, ybuf[cl][f],
+                                          pkind);
+  __syncthreads();
+  if (active)
+    for (int i = f; i < NL; i += 4)
+      apply_epilogue(ep, k * NL + i, ((ybuf[cl][0][i] + ybuf[cl][1][i]) + ybuf[cl][2][i]) + ybuf[cl][3][i]);
+}
+
 template <int P, typename T, int NC, bool MASS, int MODE>
 void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
   const DGPlan& pl = op.dg;
@@ -306,7 +330,38 @@ void launch_dg_prec(const Operator& op, int mode, T* dst, const T* src, hipStrea
   }
 }
 
+template <int P, typename T>
+void launch_dg_ep_t(const Operator& op, const T* x, RowEpilogue<T> ep, hipStream_t s) {
+  const DGPlan& pl = op.dg;
+  if (pl.n_cells > 0)
+    hipLaunchKernelGGL((dg_quad_ep_kernel<P, T>), dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells)),
+                       dim3(4 * kQuadCells), 0, s, pl.n_cells, op.d_dg_geo.as<T>(), op.d_dg_nbr.as<int32_t>(),
+                       op.d_dg_nface.as<uint32_t>(), (T)pl.penalty_factor, x, pl.penalty_kind, ep);
+  SMF_HIP(hipGetLastError());
+}
+
 }  // namespace
+
+bool dg_has_epilogue(const Operator& op) {
+  return op.kind == MF_DG_SIPG && (op.opt.dg_kernel == 3 || (op.opt.dg_kernel == 0 && op.dg.p >= 2));
+}
+
+template <typename T>
+void launch_dg_epilogue(const Operator& op, const T* x, RowEpilogue<T> ep, hipStream_t s) {
+  if (!dg_has_epilogue(op)) throw std::runtime_error("DG epilogue: scalar SIPG with the four-lane kernel");
+  if ((op.precision == MF_F64) != (sizeof(T) == 8)) throw std::runtime_error("DG epilogue: precision mismatch");
+  if (ep.kind == 1 && (ep.xout == nullptr || ep.xout == ep.x))
+    throw std::runtime_error("DG epilogue: the Chebyshev step writes a separate output vector");
+  if (ep.kind != 1 && ep.kind != 2) throw std::runtime_error("DG epilogue: kind 1 or 2");
+  switch (op.dg.p) {
+    case 1: launch_dg_ep_t<1, T>(op, x, ep, s); break;
+    case 2: launch_dg_ep_t<2, T>(op, x, ep, s); break;
+    case 3: launch_dg_ep_t<3, T>(op, x, ep, s); break;
+    default: throw std::runtime_error("unsupported DG degree");
+  }
+}
+template void launch_dg_epilogue<double>(const Operator&, const double*, RowEpilogue<double>, hipStream_t);
+template void launch_dg_epilogue<float>(const Operator&, const float*, RowEpilogue<float>, hipStream_t);
 
 void launch_dg(const Operator& op, int mode, void* dst, const void* src, hipStream_t s) {
   if (op.precision == MF_F64)

@@ -399,26 +399,29 @@ double estimate_lambda_max(Multigrid& mg, MGLevel& L, int iters, hipStream_t s) 
 }
 
 // Chebyshev smoothing of A x = b on level L (Saad, Iterative Methods, Alg.
-// 12.1, on the Jacobi-preconditioned system), `degree` steps; zero_start
-// skips the first residual product.
+// 12.1, on the Jacobi-preconditioned system), `degree` steps, x = L.x;
+// zero_start skips the first residual product.  CG levels apply each step
+// inside the operator's kernels, in place; DG levels with the four-lane
+// kernel write x + d to L.t and swap L.t with L.x (the neighbour gathers of
+// other cells still read the old x), so L.x holds the iterate after every
+// step.
 template <typename T>
-void chebyshev(Multigrid& mg, MGLevel& L, T* x, const T* b, bool zero_start, hipStream_t s) {
+void chebyshev(Multigrid& mg, MGLevel& L, const T* b, bool zero_start, hipStream_t s) {
   const int64_t n = L.op->m();
   const double theta = 0.5 * (L.lambda_max + L.lambda_min), delta = 0.5 * (L.lambda_max - L.lambda_min);
   const double sigma = theta / delta;
   double rho = 1.0 / sigma;
-  T* ax = L.t.as<T>();
   T* d = L.d.as<T>();
   const T* dinv = L.inv_diag.as<T>();
-  // CG levels apply the step inside the operator's kernels (no A x vector)
-  const bool fused = !L.op->is_dg();
-  CgEpilogue<T> ep;
+  const bool dg = L.op->is_dg();
+  const bool fused = !dg || dg_has_epilogue(*L.op);
+  RowEpilogue<T> ep;
   ep.kind = 1;
   ep.b = b;
   ep.dinv = dinv;
   ep.d = d;
-  ep.x = x;
   for (int k = 0; k < mg.opt.smoother_degree; ++k) {
+    T* x = L.x.as<T>();
     T c1 = T(0), c2;
     if (k == 0) {
       c2 = T(1.0 / theta);
@@ -436,8 +439,16 @@ void chebyshev(Multigrid& mg, MGLevel& L, T* x, const T* b, bool zero_start, hip
       ep.first = k == 0;
       ep.c1 = c1;
       ep.c2 = c2;
-      launch_cg_epilogue<T>(*L.op, x, ep, s);
+      ep.x = x;
+      if (dg) {
+        ep.xout = L.t.as<T>();
+        launch_dg_epilogue<T>(*L.op, x, ep, s);
+        std::swap(L.x.p, L.t.p);
+      } else {
+        launch_cg_epilogue<T>(*L.op, x, ep, s);
+      }
     } else {
+      T* ax = L.t.as<T>();
       level_vmult<T>(L, ax, x, s);
       blas_cheby_step<T>(n, c1, c2, dinv, b, ax, d, x, k == 0, s);
     }
@@ -458,15 +469,17 @@ void vcycle(Multigrid& mg, int l, hipStream_t s) {
     return;
   }
   MGLevel& C = mg.levels[l - 1];
-  chebyshev<T>(mg, L, x, b, true, s);
+  chebyshev<T>(mg, L, b, true, s);
+  x = L.x.as<T>();
   // r = b - A x ; b_c = R r
   T* r = L.r.as<T>();
-  if (!L.op->is_dg()) {
-    CgEpilogue<T> ep;
+  if (!L.op->is_dg() || dg_has_epilogue(*L.op)) {
+    RowEpilogue<T> ep;
     ep.kind = 2;
     ep.b = b;
     ep.r = r;
-    launch_cg_epilogue<T>(*L.op, x, ep, s);
+    if (L.op->is_dg()) launch_dg_epilogue<T>(*L.op, x, ep, s);
+    else launch_cg_epilogue<T>(*L.op, x, ep, s);
   } else {
     level_vmult<T>(L, r, x, s);
     blas_axpby_t<T>(n, T(1), b, T(-1), r, s);
@@ -477,7 +490,7 @@ void vcycle(Multigrid& mg, int l, hipStream_t s) {
   // x += P x_c
   blas_csr<T>(L.P.nrows, L.P.d_ptr.as<int64_t>(), L.P.d_col.as<int32_t>(), L.P.d_val.as<T>(), C.x.as<T>(), x, true,
               s);
-  chebyshev<T>(mg, L, x, b, false, s);
+  chebyshev<T>(mg, L, b, false, s);
 }
 
 }  // namespace

@@ -153,25 +153,34 @@ struct Operator {
   size_t wbytes() const { return precision == MF_F64 ? 8 : 4; }
 };
 
-// What the CG kernels do with each row of A x (scalar operators, MODE 0):
-// kind 0 store it; kind 1 one Chebyshev step (d = c1 d + c2 Dinv (b - Ax),
-// or d = c2 Dinv (b - Ax) when first; x += d, x being the vmult's input,
-// updated in place); kind 2 the residual r = b - A x.  bnd: Nitsche boundary
-// row sums B x (dense), added to A x.
+// What the operator kernels do with each row of A x (scalar operators,
+// MODE 0): kind 0 store it; kind 1 one Chebyshev step (d = c1 d + c2 Dinv
+// (b - Ax), or d = c2 Dinv (b - Ax) when first; then x += d in place, x being
+// the vmult's input, or xout = x + d when xout is set); kind 2 the residual
+// r = b - A x.  bnd: Nitsche boundary row sums B x (dense), added to A x.
+// In place is safe for the CG kernels (a row's epilogue runs after every read
+// of that DoF); the DG kernels read neighbour cells' values while other cells
+// finish, so they write xout.
 template <typename T>
-struct CgEpilogue {
+struct RowEpilogue {
   int kind = 0, first = 0;
   T c1 = T(0), c2 = T(0);
   const T* b = nullptr;
   const T* dinv = nullptr;
   T* d = nullptr;
   T* x = nullptr;
+  T* xout = nullptr;
   T* r = nullptr;
   const T* bnd = nullptr;
 };
 // A x with the epilogue (kind 1 or 2) applied row by row inside the kernels
 template <typename T>
-void launch_cg_epilogue(const Operator& op, const T* x, CgEpilogue<T> ep, hipStream_t s);
+void launch_cg_epilogue(const Operator& op, const T* x, RowEpilogue<T> ep, hipStream_t s);
+// the same for a scalar DG SIPG operator (four-lane kernel; ep.xout required
+// for kind 1); dg_has_epilogue says whether the operator's kernel has it
+bool dg_has_epilogue(const Operator& op);
+template <typename T>
+void launch_dg_epilogue(const Operator& op, const T* x, RowEpilogue<T> ep, hipStream_t s);
 
 void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan);
 // boundary-face matrix of the Nitsche CG operator (plan.nitsche)
