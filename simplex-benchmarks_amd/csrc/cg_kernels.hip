@@ -74,14 +74,37 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
   // it reads as zero and absorbs their contributions, so no branches
   const int lstride = 2 * (nloc + 1);
 
-  // 1. stage the patch DoFs
+  // 1. stage the patch DoFs.  Owned DoFs are the contiguous global range
+  //    [o0, o0 + no) (coalesced); ghost DoFs come through ghost_idx.  Loads
+  //    go in batches of SB per thread, all in flight before the first LDS
+  //    write (clamped indices keep the spare loads in bounds).
+  constexpr int SB = 8;
   for (int cc = 0; cc < NC; ++cc) {
     T* su = sbase + cc * lstride;
     T* sy = su + nloc + 1;
     const T* sc = src + cc * cstride;
-    for (int j = lane; j <= nloc; j += nt) {
-      if (MODE == 0) su[j] = j < no ? sc[o0 + j] : (j < nloc ? sc[ghost_idx[g0 + j - no]] : T(0));
-      sy[j] = T(0);
+    for (int j = lane; j <= nloc; j += nt) sy[j] = T(0);
+    if (MODE == 0) {
+      for (int j0 = 0; j0 < no; j0 += SB * nt) {
+        T v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) v[u] = sc[o0 + min(j0 + u * nt + lane, no - 1)];
+#pragma unroll
+        for (int u = 0; u < SB; ++u)
+          if (j0 + u * nt + lane < no) su[j0 + u * nt + lane] = v[u];
+      }
+      for (int j0 = 0; j0 < ng; j0 += SB * nt) {
+        int32_t gi[SB];
+        T v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) gi[u] = ghost_idx[g0 + min(j0 + u * nt + lane, ng - 1)];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) v[u] = sc[gi[u]];
+#pragma unroll
+        for (int u = 0; u < SB; ++u)
+          if (j0 + u * nt + lane < ng) su[no + j0 + u * nt + lane] = v[u];
+      }
+      if (lane == 0) su[nloc] = T(0);
     }
   }
   __syncthreads();
@@ -216,19 +239,18 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
   }
   __syncthreads();
 
-  // 4. owned DoFs: store; ghost DoFs: partial sums for the fix-up
+  // 4. owned DoFs: store (or the row epilogue: owned DoFs are read by this
+  //    patch only); ghost DoFs: partial sums for the fix-up
   for (int cc = 0; cc < NC; ++cc) {
     const T* sy = sbase + cc * lstride + nloc + 1;
     T* dc = dst + cc * cstride;
     T* pc = partial + cc * pstride;
-    for (int j = lane; j < nloc; j += nt) {
-      if (j >= no)
-        pc[g0 + j - no] = sy[j];
-      else if (ep.kind == 0)
-        dc[o0 + j] = sy[j];
-      else
-        apply_epilogue(ep, o0 + j, sy[j]);  // owned DoFs are read by this patch only
+    if (ep.kind == 0) {
+      for (int j = lane; j < no; j += nt) dc[o0 + j] = sy[j];
+    } else {
+      epilogue_rows<T, 4>(ep, o0, no, lane, nt, sy);
     }
+    for (int j = lane; j < ng; j += nt) pc[g0 + j] = sy[no + j];
   }
 }
 
@@ -244,11 +266,25 @@ __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* _
   for (int c = 0; c < nc; ++c) {
     const T* pc = partial + c * pstride;
     if (t < n_iface) {
+      const int64_t row = n_int + t;
+      EpilogueIn<T> in{};
+      if (ep.kind != 0) in = epilogue_load(ep, row);  // in flight with the partials
       const int32_t k0 = fix_ptr[t], k1 = fix_ptr[t + 1];
-      T s = pc[fix_src[k0]];
-      for (int32_t k = k0 + 1; k < k1; ++k) s += pc[fix_src[k]];
-      if (ep.kind == 0) dst[c * cstride + n_int + t] = s;
-      else apply_epilogue(ep, n_int + t, s);
+      // partials in patch order, eight indices then eight values per step
+      T s = T(0);
+      for (int32_t k = k0; k < k1; k += 8) {
+        int32_t src_k[8];
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) src_k[u] = fix_src[min(k + u, k1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = pc[src_k[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (k + u < k1) s = k + u == k0 ? v[u] : s + v[u];
+      }
+      if (ep.kind == 0) dst[c * cstride + row] = s;
+      else epilogue_finish(ep, row, s, in);
     }
     const int64_t j = n0 + t;
     if (j < n1) {

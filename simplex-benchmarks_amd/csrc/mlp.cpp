@@ -214,10 +214,15 @@ void ls_deserialize(LearnedSmoother& ls, const uint8_t* data, size_t size) {
 
 void ls_apply(LearnedSmoother& ls, double* corr, const double* r, hipStream_t s) {
   // features and residual norms are kept (x, scale) for a following backward pass
+  // DG: every DoF lies in one cell (DoF = cell * 10 + node), so the
+  // inference kernel writes the correction itself and nothing is averaged
+  const bool direct = ls.op->is_dg();
   ls_launch_infer_fused(ls.n_cells, ls.d_cell_dofs.as<int32_t>(), ls.d_geo.as<float>(), ls.d_kappa.as<float>(), r,
                         nullptr, ls.d_x.as<float>(), ls.d_scale.as<float>(), ls.d_frag.p, ls.d_params.as<float>(),
-                        ls.d_corr.as<float>(), s);
-  ls_launch_assemble(ls.n_dofs, ls.d_slot_ptr.as<int64_t>(), ls.d_slot.as<int32_t>(), ls.d_corr.as<float>(), corr, s);
+                        ls.d_corr.as<float>(), s, direct ? corr : nullptr);
+  if (!direct)
+    ls_launch_assemble(ls.n_dofs, ls.d_slot_ptr.as<int64_t>(), ls.d_slot.as<int32_t>(), ls.d_corr.as<float>(), corr,
+                       s);
 }
 
 void ls_forward(LearnedSmoother& ls, const float* x, int64_t n, float* y, hipStream_t s) {
@@ -414,11 +419,13 @@ FcgStats solve_fcg_learned(const Operator& A, Multigrid& mg, LearnedSmoother* ls
     if (!learned) return false;
     launch_op(A, 0, az.p, z.p, s);
     // c = L(r - az): features formed inside the inference kernel
+    const bool direct = A.is_dg();  // one cell per DoF: no averaging
     ls_launch_infer_fused(ls->n_cells, ls->d_cell_dofs.as<int32_t>(), ls->d_geo.as<float>(),
                           ls->d_kappa.as<float>(), r.as<double>(), az.as<double>(), nullptr, nullptr, ls->d_frag.p,
-                          ls->d_params.as<float>(), ls->d_corr.as<float>(), s);
-    ls_launch_assemble(ls->n_dofs, ls->d_slot_ptr.as<int64_t>(), ls->d_slot.as<int32_t>(), ls->d_corr.as<float>(),
-                       c.as<double>(), s);
+                          ls->d_params.as<float>(), ls->d_corr.as<float>(), s, direct ? c.as<double>() : nullptr);
+    if (!direct)
+      ls_launch_assemble(ls->n_dofs, ls->d_slot_ptr.as<int64_t>(), ls->d_slot.as<int32_t>(), ls->d_corr.as<float>(),
+                         c.as<double>(), s);
     launch_op(A, 0, ac.p, c.p, s);
     ls_launch_omega(n, ls->d_part2.as<double>(), lsc, c.as<double>(), r.as<double>(), az.as<double>(),
                     ac.as<double>(), z.as<double>(), az.as<double>(), s);

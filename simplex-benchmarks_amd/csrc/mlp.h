@@ -130,7 +130,7 @@ void ls_launch_dy(int64_t ncells, const int32_t* cdofs, const int64_t* ptr, cons
 // features formed in the kernel from r - az (az may be null); x_out / scale_out optional
 void ls_launch_infer_fused(int64_t ncells, const int32_t* cdofs, const float* geo, const float* kappa,
                            const double* r, const double* az, float* x_out, float* scale_out, const void* frag,
-                           const float* params, float* corr, hipStream_t s);
+                           const float* params, float* corr, hipStream_t s, double* direct = nullptr);
 int ls_dot2_parts();
 // sc = {c.(r - az), c.ac, omega, unusable}; z += omega c, azout += omega ac
 // (az and azout may alias: az is only read before azout is written); part

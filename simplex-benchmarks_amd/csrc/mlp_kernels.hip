@@ -76,10 +76,10 @@ __global__ void ls_features_kernel(int64_t ncells, const int32_t* __restrict__ c
 // (|error| <= 1.5e-7, far below the BF16 rounding of the layer inputs)
 __device__ inline float gelu_fast(float x) {
   const float z = fabsf(x) * 0.70710678118654752f;
-  const float t = __fdividef(1.0f, 1.0f + 0.3275911f * z);
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * z);
   const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f +
                                                                                          t * 1.061405429f))));
-  float e = 1.0f - poly * __expf(-z * z);
+  float e = 1.0f - poly * __builtin_amdgcn_exp2f(-1.4426950408889634f * z * z);
   e = x < 0.0f ? -e : e;
   return 0.5f * x * (1.0f + e);
 }
@@ -103,6 +103,7 @@ struct InferArgs {
   const bf16x4* frag;
   const float* params;
   float* corr;
+  double* direct;  // when set (DG: DoF = cell * 10 + node, one cell per DoF): the correction itself, FP64
 };
 
 template <bool FUSED>
@@ -128,18 +129,41 @@ __global__ __launch_bounds__(64) void ls_infer_kernel(InferArgs g) {
     const int64_t cell = grp * 64 + l;
     const bool act = cell < ncells;
     if (FUSED) {
-      double rv[kMlpOut], sq = 0.0;
+      // branch-free gathers: every load is issued before the first use
+      // (clamped cell and DoF indices stay in bounds; masked afterwards)
+      const int64_t cl = act ? cell : ncells - 1;
       int32_t dd[kMlpOut];
-      for (int i = 0; i < kMlpOut; ++i) dd[i] = act ? g.cdofs[i * ncells + cell] : -1;
+      double rv[kMlpOut], av[kMlpOut];
+      float gv[10];
+#pragma unroll
+      for (int i = 0; i < kMlpOut; ++i) dd[i] = g.cdofs[i * ncells + cl];
+#pragma unroll
+      for (int i = 0; i < 10; ++i) gv[i] = g.geo[i * ncells + cl];
+#pragma unroll
+      for (int i = 0; i < kMlpOut; ++i) rv[i] = g.r[dd[i] >= 0 ? dd[i] : 0];
+      if (g.az) {
+#pragma unroll
+        for (int i = 0; i < kMlpOut; ++i) av[i] = g.az[dd[i] >= 0 ? dd[i] : 0];
+      } else {
+#pragma unroll
+        for (int i = 0; i < kMlpOut; ++i) av[i] = 0.0;
+      }
+      const float kap = g.kappa[cl];
+      double sq = 0.0;
+#pragma unroll
       for (int i = 0; i < kMlpOut; ++i) {
-        rv[i] = dd[i] >= 0 ? g.r[dd[i]] - (g.az ? g.az[dd[i]] : 0.0) : 0.0;
+        rv[i] = act && dd[i] >= 0 ? rv[i] - av[i] : 0.0;
         sq += rv[i] * rv[i];
       }
-      const double sn = sqrt(sq) + 1e-300;
-      const double inv = 1.0 / sn;
+      // 1/|r_K| from v_rsq_f64 (features are rounded to BF16); a zero
+      // residual gives zero features
+      const double inv = __builtin_amdgcn_rsq(sq > 1e-300 ? sq : 1e-300);
+      const double sn = sq * inv;
+#pragma unroll
       for (int i = 0; i < kMlpOut; ++i) xs[l][i] = (float)(rv[i] * inv);
-      for (int i = 0; i < 10; ++i) xs[l][kMlpOut + i] = act ? g.geo[i * ncells + cell] : 0.0f;
-      ss[l] = act ? (float)sn * g.kappa[cell] : 0.0f;
+#pragma unroll
+      for (int i = 0; i < 10; ++i) xs[l][kMlpOut + i] = act ? gv[i] : 0.0f;
+      ss[l] = act ? (float)sn * kap : 0.0f;
       if (act) {
         if (g.x_out)
           for (int k = 0; k < kMlpIn; ++k) g.x_out[cell * kMlpIn + k] = xs[l][k];
@@ -188,30 +212,38 @@ __global__ __launch_bounds__(64) void ls_infer_kernel(InferArgs g) {
       if (row < kMlpOut)
         for (int j = 0; j < 4; ++j) {
           const int64_t cj = grp * 64 + r0 + kq + j;
-          if (cj < ncells) g.corr[cj * kMlpOut + row] = (o[j] + bias3) * ss[r0 + kq + j];
+          if (cj < ncells) {
+            const float v = (o[j] + bias3) * ss[r0 + kq + j];
+            if (g.direct) g.direct[cj * kMlpOut + row] = (double)v;
+            else g.corr[cj * kMlpOut + row] = v;
+          }
         }
     }
     __syncthreads();
   }
 }
 
-// out[d] = mean of the cell corrections at DoF d (0 where no cell slot: Dirichlet)
+// out[d] = mean of the cell corrections at DoF d (0 where no cell slot:
+// Dirichlet).  Eight slot indices, then eight values, are loaded per step
+// with clamped indices (two round trips per eight slots); the sum runs in
+// slot order.
 __global__ void ls_assemble_kernel(int64_t ndofs, const int64_t* __restrict__ ptr, const int32_t* __restrict__ slot,
                                    const float* __restrict__ corr, double* __restrict__ out) {
   const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= ndofs) return;
   const int32_t k0 = (int32_t)ptr[d], k1 = (int32_t)ptr[d + 1];
   float s = 0.0f;
-  int32_t k = k0;
-  for (; k + 4 <= k1; k += 4) {  // four independent gathers in flight
-    const int32_t a = slot[k], b = slot[k + 1], c = slot[k + 2], e = slot[k + 3];
-    const float va = corr[a], vb = corr[b], vc = corr[c], ve = corr[e];
-    s += va;
-    s += vb;
-    s += vc;
-    s += ve;
+  for (int32_t k = k0; k < k1; k += 8) {
+    int32_t sl[8];
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sl[u] = slot[min(k + u, k1 - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = corr[sl[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k + u < k1) s += v[u];
   }
-  for (; k < k1; ++k) s += corr[slot[k]];
   out[d] = k1 > k0 ? (double)(s * __builtin_amdgcn_rcpf((float)(k1 - k0))) : 0.0;
 }
 
@@ -473,7 +505,7 @@ void ls_launch_infer(int64_t ncells, const float* x, const void* frag, const flo
 
 void ls_launch_infer_fused(int64_t ncells, const int32_t* cdofs, const float* geo, const float* kappa,
                            const double* r, const double* az, float* x_out, float* scale_out, const void* frag,
-                           const float* params, float* corr, hipStream_t s) {
+                           const float* params, float* corr, hipStream_t s, double* direct) {
   InferArgs g{};
   g.ncells = ncells;
   g.cdofs = cdofs;
@@ -486,6 +518,7 @@ void ls_launch_infer_fused(int64_t ncells, const int32_t* cdofs, const float* ge
   g.frag = static_cast<const bf16x4*>(frag);
   g.params = params;
   g.corr = corr;
+  g.direct = direct;
   const int64_t ngroups = (ncells + 63) / 64;
   if (ncells > 0)
     hipLaunchKernelGGL(ls_infer_kernel<true>, dim3((unsigned)std::min<int64_t>(ngroups, 16384)), dim3(64), 0, s, g);
