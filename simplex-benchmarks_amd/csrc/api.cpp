@@ -46,7 +46,7 @@ void upload_cg(Operator& op) {
   op.d_ghost_idx.upload(pl.ghost_idx);
   op.d_ccol.upload(pl.ccol);
   op.d_fix_ptr.upload(pl.fix_ptr);
-  op.d_fix_src.upload(pl.fix_src);
+  op.d_pslot.upload(pl.pslot);
   op.d_lidx.upload(pl.lidx);
   op.d_partial.alloc(std::max<int64_t>(pl.n_ghost_total, 1) * op.ncomp * op.wbytes());
   if (op.precision == MF_F64) {
@@ -143,12 +143,12 @@ void host_apply_cg_t(const Operator& op, int mode, T* dst, const T* src) {
     }
     for (int j = 0; j < no + ng; ++j) {
       if (j < no) dstc[o0 + j] = sy[j];
-      else partial[g0 + j - no] = sy[j];
+      else partial[pl.pslot[g0 + j - no]] = sy[j];
     }
   }
   for (int64_t t = 0; t < pl.n_free - pl.n_int; ++t) {
-    T s = partial[pl.fix_src[pl.fix_ptr[t]]];
-    for (int32_t k = pl.fix_ptr[t] + 1; k < pl.fix_ptr[t + 1]; ++k) s += partial[pl.fix_src[k]];
+    T s = partial[pl.fix_ptr[t]];
+    for (int32_t k = pl.fix_ptr[t] + 1; k < pl.fix_ptr[t + 1]; ++k) s += partial[k];
     dstc[pl.n_int + t] = s;
   }
   for (int64_t i = pl.n_free; i < pl.n_dofs; ++i) dstc[i] = mode == 0 ? srcc[i] : T(1);

@@ -45,6 +45,66 @@ struct CGShape {
   static constexpr int NQ = P == 1 ? 1 : (P == 2 ? 4 : 11);
 };
 
+// P2 cell: y = A_K u in barycentric form at the symmetric 4-point rule
+// (point m: lambda_m = a, the other three b, a + 3b = 1, weight 1/24 each).
+// d u / d lambda_a = u_a (4 lambda_a - 1) + 4 sum_{b != a} u_ab lambda_b is
+// linear in lambda, so at point m it is P_a + c N_am with
+//   P_a = (4b - 1) u_a + 4b S_a,  S_a = sum_{b != a} u_ab,  c = 4(a - b),
+//   N_am = u_a (a = m) or u_am (a != m),
+// and the reference gradient is (D_1 - D_0, D_2 - D_0, D_3 - D_0).  The
+// transposed step uses tau(m) = (-(t_0 + t_1 + t_2), t_0, t_1, t_2) of the
+// flux t = w M g and T_a = sum_m tau_a(m):
+//   y_v  = (4b - 1) T_v + c tau_v(v),
+//   y_vw = 4b (T_v + T_w) + c (tau_v(w) + tau_w(v)).
+// 138 FMA-class operations against 320 for the gradient-table form of the
+// same quadrature (equal up to rounding).  Node order: vertices 0-3, then
+// edges (01) (02) (03) (12) (13) (23).
+template <typename T>
+__device__ __forceinline__ void p2_cell_apply(const T* u, T m0, T m1, T m2, T m3, T m4, T m5, T* y) {
+  constexpr double kB = q_tet_d2_pts[3][0], kA = q_tet_d2_pts[2][0], kW = q_tet_d2_w[0];
+  static_assert(q_tet_d2_pts[0][2] == kA && q_tet_d2_pts[1][1] == kA && q_tet_d2_pts[0][0] == kB,
+                "4-point rule with lambda_{3-q} = a at point q");
+  static_assert(q_tet_d2_w[1] == kW && q_tet_d2_w[2] == kW && q_tet_d2_w[3] == kW, "equal weights");
+  const T c4b1 = T(4 * kB - 1), c4b = T(4 * kB), c = T(4 * (kA - kB));
+  // edge value of vertex pair (a, b), a != b
+  constexpr int E[4][4] = {{-1, 4, 5, 6}, {4, -1, 7, 8}, {5, 7, -1, 9}, {6, 8, 9, -1}};
+  T P[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    int e[3], k = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (b != a) e[k++] = E[a][b];
+    P[a] = c4b1 * u[a] + c4b * ((u[e[0]] + u[e[1]]) + u[e[2]]);
+  }
+  const T w = T(kW);
+  const T M0 = w * m0, M1 = w * m1, M2 = w * m2, M3 = w * m3, M4 = w * m4, M5 = w * m5;
+  T tau[4][4];  // tau[m][a]
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    T D[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) D[a] = P[a] + c * u[a == m ? a : E[a][m]];
+    const T gx = D[1] - D[0], gy = D[2] - D[0], gz = D[3] - D[0];
+    const T tx = M0 * gx + M1 * gy + M2 * gz;
+    const T ty = M1 * gx + M3 * gy + M4 * gz;
+    const T tz = M2 * gx + M4 * gy + M5 * gz;
+    tau[m][0] = -((tx + ty) + tz);
+    tau[m][1] = tx;
+    tau[m][2] = ty;
+    tau[m][3] = tz;
+  }
+  T Ts[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) Ts[a] = ((tau[0][a] + tau[1][a]) + tau[2][a]) + tau[3][a];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) y[v] = c4b1 * Ts[v] + c * tau[v][v];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int x = v + 1; x < 4; ++x) y[E[v][x]] = c4b * (Ts[v] + Ts[x]) + c * (tau[x][v] + tau[v][x]);
+}
+
 // MODE 0: y = A_K u;  MODE 1: y_i = (A_K)_ii
 // shape: G[(q*NL + i)*3 + d] = d phi_i / d xhat_d at point q, then w[q].
 // ATOMIC: accumulate with ds_add (fast for f64 on gfx950) or with a plain
@@ -56,7 +116,8 @@ template <int P, typename T, int MODE, bool ATOMIC, int NC>
 __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ own_start,
     const int32_t* __restrict__ ghost_start, const int32_t* __restrict__ ghost_idx,
-    const uint8_t* __restrict__ ccol, const uint16_t* __restrict__ lidx, int64_t ncells_total,
+    const int32_t* __restrict__ pslot, const uint8_t* __restrict__ ccol, const uint16_t* __restrict__ lidx,
+    int64_t ncells_total,
     const T* __restrict__ metric, const T* __restrict__ shape, const T* src,
     T* __restrict__ dst, T* __restrict__ partial, int64_t cstride, int64_t pstride, RowEpilogue<T> ep) {
   constexpr int NL = CGShape<P>::NL, NQ = CGShape<P>::NQ;
@@ -141,7 +202,12 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     // P1/P2: compile-time tables (literal constants after unrolling); P3
     // has too many distinct values for scalar registers and reads the table
     // from memory one quadrature point at a time.
-    if constexpr (P <= 2) {
+    if constexpr (P == 2 && MODE == 0) {
+      T u[NL];
+#pragma unroll
+      for (int i = 0; i < NL; ++i) u[i] = su[li[i]];
+      p2_cell_apply<T>(u, m0, m1, m2, m3, m4, m5, y);
+    } else if constexpr (P <= 2) {
       constexpr tab::CellGrad<P> TB = tab::make_cell_grad<P>();
       if (MODE == 0) {
         T u[NL];
@@ -250,47 +316,47 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     } else {
       epilogue_rows<T, 4>(ep, o0, no, lane, nt, sy);
     }
-    for (int j = lane; j < ng; j += nt) pc[g0 + j] = sy[no + j];
+    for (int j = lane; j < ng; j += nt) pc[pslot[g0 + j]] = sy[no + j];
   }
 }
 
-// Interface DoF n_int + t = sum of its patch partials partial[fix_src[k]],
-// k in [fix_ptr[t], fix_ptr[t+1]) (patch order); Dirichlet rows dst = src
-// (MODE 0) or 1 (MODE 1).
+// Interface DoF n_int + t = sum of its patch partials partial[k], k in
+// [fix_ptr[t], fix_ptr[t+1]) (contiguous, patch order); Dirichlet rows
+// dst = src (MODE 0) or 1 (MODE 1).  Every load that does not depend on
+// fix_ptr (the rows' epilogue inputs, the Dirichlet source) is issued first.
 template <typename T, int MODE>
 __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* __restrict__ fix_ptr,
-                                const int32_t* __restrict__ fix_src, const T* __restrict__ partial,
-                                int64_t n0, int64_t n1, const T* src,
+                                const T* __restrict__ partial, int64_t n0, int64_t n1, const T* src,
                                 T* __restrict__ dst, int nc, int64_t cstride, int64_t pstride, RowEpilogue<T> ep) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool iface = t < n_iface, dir = n0 + t < n1;
+  const int64_t row = n_int + (iface ? t : 0), j = dir ? n0 + t : n0;
   for (int c = 0; c < nc; ++c) {
     const T* pc = partial + c * pstride;
-    if (t < n_iface) {
-      const int64_t row = n_int + t;
-      EpilogueIn<T> in{};
-      if (ep.kind != 0) in = epilogue_load(ep, row);  // in flight with the partials
+    EpilogueIn<T> in_i{}, in_d{};
+    T ax_d = T(1);
+    if (dir && MODE == 0) ax_d = src[c * cstride + j];
+    if (ep.kind != 0) {
+      if (iface) in_i = epilogue_load(ep, row);
+      if (dir) in_d = epilogue_load(ep, j);
+    }
+    if (iface) {
       const int32_t k0 = fix_ptr[t], k1 = fix_ptr[t + 1];
-      // partials in patch order, eight indices then eight values per step
       T s = T(0);
       for (int32_t k = k0; k < k1; k += 8) {
-        int32_t src_k[8];
         T v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) src_k[u] = fix_src[min(k + u, k1 - 1)];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = pc[src_k[u]];
+        for (int u = 0; u < 8; ++u) v[u] = pc[min(k + u, k1 - 1)];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
           if (k + u < k1) s = k + u == k0 ? v[u] : s + v[u];
       }
       if (ep.kind == 0) dst[c * cstride + row] = s;
-      else epilogue_finish(ep, row, s, in);
+      else epilogue_finish(ep, row, s, in_i);
     }
-    const int64_t j = n0 + t;
-    if (j < n1) {
-      const T ax = MODE == 0 ? src[c * cstride + j] : T(1);
-      if (ep.kind == 0) dst[c * cstride + j] = ax;
-      else apply_epilogue(ep, j, ax);
+    if (dir) {
+      if (ep.kind == 0) dst[c * cstride + j] = ax_d;
+      else epilogue_finish(ep, j, ax_d, in_d);
     }
   }
 }
@@ -309,7 +375,7 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpi
   if (pl.n_patches > 0)
     hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_patches), dim3(kWave * pl.waves), shmem, s,
                        op.d_cell_start.as<int32_t>(), op.d_own_start.as<int32_t>(),
-                       op.d_ghost_start.as<int32_t>(), op.d_ghost_idx.as<int32_t>(),
+                       op.d_ghost_start.as<int32_t>(), op.d_ghost_idx.as<int32_t>(), op.d_pslot.as<int32_t>(),
                        op.d_ccol.as<uint8_t>(), op.d_lidx.as<uint16_t>(), pl.n_cells,
                        op.d_metric.as<T>(), op.d_shape.as<T>(), src, dst, op.d_partial.as<T>(), pl.n_dofs,
                        std::max<int64_t>(pl.n_ghost_total, 1), ep);
@@ -318,9 +384,8 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpi
   const int64_t nthreads = std::max(nif, ncon);
   if (nthreads > 0)
     hipLaunchKernelGGL((cg_fixup_kernel<T, MODE>), dim3((unsigned)((nthreads + 255) / 256)), dim3(256),
-                       0, s, pl.n_int, nif, op.d_fix_ptr.as<int32_t>(), op.d_fix_src.as<int32_t>(),
-                       op.d_partial.as<T>(), pl.n_free, pl.n_dofs, src, dst, NC, pl.n_dofs,
-                       std::max<int64_t>(pl.n_ghost_total, 1), ep);
+                       0, s, pl.n_int, nif, op.d_fix_ptr.as<int32_t>(), op.d_partial.as<T>(), pl.n_free, pl.n_dofs,
+                       src, dst, NC, pl.n_dofs, std::max<int64_t>(pl.n_ghost_total, 1), ep);
   SMF_HIP(hipGetLastError());
 }
 

@@ -235,8 +235,8 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   plan.n_ghost_total = (int64_t)plan.ghost_idx.size();
 
   // ---- interface fix-up: interface DoF n_int + t is the sum of its patch
-  //      partials partial[fix_src[k]], k in [fix_ptr[t], fix_ptr[t+1]), in
-  //      patch order ----
+  //      partials, slots k in [fix_ptr[t], fix_ptr[t+1]) in patch order;
+  //      ghost entry g writes its partial to slot pslot[g] ----
   {
     const int64_t ng = plan.n_ghost_total, nif = plan.n_free - plan.n_int;
     std::vector<int32_t> cnt(nif + 1, 0);
@@ -249,6 +249,8 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
     plan.fix_src.resize(ng);
     std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
     for (int64_t k = 0; k < ng; ++k) plan.fix_src[pos[plan.ghost_idx[k] - plan.n_int]++] = (int32_t)k;
+    plan.pslot.resize(ng);
+    for (int64_t k = 0; k < ng; ++k) plan.pslot[plan.fix_src[k]] = (int32_t)k;
   }
 
   // ---- geometry ----

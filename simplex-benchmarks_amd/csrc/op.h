@@ -84,8 +84,10 @@ struct CGPlan {
   int waves = 1;                     // wavefronts per patch workgroup (1..4)
   std::vector<int32_t> cell_start, own_start, ghost_start, ghost_idx;
   std::vector<uint8_t> ccol;         // per cell: colour within its patch (cells sorted by colour)
-  // interface fix-up: DoF n_int+t = sum of partial[fix_src[k]], k in [fix_ptr[t], fix_ptr[t+1])
-  std::vector<int32_t> fix_ptr, fix_src;
+  // interface fix-up: DoF n_int+t = sum of the partials k in [fix_ptr[t],
+  // fix_ptr[t+1]) (patch order); ghost entry g (patch order) writes its
+  // partial to slot pslot[g], fix_src is the inverse map
+  std::vector<int32_t> fix_ptr, fix_src, pslot;
   std::vector<uint16_t> lidx;        // n_loc x n_cells patch-local index (0xFFFF = constrained)
   std::vector<double> metric;        // 6 x n_cells: |det J| J^-1 J^-T (xx xy xz yy yz zz)
   std::vector<double> jit;           // 9 x n_cells: J^-T row-major
@@ -145,7 +147,7 @@ struct Operator {
   bool is_dg() const { return kind == MF_DG_SIPG || kind == MF_DG_VECTOR || kind == MF_DG_HELMHOLTZ; }
   // device data
   DevBuf d_cell_start, d_own_start, d_ghost_start, d_ghost_idx, d_ccol, d_fix_ptr,
-      d_fix_src, d_lidx, d_metric, d_shape, d_partial;
+      d_pslot, d_lidx, d_metric, d_shape, d_partial;
   DevBuf d_bnd_rows, d_bnd_ptr, d_bnd_col, d_bnd_val, d_bnd_dense;
   DevBuf d_dg_geo, d_dg_nbr, d_dg_nface, d_dg_cell_tab, d_dg_face_tab;
   bool on_device = false;
