@@ -143,6 +143,9 @@ typedef struct mf_mg_options {
   int32_t eig_cg_iterations;  /* CG steps of the lambda_max estimate (10) */
   int32_t level_precision;    /* MF_F32 (default) or MF_F64 */
   int32_t max_coarse_dofs;    /* limit of the dense coarse solver (4096) */
+  int32_t fuse_steps;         /* 1 (default): scalar CG levels run the Chebyshev steps without a
+                                 fix-up launch per step (interface rows completed by the next step);
+                                 0: one operator application + fix-up per step */
 } mf_mg_options;
 
 #define MF_MAX_LEVELS 16

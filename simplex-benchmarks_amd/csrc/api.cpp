@@ -263,6 +263,7 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
     } else {
       op.ncomp = kind == MF_CG_VECTOR ? 3 : 1;
       op.cg.ncomp = op.ncomp;
+      op.cg.value_bytes = op.precision == MF_F64 ? 8 : 4;
       build_cg_plan(op.mesh, degree, op.opt, op.cg);
       make_cg_shape(op.cg);
       if (!op.opt.host_only) upload_cg(op);
@@ -421,6 +422,7 @@ void mf_mg_default_options(mf_mg_options* opt) {
   opt->eig_cg_iterations = d.eig_cg_iterations;
   opt->level_precision = d.level_precision;
   opt->max_coarse_dofs = d.max_coarse_dofs;
+  opt->fuse_steps = d.fuse_steps;
 }
 
 int mf_mg_create(mf_op_t fine, const mf_mg_options* opt, mf_mg_t* out) {
@@ -435,6 +437,7 @@ int mf_mg_create(mf_op_t fine, const mf_mg_options* opt, mf_mg_t* out) {
       o.eig_cg_iterations = opt->eig_cg_iterations;
       o.level_precision = opt->level_precision;
       o.max_coarse_dofs = opt->max_coarse_dofs;
+      o.fuse_steps = opt->fuse_steps;
     }
     if (o.smoother_degree < 1 || o.smoothing_range <= 1 || o.eig_cg_iterations < 1)
       throw std::runtime_error("invalid multigrid options");

@@ -105,6 +105,42 @@ __device__ __forceinline__ void p2_cell_apply(const T* u, T m0, T m1, T m2, T m3
     for (int x = v + 1; x < 4; ++x) y[E[v][x]] = c4b * (Ts[v] + Ts[x]) + c * (tau[x][v] + tau[v][x]);
 }
 
+// Interface rows left pending by the previous step of a ChebyRun: while
+// staging, ghost DoF g (interface row t = g - n_int) gets
+//   d' = c1 ds[t] + c2 dinv[g] (b[g] - sum of part[fix_ptr[t]..fix_ptr[t+1]))
+//   (d' = c2 dinv[g] (...) when first),  x' = xs[t] + d',
+// the fix-up's epilogue in the same order and form; the first patch of the
+// partial list stores x', d' to xw[t], dw[t].
+template <typename T>
+struct Pending {
+  int on = 0, first = 0;
+  T c1 = T(0), c2 = T(0);
+  int64_t n_int = 0;
+  const int32_t* fix_ptr = nullptr;
+  const T* part = nullptr;
+  const T* xs = nullptr;
+  const T* ds = nullptr;
+  const T* b = nullptr;
+  const T* dinv = nullptr;
+  T* xw = nullptr;
+  T* dw = nullptr;
+};
+
+// sum of the partials part[k0..k1) in order, eight loads in flight per step
+template <typename T>
+__device__ __forceinline__ T sum_partials(const T* __restrict__ part, int32_t k0, int32_t k1) {
+  T s = T(0);
+  for (int32_t k = k0; k < k1; k += 8) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = part[min(k + u, k1 - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k + u < k1) s = k + u == k0 ? v[u] : s + v[u];
+  }
+  return s;
+}
+
 // MODE 0: y = A_K u;  MODE 1: y_i = (A_K)_ii
 // shape: G[(q*NL + i)*3 + d] = d phi_i / d xhat_d at point q, then w[q].
 // ATOMIC: accumulate with ds_add (fast for f64 on gfx950) or with a plain
@@ -119,7 +155,8 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int32_t* __restrict__ pslot, const uint8_t* __restrict__ ccol, const uint16_t* __restrict__ lidx,
     int64_t ncells_total,
     const T* __restrict__ metric, const T* __restrict__ shape, const T* src,
-    T* __restrict__ dst, T* __restrict__ partial, int64_t cstride, int64_t pstride, RowEpilogue<T> ep) {
+    T* __restrict__ dst, T* __restrict__ partial, int64_t cstride, int64_t pstride, RowEpilogue<T> ep,
+    Pending<T> pg) {
   constexpr int NL = CGShape<P>::NL, NQ = CGShape<P>::NQ;
   extern __shared__ unsigned char smem_raw[];
   T* sbase = reinterpret_cast<T*>(smem_raw);
@@ -154,16 +191,55 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
         for (int u = 0; u < SB; ++u)
           if (j0 + u * nt + lane < no) su[j0 + u * nt + lane] = v[u];
       }
-      for (int j0 = 0; j0 < ng; j0 += SB * nt) {
-        int32_t gi[SB];
-        T v[SB];
+      if (pg.on) {
+        // ghosts are interface DoFs whose previous step is still pending
+        constexpr int GB = 2;
+        for (int j0 = 0; j0 < ng; j0 += GB * nt) {
+          int32_t gi[GB], sl[GB], k0[GB], k1[GB];
+          T xv[GB], dv[GB], bv[GB], iv[GB];
 #pragma unroll
-        for (int u = 0; u < SB; ++u) gi[u] = ghost_idx[g0 + min(j0 + u * nt + lane, ng - 1)];
+          for (int u = 0; u < GB; ++u) {
+            const int jj = min(j0 + u * nt + lane, ng - 1);
+            gi[u] = ghost_idx[g0 + jj];
+            sl[u] = pslot[g0 + jj];
+          }
 #pragma unroll
-        for (int u = 0; u < SB; ++u) v[u] = sc[gi[u]];
+          for (int u = 0; u < GB; ++u) {
+            const int64_t t = gi[u] - pg.n_int;
+            k0[u] = pg.fix_ptr[t];
+            k1[u] = pg.fix_ptr[t + 1];
+            xv[u] = pg.xs[t];
+            dv[u] = pg.first ? T(0) : pg.ds[t];
+            bv[u] = pg.b[gi[u]];
+            iv[u] = pg.dinv[gi[u]];
+          }
 #pragma unroll
-        for (int u = 0; u < SB; ++u)
-          if (j0 + u * nt + lane < ng) su[no + j0 + u * nt + lane] = v[u];
+          for (int u = 0; u < GB; ++u) {
+            const T r = bv[u] - sum_partials(pg.part, k0[u], k1[u]);
+            const T di = pg.first ? pg.c2 * iv[u] * r : pg.c1 * dv[u] + pg.c2 * iv[u] * r;
+            const T xn = xv[u] + di;
+            const int j = j0 + u * nt + lane;
+            if (j < ng) {
+              su[no + j] = xn;
+              if (sl[u] == k0[u]) {
+                pg.xw[gi[u] - pg.n_int] = xn;
+                pg.dw[gi[u] - pg.n_int] = di;
+              }
+            }
+          }
+        }
+      } else {
+        for (int j0 = 0; j0 < ng; j0 += SB * nt) {
+          int32_t gi[SB];
+          T v[SB];
+#pragma unroll
+          for (int u = 0; u < SB; ++u) gi[u] = ghost_idx[g0 + min(j0 + u * nt + lane, ng - 1)];
+#pragma unroll
+          for (int u = 0; u < SB; ++u) v[u] = sc[gi[u]];
+#pragma unroll
+          for (int u = 0; u < SB; ++u)
+            if (j0 + u * nt + lane < ng) su[no + j0 + u * nt + lane] = v[u];
+        }
       }
       if (lane == 0) su[nloc] = T(0);
     }
@@ -341,16 +417,7 @@ __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* _
       if (dir) in_d = epilogue_load(ep, j);
     }
     if (iface) {
-      const int32_t k0 = fix_ptr[t], k1 = fix_ptr[t + 1];
-      T s = T(0);
-      for (int32_t k = k0; k < k1; k += 8) {
-        T v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = pc[min(k + u, k1 - 1)];
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-          if (k + u < k1) s = k + u == k0 ? v[u] : s + v[u];
-      }
+      const T s = sum_partials(pc, fix_ptr[t], fix_ptr[t + 1]);
       if (ep.kind == 0) dst[c * cstride + row] = s;
       else epilogue_finish(ep, row, s, in_i);
     }
@@ -361,8 +428,11 @@ __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* _
   }
 }
 
+// partial_out/pg/fixup: a ChebyRun step (its own partial buffer, pending
+// interface rows of the previous step, no fix-up launch)
 template <int P, typename T, int MODE, int NC>
-void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpilogue<T> ep = {}) {
+void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpilogue<T> ep = {},
+                 Pending<T> pg = {}, T* partial_out = nullptr, bool fixup = true) {
   const CGPlan& pl = op.cg;
   const size_t shmem = NC * 2 * ((size_t)pl.max_local + 1) * sizeof(T);
   const bool atomic = op.opt.accumulate == 1 || (op.opt.accumulate == 0 && sizeof(T) == 8);
@@ -377,8 +447,13 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpi
                        op.d_cell_start.as<int32_t>(), op.d_own_start.as<int32_t>(),
                        op.d_ghost_start.as<int32_t>(), op.d_ghost_idx.as<int32_t>(), op.d_pslot.as<int32_t>(),
                        op.d_ccol.as<uint8_t>(), op.d_lidx.as<uint16_t>(), pl.n_cells,
-                       op.d_metric.as<T>(), op.d_shape.as<T>(), src, dst, op.d_partial.as<T>(), pl.n_dofs,
-                       std::max<int64_t>(pl.n_ghost_total, 1), ep);
+                       op.d_metric.as<T>(), op.d_shape.as<T>(), src, dst,
+                       partial_out ? partial_out : op.d_partial.as<T>(), pl.n_dofs,
+                       std::max<int64_t>(pl.n_ghost_total, 1), ep, pg);
+  if (!fixup) {
+    SMF_HIP(hipGetLastError());
+    return;
+  }
   const int64_t nif = pl.n_free - pl.n_int;
   const int64_t ncon = pl.n_dofs - pl.n_free;
   const int64_t nthreads = std::max(nif, ncon);
@@ -493,6 +568,119 @@ void launch_cg_epilogue(const Operator& op, const T* x, RowEpilogue<T> ep, hipSt
 }
 template void launch_cg_epilogue<double>(const Operator&, const double*, RowEpilogue<double>, hipStream_t);
 template void launch_cg_epilogue<float>(const Operator&, const float*, RowEpilogue<float>, hipStream_t);
+
+namespace {
+
+template <typename T>
+struct ChebySteps {
+  int n = 0;
+  ChebyStep<T> s[kMaxChebySteps];
+};
+
+// Close of a ChebyRun: interface row t gets the last step's pending update
+// (pg, written to x, d at n_int + t); Dirichlet row j (A row = identity)
+// runs every step of the run: d = c1 d + c2 dinv (b - x), x += d.
+template <typename T>
+__global__ void cg_cheby_close_kernel(int64_t n_iface, Pending<T> pg, int64_t n0, int64_t n1, T* __restrict__ x,
+                                      T* __restrict__ d, ChebySteps<T> st) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n_iface) {
+    const int64_t row = pg.n_int + t;
+    const T xv = pg.xs[t], dv = pg.first ? T(0) : pg.ds[t], bv = pg.b[row], iv = pg.dinv[row];
+    const T r = bv - sum_partials(pg.part, pg.fix_ptr[t], pg.fix_ptr[t + 1]);
+    const T di = pg.first ? pg.c2 * iv * r : pg.c1 * dv + pg.c2 * iv * r;
+    x[row] = xv + di;
+    d[row] = di;
+  }
+  const int64_t j = n0 + t;
+  if (j < n1) {
+    T xv = x[j], dv = d[j];
+    const T bv = pg.b[j], iv = pg.dinv[j];
+    for (int k = 0; k < st.n; ++k) {
+      const T r = bv - xv;
+      const T di = st.s[k].first ? st.s[k].c2 * iv * r : st.s[k].c1 * dv + st.s[k].c2 * iv * r;
+      dv = di;
+      xv = xv + di;
+    }
+    x[j] = xv;
+    d[j] = dv;
+  }
+}
+
+// pending update of run step k (interface rows of step k, applied by step k + 1 or the close)
+template <typename T>
+Pending<T> pending_of(const Operator& op, const ChebyRun<T>& run, int k) {
+  Pending<T> pg;
+  pg.on = 1;
+  pg.first = run.steps[k].first;
+  pg.c1 = run.steps[k].c1;
+  pg.c2 = run.steps[k].c2;
+  pg.n_int = op.cg.n_int;
+  pg.fix_ptr = op.d_fix_ptr.as<int32_t>();
+  pg.part = run.part[k % 2];
+  pg.xs = k == 0 ? run.x + op.cg.n_int : run.xi[k % 2];
+  pg.ds = k == 0 ? run.d + op.cg.n_int : run.di[k % 2];
+  pg.b = run.b;
+  pg.dinv = run.dinv;
+  return pg;
+}
+
+}  // namespace
+
+bool cg_has_cheby_run(const Operator& op) { return !op.is_dg() && op.ncomp == 1 && !op.cg.nitsche; }
+
+template <typename T>
+void launch_cg_cheby_step(const Operator& op, ChebyRun<T>& run, T c1, T c2, int first, hipStream_t s) {
+  if (!cg_has_cheby_run(op)) throw std::runtime_error("Chebyshev run: scalar CG operator without Nitsche rows");
+  if ((op.precision == MF_F64) != (sizeof(T) == 8)) throw std::runtime_error("Chebyshev run: precision mismatch");
+  const int k = run.n;
+  if (k >= kMaxChebySteps) throw std::runtime_error("Chebyshev run: too many steps");
+  RowEpilogue<T> ep;
+  ep.kind = 1;
+  ep.first = first;
+  ep.c1 = c1;
+  ep.c2 = c2;
+  ep.b = run.b;
+  ep.dinv = run.dinv;
+  ep.d = run.d;
+  ep.x = run.x;
+  Pending<T> pg;
+  if (k > 0) {
+    pg = pending_of(op, run, k - 1);
+    pg.xw = run.xi[k % 2];
+    pg.dw = run.di[k % 2];
+  }
+  switch (op.cg.p) {
+    case 1: launch_cg_t<1, T, 0, 1>(op, nullptr, run.x, s, ep, pg, run.part[k % 2], false); break;
+    case 2: launch_cg_t<2, T, 0, 1>(op, nullptr, run.x, s, ep, pg, run.part[k % 2], false); break;
+    case 3: launch_cg_t<3, T, 0, 1>(op, nullptr, run.x, s, ep, pg, run.part[k % 2], false); break;
+    default: throw std::runtime_error("unsupported CG degree");
+  }
+  run.steps[k] = ChebyStep<T>{c1, c2, first};
+  run.n = k + 1;
+}
+
+template <typename T>
+void launch_cg_cheby_close(const Operator& op, ChebyRun<T>& run, hipStream_t s) {
+  if (run.n == 0) return;
+  const CGPlan& pl = op.cg;
+  const Pending<T> pg = pending_of(op, run, run.n - 1);
+  ChebySteps<T> st;
+  st.n = run.n;
+  for (int k = 0; k < run.n; ++k) st.s[k] = run.steps[k];
+  const int64_t nif = pl.n_free - pl.n_int, ncon = pl.n_dofs - pl.n_free;
+  const int64_t nthreads = std::max(nif, ncon);
+  if (nthreads > 0)
+    hipLaunchKernelGGL(cg_cheby_close_kernel<T>, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s, nif, pg,
+                       pl.n_free, pl.n_dofs, run.x, run.d, st);
+  SMF_HIP(hipGetLastError());
+  run.n = 0;
+}
+
+template void launch_cg_cheby_step<double>(const Operator&, ChebyRun<double>&, double, double, int, hipStream_t);
+template void launch_cg_cheby_step<float>(const Operator&, ChebyRun<float>&, float, float, int, hipStream_t);
+template void launch_cg_cheby_close<double>(const Operator&, ChebyRun<double>&, hipStream_t);
+template void launch_cg_cheby_close<float>(const Operator&, ChebyRun<float>&, hipStream_t);
 
 void launch_dg(const Operator& op, int mode, void* dst, const void* src, hipStream_t s);
 

@@ -38,13 +38,14 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
 
   // cells per wave by degree (P1 1024, P2 512, P3 128); a patch is one
   // workgroup of up to 4 waves.  The default patch is 4 waves' worth of
-  // cells (1 for the 3-component operator, whose LDS is 3x larger), halved
-  // (down to 64 cells) until there are >= 1024 patches, so that small
-  // operators (multigrid levels) still fill the 256 CUs
+  // cells (2 for FP64 P2, whose patches need twice the LDS; 1 for the
+  // 3-component operator, whose LDS is 3x larger), halved (down to 64
+  // cells) until there are >= 1024 patches, so that small operators
+  // (multigrid levels) still fill the 256 CUs
   const int per_wave = opt.cells_per_wave > 0 ? opt.cells_per_wave : (p == 1 ? 1024 : (p == 2 ? 512 : 128));
   int patch = opt.patch_cells;
   if (patch <= 0) {
-    patch = (plan.ncomp == 3 ? 1 : 4) * per_wave;
+    patch = (plan.ncomp == 3 ? 1 : (p == 2 && plan.value_bytes == 8 ? 2 : 4)) * per_wave;
     while (patch > 64 && mesh.nc / patch < 1024) patch /= 2;
   }
   if (patch > 4096) throw std::runtime_error("patch_cells must be <= 4096");

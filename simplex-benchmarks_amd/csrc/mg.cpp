@@ -51,6 +51,7 @@ std::unique_ptr<Operator> make_cg_operator(const Mesh& mesh, int degree, int pre
   op->opt = opt;
   op->opt.host_only = 0;
   op->mesh = mesh;
+  op->cg.value_bytes = precision == MF_F64 ? 8 : 4;
   build_cg_plan(op->mesh, degree, op->opt, op->cg);
   make_cg_shape(op->cg);
   upload_cg(*op);
@@ -401,10 +402,12 @@ double estimate_lambda_max(Multigrid& mg, MGLevel& L, int iters, hipStream_t s) 
 // Chebyshev smoothing of A x = b on level L (Saad, Iterative Methods, Alg.
 // 12.1, on the Jacobi-preconditioned system), `degree` steps, x = L.x;
 // zero_start skips the first residual product.  CG levels apply each step
-// inside the operator's kernels, in place; DG levels with the four-lane
-// kernel write x + d to L.t and swap L.t with L.x (the neighbour gathers of
-// other cells still read the old x), so L.x holds the iterate after every
-// step.
+// inside the operator's kernels, in place; scalar CG levels without Nitsche
+// rows run the steps as a ChebyRun (one kernel per step, the interface rows
+// of a step completed by the next step's kernel, one closing kernel); DG
+// levels with the four-lane kernel write x + d to L.t and swap L.t with L.x
+// (the neighbour gathers of other cells still read the old x), so L.x holds
+// the iterate after every step.
 template <typename T>
 void chebyshev(Multigrid& mg, MGLevel& L, const T* b, bool zero_start, hipStream_t s) {
   const int64_t n = L.op->m();
@@ -420,6 +423,19 @@ void chebyshev(Multigrid& mg, MGLevel& L, const T* b, bool zero_start, hipStream
   ep.b = b;
   ep.dinv = dinv;
   ep.d = d;
+  const bool use_run = !dg && cg_has_cheby_run(*L.op) && mg.opt.fuse_steps != 0;
+  ChebyRun<T> run;
+  if (use_run) {
+    run.b = b;
+    run.dinv = dinv;
+    run.x = L.x.as<T>();
+    run.d = d;
+    for (int k = 0; k < 2; ++k) {
+      run.part[k] = L.run_part[k].as<T>();
+      run.xi[k] = L.run_xi[k].as<T>();
+      run.di[k] = L.run_di[k].as<T>();
+    }
+  }
   for (int k = 0; k < mg.opt.smoother_degree; ++k) {
     T* x = L.x.as<T>();
     T c1 = T(0), c2;
@@ -435,7 +451,9 @@ void chebyshev(Multigrid& mg, MGLevel& L, const T* b, bool zero_start, hipStream
       c2 = T(2.0 * rho_new / delta);
       rho = rho_new;
     }
-    if (fused) {
+    if (use_run) {
+      launch_cg_cheby_step<T>(*L.op, run, c1, c2, k == 0, s);
+    } else if (fused) {
       ep.first = k == 0;
       ep.c1 = c1;
       ep.c2 = c2;
@@ -453,6 +471,7 @@ void chebyshev(Multigrid& mg, MGLevel& L, const T* b, bool zero_start, hipStream
       blas_cheby_step<T>(n, c1, c2, dinv, b, ax, d, x, k == 0, s);
     }
   }
+  if (use_run) launch_cg_cheby_close<T>(*L.op, run, s);
 }
 
 template <typename T>
@@ -556,6 +575,14 @@ std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions
     MGLevel& L = mg->levels[l];
     const int64_t m = L.op->m();
     for (DevBuf* b : {&L.inv_diag, &L.x, &L.b, &L.r, &L.t, &L.d}) b->alloc(m * w);
+    if (cg_has_cheby_run(*L.op)) {
+      const CGPlan& pl = L.op->cg;
+      for (int k = 0; k < 2; ++k) {
+        L.run_part[k].alloc(std::max<int64_t>(pl.n_ghost_total, 1) * w);
+        L.run_xi[k].alloc(std::max<int64_t>(pl.n_free - pl.n_int, 1) * w);
+        L.run_di[k].alloc(std::max<int64_t>(pl.n_free - pl.n_int, 1) * w);
+      }
+    }
     launch_op(*L.op, 1, L.inv_diag.p, nullptr, s);
     launch_invert_diagonal(prec, m, L.inv_diag.p, s);
     if (l > 0) {

@@ -30,6 +30,7 @@ struct MGLevel {
   std::unique_ptr<Operator> op;  // level operator in the V-cycle precision
   int degree = 1, kuhn_n = 0;
   DevBuf inv_diag, x, b, r, t, d;  // level vectors
+  DevBuf run_part[2], run_xi[2], run_di[2];  // ChebyRun buffers (scalar CG levels)
   Csr P, R;                        // prolongation from / restriction to the next coarser level
   double lambda_max = 0, lambda_min = 0;
 };
@@ -40,6 +41,7 @@ struct MGOptions {
   int eig_cg_iterations = 10;
   int level_precision = MF_F32;
   int max_coarse_dofs = 4096;
+  int fuse_steps = 1;  // scalar CG levels: Chebyshev steps without per-step fix-up launches
 };
 
 struct Multigrid {

@@ -71,6 +71,7 @@ struct DevBuf {
 // colourings, local index tables and geometry (all in internal cell order).
 struct CGPlan {
   int p = 0, n_loc = 0, n_q = 0, ncomp = 1;
+  int value_bytes = 8;                // 8 (FP64) or 4 (FP32): sets the default patch shape
   // scalar DoFs (per component): [0,n_int) patch-interior, [n_int,n_free)
   // patch-interface, [n_free,n_dofs) Dirichlet-constrained
   int64_t n_cells = 0, n_dofs = 0, n_free = 0, n_int = 0;
@@ -178,6 +179,43 @@ struct RowEpilogue {
 // A x with the epilogue (kind 1 or 2) applied row by row inside the kernels
 template <typename T>
 void launch_cg_epilogue(const Operator& op, const T* x, RowEpilogue<T> ep, hipStream_t s);
+
+// A run of Chebyshev steps on a scalar CG level without a fix-up launch per
+// step.  Step k's kernel leaves its interface rows pending: their partial
+// sums go to part[k % 2], and the next step's kernel applies the pending
+// update to every interface DoF it stages (each patch computes the same
+// value in the same order; the first patch of the DoF's partial list also
+// stores it to xi[k % 2] / di[k % 2] for the step after).  Dirichlet rows
+// take no part in the products of free rows, so their updates wait for the
+// closing kernel, which also completes the last step's interface rows.
+template <typename T>
+struct ChebyStep {
+  T c1, c2;
+  int first;
+};
+constexpr int kMaxChebySteps = 16;
+template <typename T>
+struct ChebyRun {
+  const T* b = nullptr;
+  const T* dinv = nullptr;
+  T* x = nullptr;  // iterate (interior rows updated in place by each step)
+  T* d = nullptr;  // direction
+  T* part[2] = {nullptr, nullptr};  // partial sums, fix-up order (n_ghost_total each)
+  T* xi[2] = {nullptr, nullptr};    // interface iterate / direction of pending steps (n_iface each)
+  T* di[2] = {nullptr, nullptr};
+  int n = 0;                         // steps issued so far
+  ChebyStep<T> steps[kMaxChebySteps];
+};
+// whether the operator's kernels take part in a ChebyRun (scalar CG, no
+// Nitsche boundary)
+bool cg_has_cheby_run(const Operator& op);
+// one Chebyshev step (c1, c2, first) of the run on x
+template <typename T>
+void launch_cg_cheby_step(const Operator& op, ChebyRun<T>& run, T c1, T c2, int first, hipStream_t s);
+// completes the run: pending interface rows of the last step, Dirichlet rows
+// of every step
+template <typename T>
+void launch_cg_cheby_close(const Operator& op, ChebyRun<T>& run, hipStream_t s);
 // the same for a scalar DG SIPG operator (four-lane kernel; ep.xout required
 // for kind 1); dg_has_epilogue says whether the operator's kernel has it
 bool dg_has_epilogue(const Operator& op);

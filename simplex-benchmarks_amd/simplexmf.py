@@ -139,7 +139,7 @@ def lib():
 class MGOptions(C.Structure):
     _fields_ = [("smoother_degree", C.c_int32), ("smoothing_range", C.c_double),
                 ("eig_cg_iterations", C.c_int32), ("level_precision", C.c_int32),
-                ("max_coarse_dofs", C.c_int32)]
+                ("max_coarse_dofs", C.c_int32), ("fuse_steps", C.c_int32)]
 
 
 MF_MAX_LEVELS = 16
@@ -471,7 +471,7 @@ class Multigrid:
     (MultigridPreconditioner of CG/cells_batched/application_ph.cc:1754-2026)."""
 
     def __init__(self, fine: Operator, smoother_degree=5, smoothing_range=20.0, eig_cg_iterations=10,
-                 level_precision="f32", max_coarse_dofs=4096):
+                 level_precision="f32", max_coarse_dofs=4096, fuse_steps=True):
         L = lib()
         opt = MGOptions()
         L.mf_mg_default_options(C.byref(opt))
@@ -480,6 +480,7 @@ class Multigrid:
         opt.eig_cg_iterations = eig_cg_iterations
         opt.level_precision = PRECISIONS[level_precision] if isinstance(level_precision, str) else level_precision
         opt.max_coarse_dofs = max_coarse_dofs
+        opt.fuse_steps = 1 if fuse_steps else 0
         self.level_dtype = NP_DTYPES[opt.level_precision]
         self.fine = fine
         h = C.c_void_p()

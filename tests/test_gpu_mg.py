@@ -95,6 +95,35 @@ def test_vcycle_deterministic_and_h_independent(gpu):
     assert max(its) <= min(its) + 4, its
 
 
+@pytest.mark.parametrize("degree,prec,smoother", [(2, "f32", 5), (2, "f64", 5), (1, "f32", 3), (3, "f64", 2)])
+def test_cheby_run_matches_per_step_fixup(gpu, degree, prec, smoother):
+    # The Chebyshev run (interface rows completed by the next step's kernel,
+    # Dirichlet rows by the closing kernel) against one operator application
+    # and fix-up per step: the same recurrences and sums in the same order, so
+    # the V-cycles agree to rounding and the solves take the same iterations.
+    smf = gpu
+    print(f"\ncase: Chebyshev run vs per-step fix-up, P{degree} {prec} degree {smoother}", flush=True)
+    fine = smf.Operator(smf.Mesh.kuhn_cube(12, 0.2, 5), degree)
+    mg_run = smf.Multigrid(fine, smoother_degree=smoother, level_precision=prec, fuse_steps=True)
+    mg_ref = smf.Multigrid(fine, smoother_degree=smoother, level_precision=prec, fuse_steps=False)
+    b = smf.DeviceVector.from_numpy(np.random.default_rng(2).uniform(-1, 1, fine.m()))
+    for _ in range(2):  # the run's buffers are reused from one V-cycle to the next
+        z1, z2 = fine.initialize_dof_vector(), fine.initialize_dof_vector()
+        mg_run.vmult(z1, b)
+        mg_ref.vmult(z2, b)
+        tol = 1e-5 if prec == "f32" else 1e-12
+        assert rel_err(z1.numpy(), z2.numpy()) < tol
+    z3 = fine.initialize_dof_vector()
+    mg_run.vmult(z3, b)
+    assert np.array_equal(z1.numpy(), z3.numpy())  # deterministic
+    x1, x2 = fine.initialize_dof_vector(), fine.initialize_dof_vector()
+    s1 = smf.solve_cg(fine, x1, b, mg_run, rel_tol=1e-10)
+    s2 = smf.solve_cg(fine, x2, b, mg_ref, rel_tol=1e-10)
+    assert s1["converged"] and s2["converged"]
+    assert abs(s1["iterations"] - s2["iterations"]) <= 1
+    assert rel_err(x1.numpy(), x2.numpy()) < 1e-8
+
+
 def test_rhs_against_oracle_quadrature(gpu, orc):
     smf = gpu
     print("\ncase: RHS sine P2 n=4", flush=True)
