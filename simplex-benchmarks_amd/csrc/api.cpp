@@ -47,6 +47,7 @@ void upload_cg(Operator& op) {
   op.d_ccol.upload(pl.ccol);
   op.d_fix_ptr.upload(pl.fix_ptr);
   op.d_pslot.upload(pl.pslot);
+  op.d_gfix.upload(pl.gfix);
   op.d_lidx.upload(pl.lidx);
   op.d_partial.alloc(std::max<int64_t>(pl.n_ghost_total, 1) * op.ncomp * op.wbytes());
   if (op.precision == MF_F64) {

@@ -30,4 +30,14 @@ template <typename T>
 void blas_cheby_zero(int64_t n, T c2, const T* dinv, const T* b, T* d, T* x, hipStream_t s);  // d = x = c2 Dinv b
 void blas_dense_gemv(int n, const double* M, const double* x, double* y, hipStream_t s);
 
+// Krylov scalars kept in device memory, so that a (flexible) CG iteration
+// synchronises with the host once (for its residual norm).  Slots of the
+// scalar block S:
+enum KrylovSlot { kS_RZ = 0, kS_PQ = 1, kS_RR = 2, kS_RZN = 3, kS_ZQ = 4, kS_ALPHA = 5, kS_BETA = 6, kS_N = 8 };
+// y = (sa * S[ia]) x + S[ib] y in FP64; an index < 0 reads as 1
+void blas_axpby_dev(int64_t n, const double* S, int ia, double sa, const double* x, int ib, double* y, hipStream_t s);
+// mode 0: S[ALPHA] = S[RZ] / S[PQ];  mode 1: S[BETA] = S[RZN] / S[RZ], S[RZ] = S[RZN];
+// mode 2 (flexible, Notay): S[BETA] = -S[ALPHA] S[ZQ] / S[RZ], S[RZ] = S[RZN]
+void blas_krylov_scalars(double* S, int mode, hipStream_t s);
+
 }  // namespace smf

@@ -47,6 +47,24 @@ __global__ void fill_kernel(int64_t n, T v, T* __restrict__ y) {
   if (i < n) y[i] = v;
 }
 
+// y = (sa S[ia]) x + S[ib] y (index < 0: 1), the arithmetic of axpby_kernel
+__global__ void axpby_dev_kernel(int64_t n, const double* __restrict__ S, int ia, double sa,
+                                 const double* __restrict__ x, int ib, double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double a = ia >= 0 ? sa * S[ia] : sa, b = ib >= 0 ? S[ib] : 1.0;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+__global__ void krylov_scalars_kernel(double* S, int mode) {
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    S[kS_ALPHA] = S[kS_RZ] / S[kS_PQ];
+  } else {
+    S[kS_BETA] = mode == 1 ? S[kS_RZN] / S[kS_RZ] : -S[kS_ALPHA] * S[kS_ZQ] / S[kS_RZ];
+    S[kS_RZ] = S[kS_RZN];
+  }
+}
+
 // partial[b] = sum over block b's grid-stride range of x*y (FP64)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void dot_partial_kernel(int64_t n, const T* __restrict__ x,
@@ -171,6 +189,12 @@ void blas_cheby_step(int64_t n, T c1, T c2, const T* dinv, const T* b, const T* 
 template <typename T>
 void blas_cheby_zero(int64_t n, T c2, const T* dinv, const T* b, T* d, T* x, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(cheby_zero_kernel<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, c2, dinv, b, d, x);
+}
+void blas_axpby_dev(int64_t n, const double* S, int ia, double sa, const double* x, int ib, double* y, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(axpby_dev_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, n, S, ia, sa, x, ib, y);
+}
+void blas_krylov_scalars(double* S, int mode, hipStream_t s) {
+  hipLaunchKernelGGL(krylov_scalars_kernel, dim3(1), dim3(64), 0, s, S, mode);
 }
 void blas_dense_gemv(int n, const double* M, const double* x, double* y, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(dense_gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, s, n, M, x, y);

@@ -152,7 +152,8 @@ template <int P, typename T, int MODE, bool ATOMIC, int NC>
 __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ own_start,
     const int32_t* __restrict__ ghost_start, const int32_t* __restrict__ ghost_idx,
-    const int32_t* __restrict__ pslot, const uint8_t* __restrict__ ccol, const uint16_t* __restrict__ lidx,
+    const int32_t* __restrict__ pslot, const int32_t* __restrict__ gfix, const uint8_t* __restrict__ ccol,
+    const uint16_t* __restrict__ lidx,
     int64_t ncells_total,
     const T* __restrict__ metric, const T* __restrict__ shape, const T* src,
     T* __restrict__ dst, T* __restrict__ partial, int64_t cstride, int64_t pstride, RowEpilogue<T> ep,
@@ -193,21 +194,24 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
       }
       if (pg.on) {
         // ghosts are interface DoFs whose previous step is still pending
-        constexpr int GB = 2;
+        constexpr int GB = 4;
         for (int j0 = 0; j0 < ng; j0 += GB * nt) {
-          int32_t gi[GB], sl[GB], k0[GB], k1[GB];
+          // ghost_idx, pslot and gfix give the partial list [k0, k1) and
+          // whether this patch is its first (two dependent round trips)
+          int32_t gi[GB], sl[GB], gf[GB], k0[GB], k1[GB];
           T xv[GB], dv[GB], bv[GB], iv[GB];
 #pragma unroll
           for (int u = 0; u < GB; ++u) {
             const int jj = min(j0 + u * nt + lane, ng - 1);
             gi[u] = ghost_idx[g0 + jj];
             sl[u] = pslot[g0 + jj];
+            gf[u] = gfix[g0 + jj];
           }
 #pragma unroll
           for (int u = 0; u < GB; ++u) {
             const int64_t t = gi[u] - pg.n_int;
-            k0[u] = pg.fix_ptr[t];
-            k1[u] = pg.fix_ptr[t + 1];
+            k0[u] = sl[u] - (gf[u] >> 16);
+            k1[u] = k0[u] + (gf[u] & 0xFFFF);
             xv[u] = pg.xs[t];
             dv[u] = pg.first ? T(0) : pg.ds[t];
             bv[u] = pg.b[gi[u]];
@@ -221,7 +225,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
             const int j = j0 + u * nt + lane;
             if (j < ng) {
               su[no + j] = xn;
-              if (sl[u] == k0[u]) {
+              if ((gf[u] >> 16) == 0) {
                 pg.xw[gi[u] - pg.n_int] = xn;
                 pg.dw[gi[u] - pg.n_int] = di;
               }
@@ -446,7 +450,7 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpi
     hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_patches), dim3(kWave * pl.waves), shmem, s,
                        op.d_cell_start.as<int32_t>(), op.d_own_start.as<int32_t>(),
                        op.d_ghost_start.as<int32_t>(), op.d_ghost_idx.as<int32_t>(), op.d_pslot.as<int32_t>(),
-                       op.d_ccol.as<uint8_t>(), op.d_lidx.as<uint16_t>(), pl.n_cells,
+                       op.d_gfix.as<int32_t>(), op.d_ccol.as<uint8_t>(), op.d_lidx.as<uint16_t>(), pl.n_cells,
                        op.d_metric.as<T>(), op.d_shape.as<T>(), src, dst,
                        partial_out ? partial_out : op.d_partial.as<T>(), pl.n_dofs,
                        std::max<int64_t>(pl.n_ghost_total, 1), ep, pg);

@@ -252,6 +252,13 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
     for (int64_t k = 0; k < ng; ++k) plan.fix_src[pos[plan.ghost_idx[k] - plan.n_int]++] = (int32_t)k;
     plan.pslot.resize(ng);
     for (int64_t k = 0; k < ng; ++k) plan.pslot[plan.fix_src[k]] = (int32_t)k;
+    plan.gfix.resize(ng);
+    for (int64_t g = 0; g < ng; ++g) {
+      const int64_t t = plan.ghost_idx[g] - plan.n_int;
+      const int32_t pos = plan.pslot[g] - plan.fix_ptr[t], len = plan.fix_ptr[t + 1] - plan.fix_ptr[t];
+      if (len > 0xFFFF) throw std::runtime_error("interface DoF shared by too many patches");
+      plan.gfix[g] = (pos << 16) | len;
+    }
   }
 
   // ---- geometry ----

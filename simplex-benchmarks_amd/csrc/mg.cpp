@@ -675,29 +675,37 @@ SolveStats solve_cg(const Operator& A, Multigrid* mg, double* x, const double* b
     st.converged = 1;
     return st;
   }
+  // alpha, beta and the dot products stay in device memory (S); the host
+  // reads |r|^2 once per iteration for the stopping test
+  DevBuf S;
+  S.alloc(kS_N * sizeof(double));
+  double* Sd = S.as<double>();
   precond(z.as<double>(), r.as<double>());
   SMF_HIP(hipMemcpyAsync(p.p, z.p, n * 8, hipMemcpyDeviceToDevice, s));
-  double rz = dot(r.as<double>(), z.as<double>());
+  blas_dot<double>(n, r.as<double>(), z.as<double>(), partial.as<double>(), Sd + kS_RZ, s);
   for (int it = 1; it <= max_iter; ++it) {
     launch_op(A, 0, q.p, p.p, s);
-    const double pq = dot(p.as<double>(), q.as<double>());
-    const double alpha = rz / pq;
-    blas_axpby<double>(n, alpha, p.as<double>(), 1.0, x, s);
-    blas_axpby<double>(n, -alpha, q.as<double>(), 1.0, r.as<double>(), s);
-    st.rn = std::sqrt(dot(r.as<double>(), r.as<double>()));
+    blas_dot<double>(n, p.as<double>(), q.as<double>(), partial.as<double>(), Sd + kS_PQ, s);
+    blas_krylov_scalars(Sd, 0, s);                                        // alpha = rz / pq
+    blas_axpby_dev(n, Sd, kS_ALPHA, 1.0, p.as<double>(), -1, x, s);               // x += alpha p
+    blas_axpby_dev(n, Sd, kS_ALPHA, -1.0, q.as<double>(), -1, r.as<double>(), s);  // r -= alpha q
+    blas_dot<double>(n, r.as<double>(), r.as<double>(), partial.as<double>(), Sd + kS_RR, s);
+    double rr;
+    SMF_HIP(hipMemcpyAsync(&rr, Sd + kS_RR, sizeof(double), hipMemcpyDeviceToHost, s));
+    SMF_HIP(hipStreamSynchronize(s));
+    st.rn = std::sqrt(rr);
     st.iterations = it;
     if (st.rn <= tol || !std::isfinite(st.rn)) {
       st.converged = st.rn <= tol;
       break;
     }
     precond(z.as<double>(), r.as<double>());
-    const double rz_new = dot(r.as<double>(), z.as<double>());
+    blas_dot<double>(n, r.as<double>(), z.as<double>(), partial.as<double>(), Sd + kS_RZN, s);
     // flexible CG (Notay 2000): beta = z_new.(r_new - r_old)/rz_old with
     // r_new - r_old = -alpha q
-    double beta = rz_new / rz;
-    if (flexible) beta = -alpha * dot(z.as<double>(), q.as<double>()) / rz;
-    blas_axpby<double>(n, 1.0, z.as<double>(), beta, p.as<double>(), s);
-    rz = rz_new;
+    if (flexible) blas_dot<double>(n, z.as<double>(), q.as<double>(), partial.as<double>(), Sd + kS_ZQ, s);
+    blas_krylov_scalars(Sd, flexible ? 2 : 1, s);
+    blas_axpby_dev(n, Sd, -1, 1.0, z.as<double>(), kS_BETA, p.as<double>(), s);  // p = z + beta p
   }
   SMF_HIP(hipStreamSynchronize(s));
   return st;

@@ -443,20 +443,26 @@ FcgStats solve_fcg_learned(const Operator& A, Multigrid& mg, LearnedSmoother* ls
     return st;
   }
   if (fo.force_fallback_at == 0) fall_back(0);
+  // alpha, beta and the dot products stay in device memory (S); the host
+  // reads |r|^2 once per iteration (the stopping test and stall detector)
+  // and, with it, the last learned step's omega
+  DevBuf S;
+  S.alloc(kS_N * sizeof(double));
+  double* Sd = S.as<double>();
   bool have_az = precond();
   copy_d(p.as<double>(), z.as<double>(), n, s);
   if (have_az) copy_d(q.as<double>(), az.as<double>(), n, s);
   else launch_op(A, 0, q.p, p.p, s);
-  double rz = host_dot(partial, scalar, n, r.as<double>(), z.as<double>(), s);
-  collect(0);
+  blas_dot<double>(n, r.as<double>(), z.as<double>(), partial.as<double>(), Sd + kS_RZ, s);
   double rprev = st.base.r0;
   int slow = 0;
   for (int it = 1; it <= max_iter; ++it) {
-    const double pq = host_dot(partial, scalar, n, p.as<double>(), q.as<double>(), s);
-    const double alpha = rz / pq;
-    blas_axpby<double>(n, alpha, p.as<double>(), 1.0, x, s);
-    blas_axpby<double>(n, -alpha, q.as<double>(), 1.0, r.as<double>(), s);
+    blas_dot<double>(n, p.as<double>(), q.as<double>(), partial.as<double>(), Sd + kS_PQ, s);
+    blas_krylov_scalars(Sd, 0, s);                                                 // alpha = rz / pq
+    blas_axpby_dev(n, Sd, kS_ALPHA, 1.0, p.as<double>(), -1, x, s);               // x += alpha p
+    blas_axpby_dev(n, Sd, kS_ALPHA, -1.0, q.as<double>(), -1, r.as<double>(), s);  // r -= alpha q
     st.base.rn = std::sqrt(host_dot(partial, scalar, n, r.as<double>(), r.as<double>(), s));
+    collect(it - 1);  // the learned step of the previous preconditioner application
     st.base.iterations = it;
     if (st.base.rn <= tol || !std::isfinite(st.base.rn)) {
       st.base.converged = st.base.rn <= tol;
@@ -467,17 +473,16 @@ FcgStats solve_fcg_learned(const Operator& A, Multigrid& mg, LearnedSmoother* ls
     if (slow >= 2 || it == fo.force_fallback_at) fall_back(it);
     rprev = st.base.rn;
     have_az = precond();
-    double d[2];
-    host_dot2(partial, scalar, n, r.as<double>(), z.as<double>(), z.as<double>(), q.as<double>(), d, s);
-    collect(it);
+    blas_dot<double>(n, r.as<double>(), z.as<double>(), partial.as<double>(), Sd + kS_RZN, s);
+    blas_dot<double>(n, z.as<double>(), q.as<double>(), partial.as<double>(), Sd + kS_ZQ, s);
     // flexible CG (Notay 2000): beta = -alpha z.q / rz
-    const double beta = -alpha * d[1] / rz;
-    blas_axpby<double>(n, 1.0, z.as<double>(), beta, p.as<double>(), s);
-    if (have_az) blas_axpby<double>(n, 1.0, az.as<double>(), beta, q.as<double>(), s);
+    blas_krylov_scalars(Sd, 2, s);
+    blas_axpby_dev(n, Sd, -1, 1.0, z.as<double>(), kS_BETA, p.as<double>(), s);  // p = z + beta p
+    if (have_az) blas_axpby_dev(n, Sd, -1, 1.0, az.as<double>(), kS_BETA, q.as<double>(), s);
     else launch_op(A, 0, q.p, p.p, s);
-    rz = d[0];
   }
   SMF_HIP(hipStreamSynchronize(s));
+  collect(st.base.iterations);
   st.mean_omega = st.learned_steps ? omega_sum / st.learned_steps : 0.0;
   return st;
 }
