@@ -28,7 +28,8 @@ void blas_cheby_step(int64_t n, T c1, T c2, const T* dinv, const T* b, const T* 
                      hipStream_t s);
 template <typename T>
 void blas_cheby_zero(int64_t n, T c2, const T* dinv, const T* b, T* d, T* x, hipStream_t s);  // d = x = c2 Dinv b
-void blas_dense_gemv(int n, const double* M, const double* x, double* y, hipStream_t s);
+// y[0..nf) = M x[0..nf) for a dense row-major nf x nf matrix, y[nf..n) = x[nf..n)
+void blas_dense_gemv(int nf, int n, const double* M, const double* x, double* y, hipStream_t s);
 
 // Krylov scalars kept in device memory, so that a (flexible) CG iteration
 // synchronises with the host once (for its residual norm).  Slots of the

@@ -134,13 +134,26 @@ __global__ void cheby_zero_kernel(int64_t n, T c2, const T* __restrict__ dinv, c
 
 // y = M x for a dense row-major n x n matrix (FP64): one wave per row, lanes
 // stride the columns (coalesced), fixed-order butterfly sum (deterministic)
-__global__ __launch_bounds__(256) void dense_gemv_kernel(int n, const double* __restrict__ M,
+__global__ __launch_bounds__(256) void dense_gemv_kernel(int nf, int n, const double* __restrict__ M,
                                                          const double* __restrict__ x, double* __restrict__ y) {
   const int r = blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
   if (r >= n) return;  // whole waves exit together
-  const double* row = M + (int64_t)r * n;
-  double s = 0.0;
-  for (int c = lane; c < n; c += 64) s += row[c] * x[c];
+  if (r >= nf) {       // identity rows (Dirichlet DoFs of the coarsest level)
+    if (lane == 0) y[r] = x[r];
+    return;
+  }
+  const double* row = M + (int64_t)r * nf;
+  // four independent accumulators keep four loads in flight per lane
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int c = lane;
+  for (; c + 192 < nf; c += 256) {
+    s0 += row[c] * x[c];
+    s1 += row[c + 64] * x[c + 64];
+    s2 += row[c + 128] * x[c + 128];
+    s3 += row[c + 192] * x[c + 192];
+  }
+  for (; c < nf; c += 64) s0 += row[c] * x[c];
+  double s = (s0 + s1) + (s2 + s3);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if (lane == 0) y[r] = s;
 }
@@ -196,8 +209,8 @@ void blas_axpby_dev(int64_t n, const double* S, int ia, double sa, const double*
 void blas_krylov_scalars(double* S, int mode, hipStream_t s) {
   hipLaunchKernelGGL(krylov_scalars_kernel, dim3(1), dim3(64), 0, s, S, mode);
 }
-void blas_dense_gemv(int n, const double* M, const double* x, double* y, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(dense_gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, s, n, M, x, y);
+void blas_dense_gemv(int nf, int n, const double* M, const double* x, double* y, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(dense_gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, s, nf, n, M, x, y);
 }
 int blas_reduction_parts() { return kRedBlocks; }
 

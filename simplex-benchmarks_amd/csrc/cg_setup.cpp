@@ -44,12 +44,18 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   // (multigrid levels) still fill the 256 CUs
   const int per_wave = opt.cells_per_wave > 0 ? opt.cells_per_wave : (p == 1 ? 1024 : (p == 2 ? 512 : 128));
   int patch = opt.patch_cells;
+  int waves_default = 0;
   if (patch <= 0) {
-    patch = (plan.ncomp == 3 ? 1 : (p == 2 && plan.value_bytes == 8 ? 2 : 4)) * per_wave;
+    waves_default = plan.ncomp == 3 ? 1 : (p == 2 && plan.value_bytes == 8 ? 2 : 4);
+    patch = waves_default * per_wave;
     while (patch > 64 && mesh.nc / patch < 1024) patch /= 2;
   }
   if (patch > 4096) throw std::runtime_error("patch_cells must be <= 4096");
   plan.waves = std::min(4, std::max(1, (patch + per_wave - 1) / per_wave));
+  // a halved default patch keeps up to two waves of at least 256 cells each
+  // (48^3 multigrid levels: P2 FP32 26.4 -> 25.6 us, P1 15.5 -> 15.0 us)
+  if (waves_default > 0 && opt.cells_per_wave <= 0 && patch < waves_default * per_wave)
+    plan.waves = std::max(1, std::min(std::min(waves_default, 2), patch / 256));
   plan.max_patch_cells = 0;
 
   // ---- entities and raw DoFs (entity-based numbering) ----

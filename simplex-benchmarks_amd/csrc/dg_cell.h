@@ -108,6 +108,18 @@ constexpr DGCellTab<P, MASS> make_cell_tab() {
   return t;
 }
 
+// Face-3 mass matrices of the lane frame on the reference triangle (face
+// weights, sum 1/2): Mvv[k][l] = sum_q w_q v_k v_l over the P_p face nodes,
+// Mvg[k][j] = sum_q w_q v_k psi_j with the P_{p-1} gradient nodes.  The face
+// is affine, so the SIPG face integrals are these matrices times the face
+// Jacobian: exact, like the quadrature they replace.
+template <int P>
+struct DGFaceMass {
+  static constexpr int NFN = (P + 1) * (P + 2) / 2, NFG = P == 1 ? 1 : P * (P + 1) / 2;
+  double vv[NFN][NFN];
+  double vg[NFN][NFG];
+};
+
 template <int P>
 struct DGFaceTab {
   static constexpr int NL = nloc(P), NQF = FaceRule<P>::NQF;
@@ -182,6 +194,25 @@ constexpr DGFaceTab<P> make_face_tab() {
     }
   }
   return t;
+}
+
+template <int P>
+constexpr DGFaceMass<P> make_face_mass() {
+  constexpr DGFaceTab<P> FT = make_face_tab<P>();
+  DGFaceMass<P> m{};
+  for (int k = 0; k < DGFaceMass<P>::NFN; ++k) {
+    for (int l = 0; l < DGFaceMass<P>::NFN; ++l) {
+      double a = 0;
+      for (int q = 0; q < DGFaceTab<P>::NQF; ++q) a += FT.w[q] * FT.v[3][q][k] * FT.v[3][q][l];
+      m.vv[k][l] = a;
+    }
+    for (int g = 0; g < DGFaceMass<P>::NFG; ++g) {
+      double a = 0;
+      for (int q = 0; q < DGFaceTab<P>::NQF; ++q) a += FT.w[q] * FT.v[3][q][k] * FT.psi[3][q][g];
+      m.vg[k][g] = a;
+    }
+  }
+  return m;
 }
 
 // outward reference normals of the faces (face f opposite vertex f), unnormalised
@@ -559,6 +590,8 @@ SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* _
       gn[j] = sx * nJ[0] + sy * nJ[1] + sz * nJ[2];
     }
     T gb[NFG];
+#pragma unroll
+    for (int j = 0; j < NFG; ++j) gb[j] = T(0);
     if (interior) {
       T bb[4];
       {
@@ -584,32 +617,32 @@ SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* _
         gb[j] = sx * nb[0] + sy * nb[1] + sz * nb[2];
       }
     }
+    // SIPG face terms in matrix form: jump J of the values at the face nodes
+    // (mirror u+ = -u- on the boundary), mean A of the normal derivatives at
+    // the gradient nodes (d_n u+ = d_n u- on the boundary); then
+    //   y_face += |F| (sigma Mvv J - Mvg A),  tgj = -1/2 |F| Mvg^T J
+    constexpr DGFaceMass<P> FM = make_face_mass<P>();
+    T J[NFN], A[NFG];
+#pragma unroll
+    for (int k = 0; k < NFN; ++k) J[k] = interior ? u[FT.fn[3][k]] - ub[FT.fn[3][k]] : T(2) * u[FT.fn[3][k]];
+#pragma unroll
+    for (int j = 0; j < NFG; ++j) A[j] = interior ? T(0.5) * (gn[j] + gb[j]) : gn[j];
+#pragma unroll
+    for (int k = 0; k < NFN; ++k) {
+      T sj = T(0), sa = T(0);
+#pragma unroll
+      for (int l = 0; l < NFN; ++l) sj += T(FM.vv[k][l]) * J[l];
+#pragma unroll
+      for (int j = 0; j < NFG; ++j) sa += T(FM.vg[k][j]) * A[j];
+      y[FT.fn[3][k]] += area2 * (sigma * sj - sa);
+    }
     T tgj[NFG];
 #pragma unroll
-    for (int j = 0; j < NFG; ++j) tgj[j] = T(0);
+    for (int j = 0; j < NFG; ++j) {
+      T sj = T(0);
 #pragma unroll
-    for (int q = 0; q < NQF; ++q) {
-      T uv = T(0), dn = T(0);
-#pragma unroll
-      for (int kk = 0; kk < NFN; ++kk) uv += T(FT.v[3][q][kk]) * u[FT.fn[3][kk]];
-#pragma unroll
-      for (int j = 0; j < NFG; ++j) dn += T(FT.psi[3][q][j]) * gn[j];
-      T up = -uv, dnp = dn;
-      if (interior) {
-        up = T(0);
-        dnp = T(0);
-#pragma unroll
-        for (int kk = 0; kk < NFN; ++kk) up += T(FT.v[3][q][kk]) * ub[FT.fn[3][kk]];
-#pragma unroll
-        for (int j = 0; j < NFG; ++j) dnp += T(FT.psi[3][q][j]) * gb[j];
-      }
-      const T jxw = T(FT.w[q]) * area2;
-      const T jump = uv - up, avg = T(0.5) * (dn + dnp);
-      const T tv = (sigma * jump - avg) * jxw, tg = T(-0.5) * jump * jxw;
-#pragma unroll
-      for (int kk = 0; kk < NFN; ++kk) y[FT.fn[3][kk]] += T(FT.v[3][q][kk]) * tv;
-#pragma unroll
-      for (int j = 0; j < NFG; ++j) tgj[j] += T(FT.psi[3][q][j]) * tg;
+      for (int k = 0; k < NFN; ++k) sj += T(FM.vg[k][j]) * J[k];
+      tgj[j] = T(-0.5) * area2 * sj;
     }
 #pragma unroll
     for (int j = 0; j < NFG; ++j) {

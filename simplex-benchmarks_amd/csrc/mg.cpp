@@ -483,7 +483,8 @@ void vcycle(Multigrid& mg, int l, hipStream_t s) {
   if (l == 0) {
     // FP64 dense coarse solve
     blas_convert<double, T>(n, b, mg.d_cb.as<double>(), s);
-    blas_dense_gemv(mg.n_coarse, mg.d_coarse_inv.as<double>(), mg.d_cb.as<double>(), mg.d_cx.as<double>(), s);
+    blas_dense_gemv(mg.n_coarse_free, mg.n_coarse, mg.d_coarse_inv.as<double>(), mg.d_cb.as<double>(),
+                    mg.d_cx.as<double>(), s);
     blas_convert<T, double>(n, mg.d_cx.as<double>(), x, s);
     return;
   }
@@ -611,6 +612,16 @@ std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions
     inv = dense_inverse(c0);
   }
   mg->n_coarse = (int)c0.m();
+  // strong Dirichlet rows make A = [[A_ff, 0], [0, I]] (free DoFs first), so
+  // only the leading block of the inverse is kept
+  const int64_t nf = !c0.is_dg() && !c0.cg.nitsche ? c0.cg.n_free : c0.m();
+  mg->n_coarse_free = (int)nf;
+  if (nf < (int64_t)mg->n_coarse) {
+    const int64_t n = mg->n_coarse;
+    for (int64_t i = 0; i < nf; ++i)
+      for (int64_t j = 0; j < nf; ++j) inv[i * nf + j] = inv[i * n + j];
+    inv.resize(nf * nf);
+  }
   mg->d_coarse_inv.upload(inv);
   mg->d_cb.alloc(mg->n_coarse * sizeof(double));
   mg->d_cx.alloc(mg->n_coarse * sizeof(double));

@@ -50,7 +50,7 @@ struct Multigrid {
   int64_t n_fine = 0;
   std::vector<MGLevel> levels;  // [0] = coarsest
   // coarse solver: dense FP64 inverse of the coarsest operator
-  int n_coarse = 0;
+  int n_coarse = 0, n_coarse_free = 0;  // dense block: the leading n_coarse_free rows (the rest: identity)
   DevBuf d_coarse_inv, d_cb, d_cx;
   DevBuf d_partial, d_scalar;  // reductions
   int64_t vcycles = 0;
