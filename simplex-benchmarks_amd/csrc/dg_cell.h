@@ -196,6 +196,23 @@ constexpr DGFaceTab<P> make_face_tab() {
   return t;
 }
 
+// Every face lists its nodes in the same pattern (face vertices ascending,
+// then edge nodes) and is parametrised from its lowest vertex, so the face
+// node values and gradient-node functions at the face points are those of
+// face 3: one set of face mass matrices serves all four faces.
+template <int P>
+constexpr bool faces_share_tables() {
+  constexpr DGFaceTab<P> FT = make_face_tab<P>();
+  for (int f = 0; f < 3; ++f)
+    for (int q = 0; q < DGFaceTab<P>::NQF; ++q) {
+      for (int k = 0; k < DGFaceTab<P>::NFN; ++k)
+        if (FT.v[f][q][k] != FT.v[3][q][k]) return false;
+      for (int j = 0; j < DGFaceTab<P>::NFG; ++j)
+        if (FT.psi[f][q][j] != FT.psi[3][q][j]) return false;
+    }
+  return true;
+}
+
 template <int P>
 constexpr DGFaceMass<P> make_face_mass() {
   constexpr DGFaceTab<P> FT = make_face_tab<P>();
@@ -334,6 +351,10 @@ SMF_HDI void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, co
         }
         // neighbour, renumbered so that the shared face is its face 3
         T ub[NL], gb[NFG];
+#pragma unroll
+        for (int i = 0; i < NL; ++i) ub[i] = T(0);
+#pragma unroll
+        for (int j = 0; j < NFG; ++j) gb[j] = T(0);
         if (interior) {
           const int fb = (nf >> (8 * f)) & 3;
           const int64_t nbase = ((int64_t)kb * NC + c) * NL;
@@ -366,32 +387,32 @@ SMF_HDI void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, co
             gb[j] = sx * nb[0] + sy * nb[1] + sz * nb[2];
           }
         }
+        // face terms in matrix form (as in apply_cell_lane): own face nodes
+        // fn[f], the neighbour's fn[3] (same physical nodes, same order)
+        static_assert(faces_share_tables<P>(), "face tables differ between faces");
+        constexpr DGFaceMass<P> FM = make_face_mass<P>();
+        T J[NFN], A[NFG];
+#pragma unroll
+        for (int k = 0; k < NFN; ++k)
+          J[k] = interior ? u[FT.fn[f][k]] - ub[FT.fn[3][k]] : T(2) * u[FT.fn[f][k]];
+#pragma unroll
+        for (int j = 0; j < NFG; ++j) A[j] = interior ? T(0.5) * (gn[j] + gb[j]) : gn[j];
+#pragma unroll
+        for (int k = 0; k < NFN; ++k) {
+          T sj = T(0), sa = T(0);
+#pragma unroll
+          for (int l = 0; l < NFN; ++l) sj += T(FM.vv[k][l]) * J[l];
+#pragma unroll
+          for (int j = 0; j < NFG; ++j) sa += T(FM.vg[k][j]) * A[j];
+          y[FT.fn[f][k]] += area2 * (sigma * sj - sa);
+        }
         T tgj[NFG];
 #pragma unroll
-        for (int j = 0; j < NFG; ++j) tgj[j] = T(0);
+        for (int j = 0; j < NFG; ++j) {
+          T sj = T(0);
 #pragma unroll
-        for (int q = 0; q < NQF; ++q) {
-          T uv = T(0), dn = T(0);
-#pragma unroll
-          for (int kk = 0; kk < NFN; ++kk) uv += T(FT.v[f][q][kk]) * u[FT.fn[f][kk]];
-#pragma unroll
-          for (int j = 0; j < NFG; ++j) dn += T(FT.psi[f][q][j]) * gn[j];
-          T up = -uv, dnp = dn;
-          if (interior) {
-            up = T(0);
-            dnp = T(0);
-#pragma unroll
-            for (int kk = 0; kk < NFN; ++kk) up += T(FT.v[3][q][kk]) * ub[FT.fn[3][kk]];
-#pragma unroll
-            for (int j = 0; j < NFG; ++j) dnp += T(FT.psi[3][q][j]) * gb[j];
-          }
-          const T jxw = T(FT.w[q]) * area2;
-          const T jump = uv - up, avg = T(0.5) * (dn + dnp);
-          const T tv = (sigma * jump - avg) * jxw, tg = T(-0.5) * jump * jxw;
-#pragma unroll
-          for (int kk = 0; kk < NFN; ++kk) y[FT.fn[f][kk]] += T(FT.v[f][q][kk]) * tv;
-#pragma unroll
-          for (int j = 0; j < NFG; ++j) tgj[j] += T(FT.psi[f][q][j]) * tg;
+          for (int k = 0; k < NFN; ++k) sj += T(FM.vg[k][j]) * J[k];
+          tgj[j] = T(-0.5) * area2 * sj;
         }
         // gradient test: y_i += sum_j tgj[j] * grad phi_i(y_j) . nJ
 #pragma unroll
