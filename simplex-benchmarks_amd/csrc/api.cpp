@@ -63,6 +63,11 @@ void upload_cg(Operator& op) {
     op.d_bnd_col.upload(pl.bnd_col);
     if (op.precision == MF_F64) op.d_bnd_val.upload(pl.bnd_val);
     else op.d_bnd_val.upload(convert<float>(pl.bnd_val));
+    op.d_bcell.upload(pl.bcell);
+    if (!pl.bmat.empty()) {
+      if (op.precision == MF_F64) op.d_bmat.upload(pl.bmat);
+      else op.d_bmat.upload(convert<float>(pl.bmat));
+    }
     // dense boundary row sums for the fused epilogues: rows without boundary
     // entries stay zero
     op.d_bnd_dense.alloc(std::max<int64_t>(pl.n_dofs, 1) * op.wbytes());

@@ -12,7 +12,7 @@ namespace smf {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kRedBlocks = 256;
+constexpr int kRedBlocks = 1024;
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
@@ -71,10 +71,19 @@ __global__ __launch_bounds__(kBlock) void dot_partial_kernel(int64_t n, const T*
                                                              const T* __restrict__ y,
                                                              double* __restrict__ partial) {
   __shared__ double s[kBlock];
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    acc += (double)x[i] * (double)y[i];
-  s[threadIdx.x] = acc;
+  // four accumulators over strides i, i + g, i + 2g, i + 3g keep four loads
+  // of each vector in flight; fixed grid and order: deterministic
+  const int64_t g = (int64_t)gridDim.x * kBlock;
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (; i + 3 * g < n; i += 4 * g) {
+    a0 += (double)x[i] * (double)y[i];
+    a1 += (double)x[i + g] * (double)y[i + g];
+    a2 += (double)x[i + 2 * g] * (double)y[i + 2 * g];
+    a3 += (double)x[i + 3 * g] * (double)y[i + 3 * g];
+  }
+  for (; i < n; i += g) a0 += (double)x[i] * (double)y[i];
+  s[threadIdx.x] = (a0 + a1) + (a2 + a3);
   __syncthreads();
   for (int w = kBlock / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];

@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     int64_t ncells_total,
     const T* __restrict__ metric, const T* __restrict__ shape, const T* src,
     T* __restrict__ dst, T* __restrict__ partial, int64_t cstride, int64_t pstride, RowEpilogue<T> ep,
-    Pending<T> pg) {
+    Pending<T> pg, const int32_t* __restrict__ bcell, const T* __restrict__ bmat) {
   constexpr int NL = CGShape<P>::NL, NQ = CGShape<P>::NQ;
   extern __shared__ unsigned char smem_raw[];
   T* sbase = reinterpret_cast<T*>(smem_raw);
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
   // per-round cell data, loaded one round ahead (software pipelining)
   uint16_t li_n[NL];
   T m_n[6];
-  int col_n;
+  int col_n, bc_n = -1;
   auto load_round = [&](int r0) {
     const int64_t cell = c0 + min(r0 + lane, nc - 1);
 #pragma unroll
@@ -261,6 +261,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
 #pragma unroll
     for (int t = 0; t < 6; ++t) m_n[t] = metric[t * ncells_total + cell];
     col_n = ccol[cell];
+    if (bcell) bc_n = bcell[cell];
   };
   if (nc > 0) load_round(0);
   for (int r0 = 0; r0 < nc; r0 += nt) {
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
 #pragma unroll
     for (int i = 0; i < NL; ++i) li[i] = min((int)li_n[i], nloc);
     const T m0 = m_n[0], m1 = m_n[1], m2 = m_n[2], m3 = m_n[3], m4 = m_n[4], m5 = m_n[5];
-    const int mycol = col_n;
+    const int mycol = col_n, mybc = bc_n;
     if (r0 + nt < nc) load_round(r0 + nt);
 #pragma unroll 1
     for (int cc = 0; cc < NC; ++cc) {
@@ -279,6 +280,18 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     T y[NL];
 #pragma unroll
     for (int i = 0; i < NL; ++i) y[i] = T(0);
+    // Nitsche boundary terms of the cell (Chebyshev runs of Nitsche levels)
+    auto add_boundary = [&](const T* uu) {
+      if (mybc < 0) return;
+      const T* B = bmat + (int64_t)mybc * NL * NL;
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        T sb = T(0);
+#pragma unroll
+        for (int j = 0; j < NL; ++j) sb += B[i * NL + j] * uu[j];
+        y[i] += sb;
+      }
+    };
     // P1/P2: compile-time tables (literal constants after unrolling); P3
     // has too many distinct values for scalar registers and reads the table
     // from memory one quadrature point at a time.
@@ -287,6 +300,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
 #pragma unroll
       for (int i = 0; i < NL; ++i) u[i] = su[li[i]];
       p2_cell_apply<T>(u, m0, m1, m2, m3, m4, m5, y);
+      add_boundary(u);
     } else if constexpr (P <= 2) {
       constexpr tab::CellGrad<P> TB = tab::make_cell_grad<P>();
       if (MODE == 0) {
@@ -310,6 +324,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
           for (int i = 0; i < NL; ++i)
             y[i] += T(TB.g[q][i][0]) * tx + T(TB.g[q][i][1]) * ty + T(TB.g[q][i][2]) * tz;
         }
+        add_boundary(u);
       } else {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
@@ -345,6 +360,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
 #pragma unroll
           for (int i = 0; i < NL; ++i) y[i] += G[i * 3 + 0] * tx + G[i * 3 + 1] * ty + G[i * 3 + 2] * tz;
         }
+        add_boundary(u);
       } else {
 #pragma unroll 1
         for (int q = 0; q < NQ; ++q) {
@@ -436,7 +452,7 @@ __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* _
 // interface rows of the previous step, no fix-up launch)
 template <int P, typename T, int MODE, int NC>
 void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpilogue<T> ep = {},
-                 Pending<T> pg = {}, T* partial_out = nullptr, bool fixup = true) {
+                 Pending<T> pg = {}, T* partial_out = nullptr, bool fixup = true, bool cell_boundary = false) {
   const CGPlan& pl = op.cg;
   const size_t shmem = NC * 2 * ((size_t)pl.max_local + 1) * sizeof(T);
   const bool atomic = op.opt.accumulate == 1 || (op.opt.accumulate == 0 && sizeof(T) == 8);
@@ -453,7 +469,9 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpi
                        op.d_gfix.as<int32_t>(), op.d_ccol.as<uint8_t>(), op.d_lidx.as<uint16_t>(), pl.n_cells,
                        op.d_metric.as<T>(), op.d_shape.as<T>(), src, dst,
                        partial_out ? partial_out : op.d_partial.as<T>(), pl.n_dofs,
-                       std::max<int64_t>(pl.n_ghost_total, 1), ep, pg);
+                       std::max<int64_t>(pl.n_ghost_total, 1), ep, pg,
+                       cell_boundary ? op.d_bcell.as<int32_t>() : nullptr,
+                       cell_boundary ? op.d_bmat.as<T>() : nullptr);
   if (!fixup) {
     SMF_HIP(hipGetLastError());
     return;
@@ -631,11 +649,11 @@ Pending<T> pending_of(const Operator& op, const ChebyRun<T>& run, int k) {
 
 }  // namespace
 
-bool cg_has_cheby_run(const Operator& op) { return !op.is_dg() && op.ncomp == 1 && !op.cg.nitsche; }
+bool cg_has_cheby_run(const Operator& op) { return !op.is_dg() && op.ncomp == 1; }
 
 template <typename T>
 void launch_cg_cheby_step(const Operator& op, ChebyRun<T>& run, T c1, T c2, int first, hipStream_t s) {
-  if (!cg_has_cheby_run(op)) throw std::runtime_error("Chebyshev run: scalar CG operator without Nitsche rows");
+  if (!cg_has_cheby_run(op)) throw std::runtime_error("Chebyshev run: scalar CG operator");
   if ((op.precision == MF_F64) != (sizeof(T) == 8)) throw std::runtime_error("Chebyshev run: precision mismatch");
   const int k = run.n;
   if (k >= kMaxChebySteps) throw std::runtime_error("Chebyshev run: too many steps");
@@ -654,10 +672,13 @@ void launch_cg_cheby_step(const Operator& op, ChebyRun<T>& run, T c1, T c2, int 
     pg.xw = run.xi[k % 2];
     pg.dw = run.di[k % 2];
   }
+  // Nitsche rows: the boundary terms are added per cell inside the kernel,
+  // which stages the pending interface values they read
+  const bool nb = op.cg.nitsche;
   switch (op.cg.p) {
-    case 1: launch_cg_t<1, T, 0, 1>(op, nullptr, run.x, s, ep, pg, run.part[k % 2], false); break;
-    case 2: launch_cg_t<2, T, 0, 1>(op, nullptr, run.x, s, ep, pg, run.part[k % 2], false); break;
-    case 3: launch_cg_t<3, T, 0, 1>(op, nullptr, run.x, s, ep, pg, run.part[k % 2], false); break;
+    case 1: launch_cg_t<1, T, 0, 1>(op, nullptr, run.x, s, ep, pg, run.part[k % 2], false, nb); break;
+    case 2: launch_cg_t<2, T, 0, 1>(op, nullptr, run.x, s, ep, pg, run.part[k % 2], false, nb); break;
+    case 3: launch_cg_t<3, T, 0, 1>(op, nullptr, run.x, s, ep, pg, run.part[k % 2], false, nb); break;
     default: throw std::runtime_error("unsupported CG degree");
   }
   run.steps[k] = ChebyStep<T>{c1, c2, first};

@@ -337,6 +337,8 @@ void build_nitsche_boundary(const Mesh& mesh, CGPlan& plan, int penalty_kind) {
   };
   std::vector<Trip> trips;
   std::vector<double> val(nl), grad(3 * nl), dn(nl);
+  plan.bcell.assign(plan.n_cells, -1);
+  plan.bmat.clear();
   for (int64_t f = 0; f < fi.n_faces; ++f) {
     if (fi.face_cell[f][1] >= 0) continue;
     const int64_t c = fi.face_cell[f][0];
@@ -374,6 +376,12 @@ void build_nitsche_boundary(const Mesh& mesh, CGPlan& plan, int penalty_kind) {
     for (int i = 0; i < nl; ++i)
       for (int j = 0; j < nl; ++j)
         trips.push_back({(int32_t)plan.cell_dofs[k * nl + i], (int32_t)plan.cell_dofs[k * nl + j], B[i * nl + j]});
+    if (plan.bcell[k] < 0) {
+      plan.bcell[k] = (int32_t)(plan.bmat.size() / ((size_t)nl * nl));
+      plan.bmat.resize(plan.bmat.size() + (size_t)nl * nl, 0.0);
+    }
+    double* Bk = &plan.bmat[(size_t)plan.bcell[k] * nl * nl];
+    for (int e = 0; e < nl * nl; ++e) Bk[e] += B[e];
   }
   // stable: contributions to one entry are summed in boundary-face order
   std::stable_sort(trips.begin(), trips.end(),

@@ -402,9 +402,10 @@ double estimate_lambda_max(Multigrid& mg, MGLevel& L, int iters, hipStream_t s) 
 // Chebyshev smoothing of A x = b on level L (Saad, Iterative Methods, Alg.
 // 12.1, on the Jacobi-preconditioned system), `degree` steps, x = L.x;
 // zero_start skips the first residual product.  CG levels apply each step
-// inside the operator's kernels, in place; scalar CG levels without Nitsche
-// rows run the steps as a ChebyRun (one kernel per step, the interface rows
-// of a step completed by the next step's kernel, one closing kernel); DG
+// inside the operator's kernels, in place; scalar CG levels run the steps
+// as a ChebyRun (one kernel per step, the interface rows of a step completed
+// by the next step's kernel, one closing kernel; Nitsche boundary terms added
+// per cell inside the kernel); DG
 // levels with the four-lane kernel write x + d to L.t and swap L.t with L.x
 // (the neighbour gathers of other cells still read the old x), so L.x holds
 // the iterate after every step.

@@ -102,6 +102,11 @@ struct CGPlan {
   bool nitsche = false;
   std::vector<int32_t> bnd_rows, bnd_ptr, bnd_col;
   std::vector<double> bnd_val;
+  // the same boundary terms per cell (Chebyshev runs add them inside the
+  // cell kernel): bcell[k] = index of cell k's dense n_loc x n_loc matrix
+  // in bmat (row-major, y_i += sum_j B_ij u_j), -1 without boundary faces
+  std::vector<int32_t> bcell;
+  std::vector<double> bmat;
 };
 
 // Host-side plan of a discontinuous-Galerkin (SIPG) operator.  Cell local
@@ -151,7 +156,7 @@ struct Operator {
   // device data
   DevBuf d_cell_start, d_own_start, d_ghost_start, d_ghost_idx, d_ccol, d_fix_ptr,
       d_pslot, d_gfix, d_lidx, d_metric, d_shape, d_partial;
-  DevBuf d_bnd_rows, d_bnd_ptr, d_bnd_col, d_bnd_val, d_bnd_dense;
+  DevBuf d_bnd_rows, d_bnd_ptr, d_bnd_col, d_bnd_val, d_bnd_dense, d_bcell, d_bmat;
   DevBuf d_dg_geo, d_dg_nbr, d_dg_nface, d_dg_cell_tab, d_dg_face_tab;
   bool on_device = false;
   int64_t m() const { return is_dg() ? dg.n_dofs : cg.n_dofs * ncomp; }
@@ -208,8 +213,8 @@ struct ChebyRun {
   int n = 0;                         // steps issued so far
   ChebyStep<T> steps[kMaxChebySteps];
 };
-// whether the operator's kernels take part in a ChebyRun (scalar CG, no
-// Nitsche boundary)
+// whether the operator's kernels take part in a ChebyRun (scalar CG; a
+// Nitsche boundary is applied per cell inside the kernel during a run)
 bool cg_has_cheby_run(const Operator& op);
 // one Chebyshev step (c1, c2, first) of the run on x
 template <typename T>
