@@ -478,12 +478,39 @@ constexpr DGCellShare<P, MASS> make_cell_share() {
   return r;
 }
 
+// Slot of cell node i in lane g's row (lane g stores the row of its frame
+// node j, cell node perm[g][j], at j): the inverse permutations.
+template <int P>
+struct LaneInv {
+  int j[4][nloc(P)];
+};
+template <int P>
+constexpr LaneInv<P> make_lane_inv() {
+  constexpr DGFaceTab<P> FT = make_face_tab<P>();
+  LaneInv<P> r{};
+  for (int g = 0; g < 4; ++g)
+    for (int j = 0; j < nloc(P); ++j) r.j[g][FT.perm[g][j]] = j;
+  return r;
+}
+// slot of cell node f + 4 t in lane g's row for a run-time f in 0..3 and
+// unrolled (compile-time) g, t: three selects between constants
+template <int P>
+SMF_HDI int lane_slot4(int g, int t, int f) {
+  constexpr LaneInv<P> LI = make_lane_inv<P>();
+  constexpr int NL = nloc(P);
+  const int i0 = 4 * t;
+  return select4<int>(f, LI.j[g][i0 < NL ? i0 : 0], LI.j[g][i0 + 1 < NL ? i0 + 1 : 0],
+                      LI.j[g][i0 + 2 < NL ? i0 + 2 : 0], LI.j[g][i0 + 3 < NL ? i0 + 3 : 0]);
+}
+
 // Lane f of cell k, component c: the cell term's share (above) and the SIPG
 // flux of face f, evaluated with the cell's nodes renumbered so that face f
 // is face 3 (the neighbour's frame of apply_cell), so every table is the
-// compile-time face-3 table.  yrow[i] (original node order) receives the
-// lane's contribution to row i (MODE 0) or to the diagonal (MODE 1); the four
-// lanes' rows add up to apply_cell's result.
+// compile-time face-3 table.  yrow[j] receives the lane's contribution to the
+// row of its frame node j (cell node perm[f][j]; make_lane_inv inverts it)
+// (MODE 0) or to that diagonal entry (MODE 1); the four lanes' rows add up to
+// apply_cell's result.  Storing in the lane frame keeps the permutation out
+// of the registers live across the lane's work.
 template <int P, typename T, int NC, bool MASS, int MODE>
 SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __restrict__ geo,
                             const int32_t* __restrict__ nbr, const uint32_t* __restrict__ nface, T factor, T mass,
@@ -698,7 +725,7 @@ SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* _
     }
   }
 #pragma unroll
-  for (int i = 0; i < NL; ++i) yrow[pf[i]] = y[i];
+  for (int i = 0; i < NL; ++i) yrow[i] = y[i];
 }
 
 }  // namespace dgt

@@ -242,8 +242,13 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells,
                                                ybuf[cl][f], pkind);
     __syncthreads();
     if (active)
-      for (int i = f; i < NL; i += 4)
-        dst[(k * NC + c) * NL + i] = ((ybuf[cl][0][i] + ybuf[cl][1][i]) + ybuf[cl][2][i]) + ybuf[cl][3][i];
+#pragma unroll
+      for (int t = 0; t < (NL + 3) / 4; ++t)
+        if (f + 4 * t < NL)
+          dst[(k * NC + c) * NL + f + 4 * t] =
+              ((ybuf[cl][0][dgt::lane_slot4<P>(0, t, f)] + ybuf[cl][1][dgt::lane_slot4<P>(1, t, f)]) +
+               ybuf[cl][2][dgt::lane_slot4<P>(2, t, f)]) +
+              ybuf[cl][3][dgt::lane_slot4<P>(3, t, f)];
     __syncthreads();
   }
 }
@@ -267,8 +272,13 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_ep_kernel(int64_t ncel
                                           pkind);
   __syncthreads();
   if (active)
-    for (int i = f; i < NL; i += 4)
-      apply_epilogue(ep, k * NL + i, ((ybuf[cl][0][i] + ybuf[cl][1][i]) + ybuf[cl][2][i]) + ybuf[cl][3][i]);
+#pragma unroll
+    for (int t = 0; t < (NL + 3) / 4; ++t)
+      if (f + 4 * t < NL)
+        apply_epilogue(ep, k * NL + f + 4 * t,
+                       ((ybuf[cl][0][dgt::lane_slot4<P>(0, t, f)] + ybuf[cl][1][dgt::lane_slot4<P>(1, t, f)]) +
+                        ybuf[cl][2][dgt::lane_slot4<P>(2, t, f)]) +
+                           ybuf[cl][3][dgt::lane_slot4<P>(3, t, f)]);
 }
 
 template <int P, typename T, int NC, bool MASS, int MODE>

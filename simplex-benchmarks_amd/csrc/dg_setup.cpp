@@ -190,7 +190,10 @@ void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
                                                     (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, yr[f],
                                                     pl.penalty_kind);
         }
-        for (int i = 0; i < NL; ++i) dst[(k * NC + c) * NL + i] = ((yr[0][i] + yr[1][i]) + yr[2][i]) + yr[3][i];
+        constexpr dgt::LaneInv<P> LI = dgt::make_lane_inv<P>();
+        for (int i = 0; i < NL; ++i)
+          dst[(k * NC + c) * NL + i] =
+              ((yr[0][LI.j[0][i]] + yr[1][LI.j[1][i]]) + yr[2][LI.j[2][i]]) + yr[3][LI.j[3][i]];
       }
     return;
   }
