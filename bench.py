@@ -295,15 +295,18 @@ def main():
             traffic = json.load(f).get("bytes_per_vmult")
     variants = []
     if args.variants:
-        todo = [("dg", 64, 1, "f64"), ("dg", 64, 3, "f64"), ("dg", 64, 1, "f32"), ("dg", 64, 2, "f32"),
-                ("dg", 64, 3, "f32"), ("dg_helmholtz", 32, 2, "f64"), ("dg_helmholtz", 32, 2, "f32"),
-                ("cg", 128, 2, "f64"), ("cg", 128, 2, "f32")]
-        for kind, n, deg, prec in todo:
-            if (kind, n, deg, prec) == (args.kind, args.n, args.degree, args.precision):
+        # (kind, n, degree, precision, hierarchical reordering)
+        todo = [("dg", 64, 1, "f64", True), ("dg", 64, 3, "f64", True), ("dg", 64, 1, "f32", True),
+                ("dg", 64, 2, "f32", True), ("dg", 64, 3, "f32", True), ("dg", 64, 2, "f64", False),
+                ("dg_helmholtz", 32, 2, "f64", True), ("dg_helmholtz", 32, 2, "f32", True),
+                ("cg", 128, 2, "f64", True), ("cg", 128, 2, "f32", True)]
+        for kind, n, deg, prec, reorder in todo:
+            if (kind, n, deg, prec, reorder) == (args.kind, args.n, args.degree, args.precision, not args.no_reorder):
                 continue
-            r = measure(smf, kind, n, deg, prec, max(10, args.steps // 2), 3, args.seed)
+            r = measure(smf, kind, n, deg, prec, max(10, args.steps // 2), 3, args.seed, reorder=reorder)
+            r["reorder"] = reorder
             variants.append({k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()
-                             if k in ("kind", "degree", "precision", "n", "dofs", "ms_per_step", "gdofs",
+                             if k in ("kind", "degree", "precision", "n", "reorder", "dofs", "ms_per_step", "gdofs",
                                       "achieved_gbs", "frac")})
     base = None
     if args.cpu_baseline:
