@@ -106,15 +106,33 @@ __global__ __launch_bounds__(kBlock) void reduce_final_kernel(int nparts, const 
   if (threadIdx.x == 0) *out = s[0];
 }
 
-// y = A x for a CSR matrix (one thread per row, fixed summation order)
+// y = A x for a CSR matrix (one thread per row, fixed summation order);
+// four entries per step: their column indices and values, then the four x
+// gathers, are in flight together (clamped indices, masked sums)
 template <typename T>
 __global__ void csr_kernel(int64_t nrows, const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
                            const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y, int add) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
+  const int64_t k0 = ptr[r], k1 = ptr[r + 1];
+  const T y0 = add ? y[r] : T(0);
   T s = T(0);
-  for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k) s += val[k] * x[col[k]];
-  y[r] = add ? y[r] + s : s;
+  for (int64_t k = k0; k < k1; k += 4) {
+    int32_t c[4];
+    T v[4], xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t kk = k + u < k1 ? k + u : k1 - 1;
+      c[u] = col[kk];
+      v[u] = val[kk];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k + u < k1) s += v[u] * xv[u];
+  }
+  y[r] = add ? y0 + s : s;
 }
 
 // Chebyshev step (three-term recurrence on the Jacobi-preconditioned
