@@ -148,7 +148,10 @@ __device__ __forceinline__ T sum_partials(const T* __restrict__ part, int32_t k0
 // NC components (components-batched vector Laplacian,
 // CG/components_batched/application.cc:1024-1216) share the index tables and
 // the geometry of a cell; component c of scalar DoF d is row c*cstride + d.
-template <int P, typename T, int MODE, bool ATOMIC, int NC>
+// FEAT: bit 0 = pending interface staging (ChebyRun steps), bit 1 = per-cell
+// Nitsche boundary terms; both compile out of the plain operator kernels
+// (they cost registers: P2 FP32 64 -> 120 VGPRs with both)
+template <int P, typename T, int MODE, bool ATOMIC, int NC, int FEAT>
 __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ own_start,
     const int32_t* __restrict__ ghost_start, const int32_t* __restrict__ ghost_idx,
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
         for (int u = 0; u < SB; ++u)
           if (j0 + u * nt + lane < no) su[j0 + u * nt + lane] = v[u];
       }
-      if (pg.on) {
+      if ((FEAT & 1) && pg.on) {
         // ghosts are interface DoFs whose previous step is still pending
         constexpr int GB = 4;
         for (int j0 = 0; j0 < ng; j0 += GB * nt) {
@@ -261,7 +264,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
 #pragma unroll
     for (int t = 0; t < 6; ++t) m_n[t] = metric[t * ncells_total + cell];
     col_n = ccol[cell];
-    if (bcell) bc_n = bcell[cell];
+    if ((FEAT & 2) && bcell) bc_n = bcell[cell];
   };
   if (nc > 0) load_round(0);
   for (int r0 = 0; r0 < nc; r0 += nt) {
@@ -282,7 +285,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     for (int i = 0; i < NL; ++i) y[i] = T(0);
     // Nitsche boundary terms of the cell (Chebyshev runs of Nitsche levels)
     auto add_boundary = [&](const T* uu) {
-      if (mybc < 0) return;
+      if (!(FEAT & 2) || mybc < 0) return;
       const T* B = bmat + (int64_t)mybc * NL * NL;
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
@@ -450,13 +453,13 @@ __global__ void cg_fixup_kernel(int64_t n_int, int64_t n_iface, const int32_t* _
 
 // partial_out/pg/fixup: a ChebyRun step (its own partial buffer, pending
 // interface rows of the previous step, no fix-up launch)
-template <int P, typename T, int MODE, int NC>
-void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpilogue<T> ep = {},
-                 Pending<T> pg = {}, T* partial_out = nullptr, bool fixup = true, bool cell_boundary = false) {
+template <int P, typename T, int MODE, int NC, int FEAT>
+void launch_cg_k(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpilogue<T> ep, Pending<T> pg,
+                 T* partial_out, bool fixup, bool cell_boundary) {
   const CGPlan& pl = op.cg;
   const size_t shmem = NC * 2 * ((size_t)pl.max_local + 1) * sizeof(T);
   const bool atomic = op.opt.accumulate == 1 || (op.opt.accumulate == 0 && sizeof(T) == 8);
-  auto kern = atomic ? cg_wave_kernel<P, T, MODE, true, NC> : cg_wave_kernel<P, T, MODE, false, NC>;
+  auto kern = atomic ? cg_wave_kernel<P, T, MODE, true, NC, FEAT> : cg_wave_kernel<P, T, MODE, false, NC, FEAT>;
   if (shmem > 160 * 1024)
     throw std::runtime_error("CG patch needs " + std::to_string(shmem) + " bytes of LDS (max 160 KiB): use smaller patch_cells");
   if (shmem > 65536)
@@ -484,6 +487,18 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpi
                        0, s, pl.n_int, nif, op.d_fix_ptr.as<int32_t>(), op.d_partial.as<T>(), pl.n_free, pl.n_dofs,
                        src, dst, NC, pl.n_dofs, std::max<int64_t>(pl.n_ghost_total, 1), ep);
   SMF_HIP(hipGetLastError());
+}
+
+template <int P, typename T, int MODE, int NC>
+void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpilogue<T> ep = {},
+                 Pending<T> pg = {}, T* partial_out = nullptr, bool fixup = true, bool cell_boundary = false) {
+  if constexpr (MODE == 0 && NC == 1) {
+    if (partial_out && cell_boundary)
+      return launch_cg_k<P, T, MODE, NC, 3>(op, dst, src, s, ep, pg, partial_out, fixup, true);
+    if (partial_out) return launch_cg_k<P, T, MODE, NC, 1>(op, dst, src, s, ep, pg, partial_out, fixup, false);
+  }
+  if (pg.on || cell_boundary) throw std::runtime_error("CG kernel: pending staging / cell boundary need a Chebyshev run");
+  launch_cg_k<P, T, MODE, NC, 0>(op, dst, src, s, ep, pg, partial_out, fixup, false);
 }
 
 // Nitsche boundary rows: dst[row] += sum_k val[k] src[col[k]] (MODE 0) or
