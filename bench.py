@@ -299,7 +299,7 @@ def main():
         todo = [("dg", 64, 1, "f64", True), ("dg", 64, 3, "f64", True), ("dg", 64, 1, "f32", True),
                 ("dg", 64, 2, "f32", True), ("dg", 64, 3, "f32", True), ("dg", 64, 2, "f64", False),
                 ("dg_helmholtz", 32, 2, "f64", True), ("dg_helmholtz", 32, 2, "f32", True),
-                ("cg", 128, 2, "f64", True), ("cg", 128, 2, "f32", True)]
+                ("cg", 128, 2, "f64", True), ("cg", 128, 2, "f32", True), ("cg_vector", 64, 2, "f64", True)]
         for kind, n, deg, prec, reorder in todo:
             if (kind, n, deg, prec, reorder) == (args.kind, args.n, args.degree, args.precision, not args.no_reorder):
                 continue
