@@ -126,16 +126,17 @@ struct Pending {
   T* dw = nullptr;
 };
 
-// sum of the partials part[k0..k1) in order, eight loads in flight per step
+// sum of the partials part[k0..k1) in order, four loads in flight per step
+// (most interface DoFs have two to four partials)
 template <typename T>
 __device__ __forceinline__ T sum_partials(const T* __restrict__ part, int32_t k0, int32_t k1) {
   T s = T(0);
-  for (int32_t k = k0; k < k1; k += 8) {
-    T v[8];
+  for (int32_t k = k0; k < k1; k += 4) {
+    T v[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = part[min(k + u, k1 - 1)];
+    for (int u = 0; u < 4; ++u) v[u] = part[min(k + u, k1 - 1)];
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < 4; ++u)
       if (k + u < k1) s = k + u == k0 ? v[u] : s + v[u];
   }
   return s;
