@@ -8,7 +8,9 @@ cells-batched operator (DG/cells_batched/application.cc:1318-1332; metric
 GDoF/s = n_dofs * steps / time, CG/cells_batched/application.cc:1871-1889).
 The other degrees and precisions of that config, the Helmholtz-like vector
 operator (32^3) and the CG P2 operator at 128^3 (the HBM-resident size of
-CG/cells_batched/application.cc:1236-1416) are reported under "variants".
+CG/cells_batched/application.cc:1236-1416) are reported under "variants";
+the CG operator against its assembled CSR matrix (the matrix-based timing
+of CG/cells_batched/application.cc:1944-1988) under "matrix_based".
 
 Prints one JSON line with the metric, a roofline object (algorithmic bytes of
 the reference data model / event-timed vmult duration vs 8 TB/s) and a CPU
@@ -44,6 +46,9 @@ def parse():
     ap.add_argument("--degree", type=int, default=2)
     ap.add_argument("--kind", default="dg")
     ap.add_argument("--variants", type=int, default=1, help="0 to skip the other configs' lines")
+    ap.add_argument("--matrix-n", type=int, default=64,
+                    help="matrix-free vs assembled CSR comparison, P2 on an n^3 cube, P1 on (2n)^3, "
+                         "P3 on (n/2)^3 (0 skips)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied device-to-device once before timing (scale for rocprof counters)")
     ap.add_argument("--precision", default="f64")
@@ -235,6 +240,51 @@ def measure(smf, kind, n, degree, precision, steps, warmup, seed, reorder=True, 
     return res
 
 
+def _event_time(smf, fn, steps, warmup):
+    """seconds per call of fn(stream), HIP events on the stream the kernels run on"""
+    stream = smf.Stream()
+    for _ in range(warmup):
+        fn(stream)
+    stream.synchronize()
+    ev0, ev1 = smf.Event(), smf.Event()
+    ev0.record(stream)
+    for _ in range(steps):
+        fn(stream)
+    ev1.record(stream)
+    stream.synchronize()
+    return ev0.elapsed_ms(ev1) / 1e3 / steps
+
+
+def matrix_compare(smf, n, degree, precision, steps, seed):
+    """Matrix-free vmult against the assembled CSR matrix of the same CG
+    operator (the comparison of CG/cells_batched/application.cc:1944-1988),
+    both event-timed on the GPU at the same size."""
+    t0 = time.time()
+    op = smf.Operator(smf.Mesh.kuhn_cube(n, 0.2, seed), degree, kind="cg", precision=precision)
+    M = smf.SystemMatrix(op)
+    log(f"setup matrix comparison p{degree} {precision} n{n}: {time.time() - t0:.1f}s")
+    x = np.random.default_rng(seed).uniform(-1, 1, op.m()).astype(op.dtype)
+    src = smf.DeviceVector.from_numpy(x)
+    dst = op.initialize_dof_vector()
+    t_mf = _event_time(smf, lambda st: op.vmult(dst, src, st), steps, 3)
+    t_mb = _event_time(smf, lambda st: M.vmult(dst, src, st), steps, 3)
+    mi = M.info
+    return {"kind": "cg", "degree": degree, "precision": precision, "n": n, "dofs": op.m(),
+            "nnz": mi["nnz"], "nnz_per_row": round(mi["nnz"] / op.m(), 3), "row_blocks": mi["n_row_blocks"],
+            "matrix_free": {"ms_per_step": round(t_mf * 1e3, 5), "gdofs": round(op.m() / t_mf / 1e9, 4),
+                            "algorithmic_bytes": op.info["algorithmic_bytes"],
+                            "frac": round(op.info["algorithmic_bytes"] / t_mf / 1e9 / HBM_PEAK_GBS, 4)},
+            "csr_spmv": {"ms_per_step": round(t_mb * 1e3, 5), "gdofs": round(op.m() / t_mb / 1e9, 4),
+                         "algorithmic_bytes": mi["algorithmic_bytes"],
+                         "frac": round(mi["algorithmic_bytes"] / t_mb / 1e9 / HBM_PEAK_GBS, 4),
+                         "kernel": "spmv_stream_kernel (csrc/spmv_kernels.hip)"},
+            "speedup_matrix_free": round(t_mb / t_mf, 3),
+            "note": "matrix-based = assembled CSR (int32 columns, values in the operator precision); "
+                    "bytes: nnz (4 + w) + rows (8 + 2 w); the matrix-free operator's data at these sizes is "
+                    "partly resident in the 256 MB Infinity Cache when its model bytes are below that size, "
+                    "the CSR matrices are not"}
+
+
 def cpu_baseline_dg(smf, degree, precision_unused, seconds, seed):
     """DG oracle (CPU restatement of the reference SIPG vmult) with OpenMP on
     the host cores, on a bounded sample: the same operator on a 24^3 cube."""
@@ -348,6 +398,11 @@ def main():
         "cpu_baseline": base,
         "variants": variants,
     }
+    if args.matrix_n > 0:
+        # P1 at 2n, P2 at n, P3 at n/2: about the same number of DoFs
+        out["matrix_based"] = [matrix_compare(smf, m, deg, "f64", max(10, args.steps // 2), args.seed)
+                               for deg, m in ((1, 2 * args.matrix_n), (2, args.matrix_n),
+                                              (3, max(1, args.matrix_n // 2)))]
     if args.solve_n > 0:
         out["solve"] = solve_compare(smf, args.solve_n, 2, args.seed, args.solves)
         out["solve_dg"] = solve_compare(smf, args.solve_n, 2, args.seed, args.solves, kind="dg")

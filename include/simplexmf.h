@@ -277,6 +277,35 @@ int mf_solve_fcg_learned(mf_op_t A, mf_mg_t mg, mf_ls_t ls, void* x, const void*
  * a GPU; not an operator path (mf_vmult never calls it). */
 int mf_debug_host_apply(mf_op_t op, int32_t mode, void* dst, const void* src);
 
+/* ---- matrix-based comparison path ----------------------------------------
+ * The assembled sparse system matrix of a CG operator and its product, for
+ * timing matrix-free against matrix-based vmult as the reference does.
+ *   mf_get_system_matrix  Operator::get_system_matrix, CG/cells_batched/application_hp.cc:1411-1438
+ *                         (pattern: DoFTools::make_sparsity_pattern, CG/cells_batched/application.cc:1946-1961)
+ *   mf_csr_vmult          TrilinosWrappers::SparseMatrix::vmult, CG/cells_batched/application.cc:1964-1983
+ * CSR rows follow the operator's DoF numbering (all components for
+ * MF_CG_VECTOR, component-major like the vectors); columns ascend within a
+ * row.  A Dirichlet row holds its diagonal 1 only and free rows hold no
+ * constrained column, so A x equals mf_vmult up to summation order.  Values
+ * are the element integrals summed over cells in the operator's internal cell
+ * order, in FP64, then stored in the operator's precision.  DG operators are
+ * rejected (the reference assembles CG matrices only).  A host_only operator
+ * gives a host-only matrix (mf_csr_vmult fails on it). */
+typedef struct mf_sparse* mf_csr_t;
+typedef struct mf_csr_info {
+  int64_t n_rows, nnz;
+  int64_t n_row_blocks;    /* workgroups of mf_csr_vmult: consecutive rows, at most block_entries entries */
+  int32_t precision;       /* of the stored values */
+  int32_t block_entries;   /* and at most 256 rows per block */
+  double algorithmic_bytes; /* per product: nnz (4 + w) + n_rows (8 + 2 w) */
+} mf_csr_info;
+int mf_get_system_matrix(mf_op_t op, mf_csr_t* out);
+int mf_csr_vmult(mf_csr_t A, void* dst, const void* src, void* stream);   /* overwrites dst */
+int mf_csr_get_info(mf_csr_t A, mf_csr_info* info);
+/* host copy: ptr[n_rows + 1], col[nnz], val[nnz] (FP64 whatever the stored precision) */
+int mf_csr_copy_to_host(mf_csr_t A, int64_t* ptr, int32_t* col, double* val);
+int mf_csr_destroy(mf_csr_t A);
+
 /* ---- device plumbing (host) ----------------------------------------------
  * Thin wrappers over the HIP runtime this library is linked against, so that
  * callers without their own HIP bindings (tests, benchmarks) share one runtime

@@ -54,6 +54,14 @@ class Info(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class CsrInfo(C.Structure):
+    _fields_ = [("n_rows", C.c_int64), ("nnz", C.c_int64), ("n_row_blocks", C.c_int64),
+                ("precision", C.c_int32), ("block_entries", C.c_int32), ("algorithmic_bytes", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -127,6 +135,11 @@ def lib():
         "mf_ls_train": (C.c_int, [vp, vp, C.POINTER(TrainOptions), C.POINTER(TrainStats), vp]),
         "mf_solve_fcg_learned": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, i32, C.POINTER(FcgOptions),
                                            C.POINTER(FcgStats), vp]),
+        "mf_get_system_matrix": (C.c_int, [vp, C.POINTER(vp)]),
+        "mf_csr_vmult": (C.c_int, [vp, vp, vp, vp]),
+        "mf_csr_get_info": (C.c_int, [vp, C.POINTER(CsrInfo)]),
+        "mf_csr_copy_to_host": (C.c_int, [vp, vp, vp, vp]),
+        "mf_csr_destroy": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -220,6 +233,7 @@ _SIGNATURE_NAMES = [
     "mf_ls_default_train_options", "mf_ls_default_fcg_options", "mf_ls_create", "mf_ls_destroy",
     "mf_ls_set_params", "mf_ls_get_params", "mf_ls_save", "mf_ls_load", "mf_ls_apply", "mf_ls_forward",
     "mf_ls_features", "mf_ls_gradient", "mf_ls_train_step", "mf_ls_train", "mf_solve_fcg_learned",
+    "mf_get_system_matrix", "mf_csr_vmult", "mf_csr_get_info", "mf_csr_copy_to_host", "mf_csr_destroy",
 ]
 
 
@@ -461,6 +475,45 @@ class Operator:
         check(lib().mf_debug_host_apply(self.handle, mode, dst.ctypes.data,
                                          None if s is None else s.ctypes.data))
         return dst
+
+
+class SystemMatrix:
+    """Assembled sparse matrix of a CG operator (``get_system_matrix``,
+    CG/cells_batched/application_hp.cc:1411-1438) with a device CSR product
+    (the matrix-based timing of CG/cells_batched/application.cc:1944-1988)."""
+
+    def __init__(self, op: Operator):
+        h = C.c_void_p()
+        check(lib().mf_get_system_matrix(op.handle, C.byref(h)))
+        self.handle = h.value
+        self.op = op
+        self.dtype = op.dtype
+        inf = CsrInfo()
+        check(lib().mf_csr_get_info(self.handle, C.byref(inf)))
+        self.info = inf.as_dict()
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.mf_csr_destroy(self.handle)
+            self.handle = None
+
+    def m(self):
+        return self.info["n_rows"]
+
+    def vmult(self, dst: DeviceVector, src: DeviceVector, stream: Stream | None = None):
+        for v in (dst, src):
+            if v.n != self.m() or v.dtype != self.dtype:
+                raise ValueError(f"vector of {v.n} {v.dtype} does not match the matrix ({self.m()} {self.dtype})")
+        check(lib().mf_csr_vmult(self.handle, dst.ptr, src.ptr, stream.handle if stream else None))
+
+    def to_host(self):
+        """(ptr int64[n+1], col int32[nnz], val float64[nnz])"""
+        n, nnz = self.info["n_rows"], self.info["nnz"]
+        ptr = np.empty(n + 1, np.int64)
+        col = np.empty(max(nnz, 1), np.int32)
+        val = np.empty(max(nnz, 1), np.float64)
+        check(lib().mf_csr_copy_to_host(self.handle, ptr.ctypes.data, col.ctypes.data, val.ctypes.data))
+        return ptr, col[:nnz], val[:nnz]
 
 
 # --------------------------------------------------------------------------
