@@ -10,7 +10,9 @@ The other degrees and precisions of that config, the Helmholtz-like vector
 operator (32^3) and the CG P2 operator at 128^3 (the HBM-resident size of
 CG/cells_batched/application.cc:1236-1416) are reported under "variants";
 the CG operator against its assembled CSR matrix (the matrix-based timing
-of CG/cells_batched/application.cc:1944-1988) under "matrix_based".
+of CG/cells_batched/application.cc:1944-1988) under "matrix_based"; CG and
+DG P2 on a mesh in a scrambled input order with the hierarchical reordering
+on and off (CG/cells_batched/application.cc:1699-1756) under "external_mesh".
 
 Prints one JSON line with the metric, a roofline object (algorithmic bytes of
 the reference data model / event-timed vmult duration vs 8 TB/s) and a CPU
@@ -46,6 +48,8 @@ def parse():
     ap.add_argument("--degree", type=int, default=2)
     ap.add_argument("--kind", default="dg")
     ap.add_argument("--variants", type=int, default=1, help="0 to skip the other configs' lines")
+    ap.add_argument("--external-n", type=int, default=64,
+                    help="CG/DG P2 on an n^3 cube in a scrambled input order, reordering on and off (0 skips)")
     ap.add_argument("--matrix-n", type=int, default=64,
                     help="matrix-free vs assembled CSR comparison, P2 on an n^3 cube, P1 on (2n)^3, "
                          "P3 on (n/2)^3 (0 skips)")
@@ -285,6 +289,30 @@ def matrix_compare(smf, n, degree, precision, steps, seed):
                     "the CSR matrices are not"}
 
 
+def external_compare(smf, n, steps, seed):
+    """Operators on a mesh in an unstructured input order (the Kuhn cube with
+    vertices and cells randomly permuted, as an external mesh would arrive),
+    with the hierarchical reordering on and off: the reference's reorder_grid
+    study (CG/cells_batched/application.cc:1699-1756)."""
+    mesh = smf.Mesh.kuhn_cube(n, 0.2, seed).scrambled(seed=seed + 1)
+    out = []
+    for kind in ("cg", "dg"):
+        for reorder in (True, False):
+            t0 = time.time()
+            op = smf.Operator(mesh, 2, kind=kind, reorder=reorder)
+            log(f"setup external {kind} reorder={reorder} n{n}: {time.time() - t0:.1f}s")
+            x = np.random.default_rng(seed).uniform(-1, 1, op.m())
+            src = smf.DeviceVector.from_numpy(x)
+            dst = op.initialize_dof_vector()
+            t = _event_time(smf, lambda st: op.vmult(dst, src, st), steps, 3)
+            out.append({"kind": kind, "degree": 2, "precision": "f64", "n": n, "reorder": reorder,
+                        "dofs": op.m(), "ms_per_step": round(t * 1e3, 5), "gdofs": round(op.m() / t / 1e9, 4),
+                        "frac": round(op.info["algorithmic_bytes"] / t / 1e9 / HBM_PEAK_GBS, 4)})
+            del op, src, dst
+    return {"mesh": f"{n}^3 x 6 perturbed Kuhn tets with vertices and cells in a seeded random order, "
+                    "local vertices rotated", "runs": out}
+
+
 def cpu_baseline_dg(smf, degree, precision_unused, seconds, seed):
     """DG oracle (CPU restatement of the reference SIPG vmult) with OpenMP on
     the host cores, on a bounded sample: the same operator on a 24^3 cube."""
@@ -403,6 +431,8 @@ def main():
         out["matrix_based"] = [matrix_compare(smf, m, deg, "f64", max(10, args.steps // 2), args.seed)
                                for deg, m in ((1, 2 * args.matrix_n), (2, args.matrix_n),
                                               (3, max(1, args.matrix_n // 2)))]
+    if args.external_n > 0:
+        out["external_mesh"] = external_compare(smf, args.external_n, max(10, args.steps // 2), args.seed)
     if args.solve_n > 0:
         out["solve"] = solve_compare(smf, args.solve_n, 2, args.seed, args.solves)
         out["solve_dg"] = solve_compare(smf, args.solve_n, 2, args.seed, args.solves, kind="dg")

@@ -84,6 +84,18 @@ typedef struct mf_info {
  * vertices: 3*(n+1)^3 doubles, cells: 4*6*n^3 int32 (caller-allocated). */
 int mf_mesh_kuhn_cube(int32_t n, double perturb, uint64_t seed, double* vertices,
                       int32_t* cells);
+/* External mesh: a legacy ASCII VTK UNSTRUCTURED_GRID of tetrahedra (cell
+ * type 10; classic "k i0 .. ik-1" CELLS or the VTK 5 OFFSETS/CONNECTIVITY
+ * layout).  Points, lines and triangles (types 1, 3, 5) are skipped, other
+ * cell types rejected.  Tetrahedra of negative volume get local vertices 1
+ * and 2 swapped (counted in n_flipped).  Call with vertices == cells == NULL
+ * for the sizes, then with 3*n_vertices doubles and 4*n_cells int32.  The
+ * cells keep the file's order; mf_create's hierarchical reordering
+ * (mf_options.reorder) plays the role of the reference's reorder_grid.
+ * Replaces GridIn::read_vtk + SparsityTools::reorder_hierarchical,
+ * CG/cells_batched/application.cc:1699-1756. */
+int mf_mesh_read_vtk(const char* path, int64_t* n_vertices, int64_t* n_cells, int64_t* n_flipped,
+                     double* vertices, int32_t* cells);
 
 /* ---- operator lifetime -------------------------------------------------- */
 

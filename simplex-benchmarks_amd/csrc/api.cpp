@@ -219,6 +219,26 @@ int mf_mesh_kuhn_cube(int32_t n, double perturb, uint64_t seed, double* vertices
   });
 }
 
+int mf_mesh_read_vtk(const char* path, int64_t* n_vertices, int64_t* n_cells, int64_t* n_flipped,
+                     double* vertices, int32_t* cells) {
+  MF_TRY({
+    if (!path || !n_vertices || !n_cells) throw std::runtime_error("null argument");
+    if ((vertices == nullptr) != (cells == nullptr)) throw std::runtime_error("give both arrays or neither");
+    int64_t flipped = 0;
+    Mesh m = read_vtk(path, &flipped);
+    if (vertices) {
+      if (*n_vertices != m.nv || *n_cells != m.nc)
+        throw std::runtime_error("VTK: array sizes differ from the file's (" + std::to_string(m.nv) + " points, " +
+                                 std::to_string(m.nc) + " tetrahedra)");
+      std::memcpy(vertices, m.x.data(), m.x.size() * sizeof(double));
+      std::memcpy(cells, m.cells.data(), m.cells.size() * sizeof(int32_t));
+    }
+    *n_vertices = m.nv;
+    *n_cells = m.nc;
+    if (n_flipped) *n_flipped = flipped;
+  });
+}
+
 int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t precision,
               const mf_options* opt, mf_op_t* out) {
   MF_TRY({

@@ -4,6 +4,7 @@
 #pragma once
 #include <array>
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace smf {
@@ -41,6 +42,11 @@ struct SplitMix64 {
 // Interior vertices are displaced per coordinate by perturb*h*U[-1,1) drawn
 // from splitmix64(seed) in vertex-id order (x, y, z per vertex).
 Mesh kuhn_cube(int n, double perturb, uint64_t seed);
+
+// Tetrahedral mesh from a legacy ASCII VTK unstructured grid (vtk.cpp);
+// n_flipped counts tetrahedra whose local vertices 1 and 2 were swapped to
+// make det J > 0.
+Mesh read_vtk(const std::string& path, int64_t* n_flipped);
 
 struct FaceInfo {
   // per cell 4 face ids (face f opposite local vertex f)

@@ -135,6 +135,7 @@ def lib():
         "mf_ls_train": (C.c_int, [vp, vp, C.POINTER(TrainOptions), C.POINTER(TrainStats), vp]),
         "mf_solve_fcg_learned": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, i32, C.POINTER(FcgOptions),
                                            C.POINTER(FcgStats), vp]),
+        "mf_mesh_read_vtk": (C.c_int, [C.c_char_p, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), vp, vp]),
         "mf_get_system_matrix": (C.c_int, [vp, C.POINTER(vp)]),
         "mf_csr_vmult": (C.c_int, [vp, vp, vp, vp]),
         "mf_csr_get_info": (C.c_int, [vp, C.POINTER(CsrInfo)]),
@@ -233,7 +234,7 @@ _SIGNATURE_NAMES = [
     "mf_ls_default_train_options", "mf_ls_default_fcg_options", "mf_ls_create", "mf_ls_destroy",
     "mf_ls_set_params", "mf_ls_get_params", "mf_ls_save", "mf_ls_load", "mf_ls_apply", "mf_ls_forward",
     "mf_ls_features", "mf_ls_gradient", "mf_ls_train_step", "mf_ls_train", "mf_solve_fcg_learned",
-    "mf_get_system_matrix", "mf_csr_vmult", "mf_csr_get_info", "mf_csr_copy_to_host", "mf_csr_destroy",
+    "mf_mesh_read_vtk", "mf_get_system_matrix", "mf_csr_vmult", "mf_csr_get_info", "mf_csr_copy_to_host", "mf_csr_destroy",
 ]
 
 
@@ -256,6 +257,60 @@ class Mesh:
         c = np.empty((6 * n ** 3, 4), np.int32)
         check(lib().mf_mesh_kuhn_cube(n, perturb, seed, v.ctypes.data, c.ctypes.data))
         return Mesh(v, c, kuhn_n=n)
+
+    @staticmethod
+    def read_vtk(path):
+        """Tetrahedral mesh from a legacy ASCII VTK unstructured grid (the
+        external-mesh input of CG/cells_batched/application.cc:1699-1756).
+        Returns the mesh; .n_flipped counts cells whose local vertices 1 and 2
+        were swapped to make det J > 0."""
+        L = lib()
+        nv, nc, nf = C.c_int64(), C.c_int64(), C.c_int64()
+        p = os.fsencode(path)
+        check(L.mf_mesh_read_vtk(p, C.byref(nv), C.byref(nc), C.byref(nf), None, None))
+        v = np.empty((nv.value, 3), np.float64)
+        c = np.empty((nc.value, 4), np.int32)
+        check(L.mf_mesh_read_vtk(p, C.byref(nv), C.byref(nc), C.byref(nf), v.ctypes.data, c.ctypes.data))
+        m = Mesh(v, c)
+        m.n_flipped = nf.value
+        return m
+
+    def write_vtk(self, path, layout="classic"):
+        """Writes the mesh as a legacy ASCII VTK unstructured grid of
+        tetrahedra; layout "classic" (version 3.0 CELLS) or "vtk5"
+        (OFFSETS/CONNECTIVITY)."""
+        nv, nc = self.vertices.shape[0], self.cells.shape[0]
+        with open(path, "w") as f:
+            f.write("# vtk DataFile Version %s\nsimplexmf mesh\nASCII\nDATASET UNSTRUCTURED_GRID\n"
+                    % ("3.0" if layout == "classic" else "5.1"))
+            f.write(f"POINTS {nv} double\n")
+            np.savetxt(f, self.vertices, fmt="%.17g")
+            if layout == "classic":
+                f.write(f"CELLS {nc} {5 * nc}\n")
+                np.savetxt(f, np.hstack([np.full((nc, 1), 4, np.int64), self.cells]), fmt="%d")
+            elif layout == "vtk5":
+                f.write(f"CELLS {nc + 1} {4 * nc}\nOFFSETS vtktypeint64\n")
+                np.savetxt(f, 4 * np.arange(nc + 1, dtype=np.int64)[None, :], fmt="%d")
+                f.write("CONNECTIVITY vtktypeint64\n")
+                np.savetxt(f, self.cells.reshape(1, -1), fmt="%d")
+            else:
+                raise ValueError("layout must be 'classic' or 'vtk5'")
+            f.write(f"CELL_TYPES {nc}\n")
+            np.savetxt(f, np.full((1, nc), 10, np.int64), fmt="%d")
+
+    def scrambled(self, seed=0):
+        """The same mesh with vertices and cells in a seeded random order and
+        each cell's vertices rotated (an unstructured input order; no
+        lattice, so only p-multigrid levels are available)."""
+        rng = np.random.default_rng(seed)
+        vp = rng.permutation(self.vertices.shape[0])
+        inv = np.empty_like(vp)
+        inv[vp] = np.arange(vp.size)
+        cells = inv[self.cells[rng.permutation(self.cells.shape[0])]]
+        # even permutations of the local vertices keep the orientation
+        rot = np.array([[0, 1, 2, 3], [1, 2, 0, 3], [2, 0, 1, 3], [0, 2, 3, 1]])
+        cells = np.take_along_axis(cells, rot[rng.integers(0, len(rot), cells.shape[0])], axis=1)
+        return Mesh(self.vertices[vp], cells.astype(np.int32))
 
     def desc(self):
         d = MeshDesc()
