@@ -206,14 +206,14 @@ def solve_mode(args):
 
 
 def measure(smf, kind, n, degree, precision, steps, warmup, seed, reorder=True, patch_cells=0, cells_per_wave=0,
-            accumulate=0, keep=False, dg_kernel=0):
+            accumulate=0, keep=False, dg_kernel=0, manifold="none"):
     """Event-timed vmults of one operator on `stream`; returns the numbers of
     one bench line (and the operator and input when keep=True)."""
     t0 = time.time()
     mesh = smf.Mesh.kuhn_cube(n, 0.2, seed)
     op = smf.Operator(mesh, degree, kind=kind, precision=precision, reorder=reorder,
                       patch_cells=patch_cells, accumulate=accumulate, dg_kernel=dg_kernel, mass_coefficient=1.0,
-                      cells_per_wave=cells_per_wave,
+                      cells_per_wave=cells_per_wave, manifold=manifold,
                       laplace_coefficient=1.0)
     info = op.info
     log(f"setup {kind} p{degree} {precision} n{n}: {time.time() - t0:.1f}s")
@@ -377,15 +377,22 @@ def main():
         todo = [("dg", 64, 1, "f64", True), ("dg", 64, 3, "f64", True), ("dg", 64, 1, "f32", True),
                 ("dg", 64, 2, "f32", True), ("dg", 64, 3, "f32", True), ("dg", 64, 2, "f64", False),
                 ("dg_helmholtz", 32, 2, "f64", True), ("dg_helmholtz", 32, 2, "f32", True),
-                ("cg", 128, 2, "f64", True), ("cg", 128, 2, "f32", True), ("cg_vector", 64, 2, "f64", True)]
+                ("cg", 128, 2, "f64", True), ("cg", 128, 2, "f32", True), ("cg_vector", 64, 2, "f64", True),
+                ("cg", 128, 2, "f64", "spherical")]
         for kind, n, deg, prec, reorder in todo:
-            if (kind, n, deg, prec, reorder) == (args.kind, args.n, args.degree, args.precision, not args.no_reorder):
+            # a string in the last slot: curved cells on that manifold (reordering on)
+            manifold = reorder if isinstance(reorder, str) else "none"
+            reorder = True if isinstance(reorder, str) else reorder
+            if (kind, n, deg, prec, reorder, manifold) == (args.kind, args.n, args.degree, args.precision,
+                                                           not args.no_reorder, "none"):
                 continue
-            r = measure(smf, kind, n, deg, prec, max(10, args.steps // 2), 3, args.seed, reorder=reorder)
+            r = measure(smf, kind, n, deg, prec, max(10, args.steps // 2), 3, args.seed, reorder=reorder,
+                        manifold=manifold)
             r["reorder"] = reorder
+            r["manifold"] = manifold
             variants.append({k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()
-                             if k in ("kind", "degree", "precision", "n", "reorder", "dofs", "ms_per_step", "gdofs",
-                                      "achieved_gbs", "frac")})
+                             if k in ("kind", "degree", "precision", "n", "reorder", "manifold", "dofs", "ms_per_step",
+                                      "gdofs", "achieved_gbs", "frac")})
     base = None
     if args.cpu_baseline:
         log("cpu baseline ...")

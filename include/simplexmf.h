@@ -63,6 +63,14 @@ typedef struct mf_options {
                                multigrid, DG/CG c-transfer, application_phc.cc:1424-1456, 1798-1832) */
   int32_t cells_per_wave;   /* CG: cells per wavefront of a patch workgroup (0: 1024 P1, 512 P2, 128 P3);
                                waves per patch = patch_cells / cells_per_wave, at most 4 */
+  int32_t manifold;         /* CG, Dirichlet boundary: 0 affine cells (default); 1 curved cells: a P2
+                               geometry map whose edge nodes lie on the spherical manifold around
+                               manifold_center (MappingFE(FE_SimplexP(2)) + SphericalManifold,
+                               CG/cells_batched/application.cc:1613, 1668-1677, 1763-1767), integrated
+                               per quadrature point (the general D_e branch, :1361-1376) with a rule of
+                               degree 2p; 2 the same P2 map with straight-edge midpoints (affine cells
+                               through the curved-cell path) */
+  double manifold_center[3];  /* centre of the spherical manifold (default (5, 0, 0), as the reference) */
 } mf_options;
 
 typedef struct mf_info {
@@ -137,6 +145,11 @@ int mf_get_cell_vertices(mf_op_t op, int32_t* verts);
 int mf_get_dof_points(mf_op_t op, double* xyz);
 /* 1 for Dirichlet-constrained scalar DoFs (n_scalar_dofs bytes) */
 int mf_get_constrained(mf_op_t op, uint8_t* mask);
+/* the cell quadrature rule of a CG operator (deal.II's QGaussSimplex(p+1),
+ * CG/cells_batched/application.cc:1614, is this library's own symmetric rule
+ * of degree 2p-2, or 2p for curved cells): n_q_points points on the unit
+ * tetrahedron (3 doubles each) and weights summing to 1/6 */
+int mf_get_quadrature(mf_op_t op, double* points, double* weights);
 
 const char* mf_last_error(void);
 

@@ -215,6 +215,63 @@ def tet_rule(n):
     return pts, w
 
 
+# ---- curved cells (numpy restatement) ----------------------------------------
+# Follows CG/cells_batched/application.cc:1361-1376 (curvilinear D_e:
+# g_q <- JxW_q jac_q^T (jac_q g_q), jac = J^-T) with the geometry of
+# MappingFE(FE_SimplexP(2)) + SphericalManifold (:1613, 1668-1677): a P2 map
+# per cell through its vertices and, per edge, the manifold's midpoint.
+# Both bases are the oracle's own (Vandermonde inversion, orc_basis); the
+# quadrature points and weights are an input (the operator's rule, checked
+# separately for exactness), so the comparison is exact up to rounding.
+
+EDGES = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+
+
+def sphere_midpoint(xa, xb, center):
+    """Geodesic midpoint on the spherical manifold around center: direction
+    the normalised mean of the two unit directions, radius the mean radius
+    (deal.II SphericalManifold intermediate point at weight 1/2)."""
+    va, vb = xa - center, xb - center
+    ra = np.linalg.norm(va, axis=-1, keepdims=True)
+    rb = np.linalg.norm(vb, axis=-1, keepdims=True)
+    e = va / ra + vb / rb
+    return center + 0.5 * (ra + rb) * e / np.linalg.norm(e, axis=-1, keepdims=True)
+
+
+def curved_nodes(vertices, cells, center=(5.0, 0.0, 0.0), straight=False):
+    """(n_cells, 10, 3) P2 geometry nodes: 4 vertices, then the 6 edge nodes"""
+    X = np.asarray(vertices, np.float64)[np.asarray(cells)]
+    c = np.asarray(center, np.float64)
+    mids = [0.5 * (X[:, a] + X[:, b]) if straight else sphere_midpoint(X[:, a], X[:, b], c) for a, b in EDGES]
+    return np.concatenate([X, np.stack(mids, axis=1)], axis=1)
+
+
+def curved_element_matrices(p, nodes, pts, w):
+    """(n_cells, n_loc, n_loc): sum_q w_q |det J_q| (J_q^-T grad phi_i).(J_q^-T grad phi_j)"""
+    gg = np.stack([basis(2, x)[1] for x in pts])          # (nq, 10, 3) geometry gradients
+    gf = np.stack([basis(p, x)[1] for x in pts])          # (nq, nl, 3) FE gradients
+    J = np.einsum("kid,qie->kqde", nodes, gg)              # J[k, q][d][e] = dx_d / dxi_e
+    det = np.linalg.det(J)
+    if np.any(det <= 0):
+        raise ValueError("curved cell with non-positive Jacobian")
+    Jit = np.linalg.inv(J).transpose(0, 1, 3, 2)           # J^-T
+    grads = np.einsum("kqde,qie->kqid", Jit, gf)            # physical gradients
+    return np.einsum("q,kq,kqid,kqjd->kij", np.asarray(w), np.abs(det), grads, grads)
+
+
+def curved_vmult(p, vertices, cells, cell_dofs, constrained, pts, w, x, center=(5.0, 0.0, 0.0), straight=False):
+    """A x with identity rows on constrained DoFs, constrained columns read as 0
+    (CG/cells_batched/application.cc:1173-1175, 1287-1292)"""
+    K = curved_element_matrices(p, curved_nodes(vertices, cells, center, straight), pts, w)
+    con = np.asarray(constrained).astype(bool)
+    xf = np.where(con, 0.0, x)
+    ye = np.einsum("kij,kj->ki", K, xf[cell_dofs])
+    y = np.zeros(len(x))
+    np.add.at(y, cell_dofs.ravel(), ye.ravel())
+    y[con] = x[con]
+    return y
+
+
 # ---- learned-smoother network (oracle_mlp.c) --------------------------------
 def _mlp_lib():
     L = lib()

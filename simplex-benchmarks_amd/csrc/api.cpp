@@ -26,8 +26,12 @@ static std::vector<T> convert(const std::vector<double>& v) {
 void make_cg_shape(CGPlan& pl) {
   // reference shape gradients at the quadrature points [q][i][d], then w[q]
   const Lagrange fe(pl.p);
-  const Quadrature q = tet_rule(cg_quadrature_degree(pl.p));
+  // curved cells: a rule of degree 2p for the non-polynomial integrand
+  const Quadrature q = tet_rule(pl.curved ? 2 * pl.p : cg_quadrature_degree(pl.p));
   pl.n_q = q.size();
+  pl.quad_pts.resize(3 * pl.n_q);
+  for (int k = 0; k < pl.n_q; ++k)
+    for (int d = 0; d < 3; ++d) pl.quad_pts[3 * k + d] = q.pts[k][d];
   pl.shape.assign(pl.n_loc * pl.n_q * 3 + pl.n_q, 0.0);
   std::vector<double> grads(pl.n_loc * 3);
   for (int k = 0; k < pl.n_q; ++k) {
@@ -50,11 +54,13 @@ void upload_cg(Operator& op) {
   op.d_gfix.upload(pl.gfix);
   op.d_lidx.upload(pl.lidx);
   op.d_partial.alloc(std::max<int64_t>(pl.n_ghost_total, 1) * op.ncomp * op.wbytes());
+  // curved cells: the kernels read the per-quadrature-point metric
+  const std::vector<double>& metric = pl.curved ? pl.qmetric : pl.metric;
   if (op.precision == MF_F64) {
-    op.d_metric.upload(pl.metric);
+    op.d_metric.upload(metric);
     op.d_shape.upload(pl.shape);
   } else {
-    op.d_metric.upload(convert<float>(pl.metric));
+    op.d_metric.upload(convert<float>(metric));
     op.d_shape.upload(convert<float>(pl.shape));
   }
   if (pl.nitsche) {
@@ -117,7 +123,8 @@ void host_apply_cg_t(const Operator& op, int mode, T* dst, const T* src) {
     for (int lc = 0; lc < nc; ++lc) {
       const int64_t cell = c0 + lc;
       T m[6];
-      for (int t = 0; t < 6; ++t) m[t] = (T)pl.metric[(size_t)t * pl.n_cells + cell];
+      if (!pl.curved)
+        for (int t = 0; t < 6; ++t) m[t] = (T)pl.metric[(size_t)t * pl.n_cells + cell];
       T u[20], y[20];
       uint16_t li[20];
       for (int i = 0; i < nl; ++i) {
@@ -127,18 +134,21 @@ void host_apply_cg_t(const Operator& op, int mode, T* dst, const T* src) {
       }
       for (int q = 0; q < nq; ++q) {
         const T* Gq = &G[(size_t)q * nl * 3];
+        if (pl.curved)  // w_q |det J_q| J_q^-1 J_q^-T, weight included
+          for (int t = 0; t < 6; ++t) m[t] = (T)pl.qmetric[((size_t)q * 6 + t) * pl.n_cells + cell];
+        const T wq = pl.curved ? T(1) : W[q];
         if (mode == 0) {
           T g[3] = {0, 0, 0};
           for (int i = 0; i < nl; ++i)
             for (int d = 0; d < 3; ++d) g[d] += Gq[i * 3 + d] * u[i];
-          const T tx = W[q] * (m[0] * g[0] + m[1] * g[1] + m[2] * g[2]);
-          const T ty = W[q] * (m[1] * g[0] + m[3] * g[1] + m[4] * g[2]);
-          const T tz = W[q] * (m[2] * g[0] + m[4] * g[1] + m[5] * g[2]);
+          const T tx = wq * (m[0] * g[0] + m[1] * g[1] + m[2] * g[2]);
+          const T ty = wq * (m[1] * g[0] + m[3] * g[1] + m[4] * g[2]);
+          const T tz = wq * (m[2] * g[0] + m[4] * g[1] + m[5] * g[2]);
           for (int i = 0; i < nl; ++i) y[i] += Gq[i * 3] * tx + Gq[i * 3 + 1] * ty + Gq[i * 3 + 2] * tz;
         } else {
           for (int i = 0; i < nl; ++i) {
             const T* g = &Gq[i * 3];
-            y[i] += W[q] * (g[0] * (m[0] * g[0] + m[1] * g[1] + m[2] * g[2]) +
+            y[i] += wq * (g[0] * (m[0] * g[0] + m[1] * g[1] + m[2] * g[2]) +
                             g[1] * (m[1] * g[0] + m[3] * g[1] + m[4] * g[2]) +
                             g[2] * (m[2] * g[0] + m[4] * g[1] + m[5] * g[2]));
           }
@@ -208,6 +218,10 @@ void mf_default_options(mf_options* opt) {
   opt->accumulate = 0;
   opt->dg_kernel = 0;
   opt->boundary = 0;
+  opt->manifold = 0;
+  opt->manifold_center[0] = 5.0;
+  opt->manifold_center[1] = 0.0;
+  opt->manifold_center[2] = 0.0;
 }
 
 int mf_mesh_kuhn_cube(int32_t n, double perturb, uint64_t seed, double* vertices, int32_t* cells) {
@@ -216,6 +230,16 @@ int mf_mesh_kuhn_cube(int32_t n, double perturb, uint64_t seed, double* vertices
     Mesh m = kuhn_cube(n, perturb, seed);
     std::memcpy(vertices, m.x.data(), m.x.size() * sizeof(double));
     std::memcpy(cells, m.cells.data(), m.cells.size() * sizeof(int32_t));
+  });
+}
+
+int mf_get_quadrature(mf_op_t h, double* points, double* weights) {
+  MF_TRY({
+    if (!h || !points || !weights) throw std::runtime_error("null argument");
+    if (h->op.is_dg()) throw std::runtime_error("quadrature query: CG operators only");
+    const CGPlan& pl = h->op.cg;
+    std::memcpy(points, pl.quad_pts.data(), pl.quad_pts.size() * sizeof(double));
+    for (int q = 0; q < pl.n_q; ++q) weights[q] = pl.shape[(size_t)pl.n_loc * pl.n_q * 3 + q];
   });
 }
 
@@ -249,6 +273,9 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
     if (opt && (opt->boundary < 0 || opt->boundary > 1)) throw std::runtime_error("boundary must be 0 or 1");
     if (opt && (opt->dg_kernel < 0 || opt->dg_kernel > 3)) throw std::runtime_error("dg_kernel must be 0..3");
     if (opt && (opt->penalty_kind < 0 || opt->penalty_kind > 1)) throw std::runtime_error("penalty_kind must be 0 or 1");
+    if (opt && (opt->manifold < 0 || opt->manifold > 2)) throw std::runtime_error("manifold must be 0, 1 or 2");
+    if (opt && opt->manifold != 0 && ((kind != MF_CG_POISSON && kind != MF_CG_VECTOR) || opt->boundary != 0))
+      throw std::runtime_error("curved cells (manifold != 0) are available for the CG operators with the Dirichlet boundary");
     if (opt && opt->boundary == 1 && kind != MF_CG_POISSON)
       throw std::runtime_error("Nitsche boundary (boundary = 1) is available for the scalar CG operator");
     if (precision != MF_F64 && precision != MF_F32) throw std::runtime_error("precision must be MF_F64 or MF_F32");
@@ -378,8 +405,9 @@ int mf_get_info(mf_op_t h, mf_info* info) {
       info->max_cell_colors = pl.max_icolors;
       info->max_local_dofs = pl.max_local;
       // reference data model: src + dst, 32-bit index per cell DoF, J^-T + det J per cell
-      info->algorithmic_bytes =
-          2.0 * w * pl.n_dofs * op.ncomp + 4.0 * pl.n_loc * pl.n_cells + 10.0 * w * pl.n_cells;
+      // (curved cells: J^-T + JxW per quadrature point)
+      info->algorithmic_bytes = 2.0 * w * pl.n_dofs * op.ncomp + 4.0 * pl.n_loc * pl.n_cells +
+                                10.0 * w * pl.n_cells * (pl.curved ? pl.n_q : 1);
       info->flops = op.ncomp * (12.0 * pl.n_loc * pl.n_q + 18.0 * pl.n_q) * pl.n_cells;
     }
   });

@@ -43,6 +43,8 @@ template <int P>
 struct CGShape {
   static constexpr int NL = (P + 1) * (P + 2) * (P + 3) / 6;
   static constexpr int NQ = P == 1 ? 1 : (P == 2 ? 4 : 11);
+  // curved cells: the rule of degree 2p
+  static constexpr int NQC = P == 1 ? 4 : (P == 2 ? 11 : 24);
 };
 
 // P2 cell: y = A_K u in barycentric form at the symmetric 4-point rule
@@ -262,8 +264,10 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int64_t cell = c0 + min(r0 + lane, nc - 1);
 #pragma unroll
     for (int i = 0; i < NL; ++i) li_n[i] = lidx[(int64_t)i * ncells_total + cell];
+    if constexpr ((FEAT & 4) == 0) {
 #pragma unroll
-    for (int t = 0; t < 6; ++t) m_n[t] = metric[t * ncells_total + cell];
+      for (int t = 0; t < 6; ++t) m_n[t] = metric[t * ncells_total + cell];
+    }
     col_n = ccol[cell];
     if ((FEAT & 2) && bcell) bc_n = bcell[cell];
   };
@@ -298,8 +302,53 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     };
     // P1/P2: compile-time tables (literal constants after unrolling); P3
     // has too many distinct values for scalar registers and reads the table
-    // from memory one quadrature point at a time.
-    if constexpr (P == 2 && MODE == 0) {
+    // from memory one quadrature point at a time.  Curved cells (FEAT bit 4)
+    // read the gradient table of the degree-2p rule the same way and the
+    // metric w_q |det J_q| J_q^-1 J_q^-T per point, the next point's metric
+    // loaded while the current one is applied (the general D_e branch,
+    // CG/cells_batched/application.cc:1361-1376).
+    if constexpr ((FEAT & 4) != 0) {
+      constexpr int NQC = CGShape<P>::NQC;
+      const T* qm = metric + (c0 + min(lc, nc - 1));
+      T mq[6], mn[6];
+#pragma unroll
+      for (int t = 0; t < 6; ++t) mq[t] = qm[(int64_t)t * ncells_total];
+      T u[NL];
+      if (MODE == 0) {
+#pragma unroll
+        for (int i = 0; i < NL; ++i) u[i] = su[li[i]];
+      }
+#pragma unroll 1
+      for (int q = 0; q < NQC; ++q) {
+        const int qn = q + 1 < NQC ? q + 1 : q;
+#pragma unroll
+        for (int t = 0; t < 6; ++t) mn[t] = qm[(int64_t)(qn * 6 + t) * ncells_total];
+        const T* G = shape + q * NL * 3;
+        if (MODE == 0) {
+          T gx = T(0), gy = T(0), gz = T(0);
+#pragma unroll
+          for (int i = 0; i < NL; ++i) {
+            gx += G[i * 3 + 0] * u[i];
+            gy += G[i * 3 + 1] * u[i];
+            gz += G[i * 3 + 2] * u[i];
+          }
+          const T tx = mq[0] * gx + mq[1] * gy + mq[2] * gz;
+          const T ty = mq[1] * gx + mq[3] * gy + mq[4] * gz;
+          const T tz = mq[2] * gx + mq[4] * gy + mq[5] * gz;
+#pragma unroll
+          for (int i = 0; i < NL; ++i) y[i] += G[i * 3 + 0] * tx + G[i * 3 + 1] * ty + G[i * 3 + 2] * tz;
+        } else {
+#pragma unroll
+          for (int i = 0; i < NL; ++i) {
+            const T gx = G[i * 3 + 0], gy = G[i * 3 + 1], gz = G[i * 3 + 2];
+            y[i] += gx * (mq[0] * gx + mq[1] * gy + mq[2] * gz) + gy * (mq[1] * gx + mq[3] * gy + mq[4] * gz) +
+                    gz * (mq[2] * gx + mq[4] * gy + mq[5] * gz);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 6; ++t) mq[t] = mn[t];
+      }
+    } else if constexpr (P == 2 && MODE == 0) {
       T u[NL];
 #pragma unroll
       for (int i = 0; i < NL; ++i) u[i] = su[li[i]];
@@ -493,6 +542,11 @@ void launch_cg_k(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpi
 template <int P, typename T, int MODE, int NC>
 void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpilogue<T> ep = {},
                  Pending<T> pg = {}, T* partial_out = nullptr, bool fixup = true, bool cell_boundary = false) {
+  if (op.cg.curved) {
+    if (pg.on || cell_boundary || partial_out) throw std::runtime_error("CG kernel: curved cells take no Chebyshev run");
+    if (op.cg.n_q != CGShape<P>::NQC) throw std::runtime_error("CG kernel: curved-cell rule does not match the kernel");
+    return launch_cg_k<P, T, MODE, NC, 4>(op, dst, src, s, ep, pg, partial_out, fixup, false);
+  }
   if constexpr (MODE == 0 && NC == 1) {
     if (partial_out && cell_boundary)
       return launch_cg_k<P, T, MODE, NC, 3>(op, dst, src, s, ep, pg, partial_out, fixup, true);
@@ -665,7 +719,7 @@ Pending<T> pending_of(const Operator& op, const ChebyRun<T>& run, int k) {
 
 }  // namespace
 
-bool cg_has_cheby_run(const Operator& op) { return !op.is_dg() && op.ncomp == 1; }
+bool cg_has_cheby_run(const Operator& op) { return !op.is_dg() && op.ncomp == 1 && !op.cg.curved; }
 
 template <typename T>
 void launch_cg_cheby_step(const Operator& op, ChebyRun<T>& run, T c1, T c2, int first, hipStream_t s) {

@@ -286,6 +286,7 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
     plan.detj[k] = g.det;
   }
   if (plan.nitsche) build_nitsche_boundary(mesh, plan, opt.penalty_kind);
+  if (opt.manifold != 0) build_curved_geometry(mesh, opt, plan);
 }
 
 namespace {

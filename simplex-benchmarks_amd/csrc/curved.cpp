@@ -1,0 +1,132 @@
+// Curved cells: a P2 geometry map per cell whose edge nodes lie on a
+// manifold, and the quadrature-point geometry of the general D_e branch.
+//
+// Reference: MappingFE(FE_SimplexP(2)) with a SphericalManifold around
+// (5, 0, 0) attached to every cell (CG/cells_batched/application.cc:1613,
+// 1668-1677, 1763-1767, 1803-1806); the curvilinear quadrature-point
+// operator g_q <- JxW_q jac_q^T (jac_q g_q) with jac = J^-T (:1361-1376).
+//
+// Geometry nodes of cell K: its four vertices as given, and per edge (a, b)
+// the manifold's midpoint.  For the spherical manifold around c this is the
+// geodesic midpoint: c + (r_a + r_b)/2 * (e_a + e_b)/|e_a + e_b| with
+// r = |x - c|, e = (x - c)/r (the weight-1/2 point of the sphere-direction
+// slerp with linearly interpolated radius).  The formula is symmetric in a
+// and b, so the two cells of an edge agree on its node and the map is
+// continuous.  Mode 2 uses the straight midpoint (x_a + x_b)/2.
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+#include "fe.h"
+#include "op.h"
+
+namespace smf {
+
+namespace {
+
+void edge_node(const double* xa, const double* xb, int manifold, const double* c, double* out) {
+  if (manifold == 2) {
+    for (int d = 0; d < 3; ++d) out[d] = 0.5 * (xa[d] + xb[d]);
+    return;
+  }
+  double va[3], vb[3], ra = 0, rb = 0;
+  for (int d = 0; d < 3; ++d) {
+    va[d] = xa[d] - c[d];
+    vb[d] = xb[d] - c[d];
+    ra += va[d] * va[d];
+    rb += vb[d] * vb[d];
+  }
+  ra = std::sqrt(ra);
+  rb = std::sqrt(rb);
+  if (!(ra > 0) || !(rb > 0)) throw std::runtime_error("curved cells: a vertex at the manifold centre");
+  double e[3], ne = 0;
+  for (int d = 0; d < 3; ++d) {
+    e[d] = va[d] / ra + vb[d] / rb;
+    ne += e[d] * e[d];
+  }
+  ne = std::sqrt(ne);
+  if (!(ne > 1e-12)) throw std::runtime_error("curved cells: an edge spans opposite directions from the centre");
+  const double r = 0.5 * (ra + rb);
+  for (int d = 0; d < 3; ++d) out[d] = c[d] + r * e[d] / ne;
+}
+
+}  // namespace
+
+void build_curved_geometry(const Mesh& mesh, const mf_options& opt, CGPlan& plan) {
+  const int64_t nc = plan.n_cells;
+  const int nl = plan.n_loc;
+  const Lagrange geo(2), fe(plan.p);
+  const Quadrature q = tet_rule(2 * plan.p);
+  const int nq = q.size();
+  // geometry-basis gradients at the rule's points and values at the FE nodes
+  std::vector<double> gg((size_t)nq * 10 * 3), gv((size_t)nl * 10);
+  for (int k = 0; k < nq; ++k) geo.eval(q.pts[k].data(), nullptr, &gg[(size_t)k * 30]);
+  for (int i = 0; i < nl; ++i) {
+    const auto xr = fe.node_point(i);
+    geo.eval(xr.data(), &gv[(size_t)i * 10], nullptr);
+  }
+  plan.curved = true;
+  plan.qmetric.assign((size_t)nq * 6 * nc, 0.0);
+  const int ab[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+  // each support point is written by the first cell of its DoF, so the
+  // result does not depend on the thread schedule
+  std::vector<int64_t> owner(plan.n_dofs, -1);
+  for (int64_t k = 0; k < nc; ++k)
+    for (int i = 0; i < nl; ++i) {
+      const int64_t dof = plan.cell_dofs[k * nl + i];
+      if (owner[dof] < 0) owner[dof] = k;
+    }
+  int bad = 0;
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < nc; ++k) {
+    const int64_t c = plan.cell_perm[k];
+    double X[10][3];
+    for (int v = 0; v < 4; ++v)
+      for (int d = 0; d < 3; ++d) X[v][d] = mesh.x[3 * (int64_t)mesh.cells[4 * c + v] + d];
+    for (int e = 0; e < 6; ++e)
+      edge_node(X[kEdgeVerts[e][0]], X[kEdgeVerts[e][1]], opt.manifold, opt.manifold_center, X[4 + e]);
+    for (int qq = 0; qq < nq; ++qq) {
+      // J[d][e] = sum_i X_i[d] d(phi_i)/d(xi_e)
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int i = 0; i < 10; ++i)
+        for (int d = 0; d < 3; ++d)
+          for (int e = 0; e < 3; ++e) J[d][e] += X[i][d] * gg[((size_t)qq * 10 + i) * 3 + e];
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                   c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      if (!(det > 0)) {
+#pragma omp atomic write
+        bad = 1;
+        continue;
+      }
+      double Ji[3][3];  // J^-1 = adj(J) / det
+      Ji[0][0] = c00 / det;
+      Ji[1][0] = c01 / det;
+      Ji[2][0] = c02 / det;
+      Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+      Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+      Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+      Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+      Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+      Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+      for (int t = 0; t < 6; ++t) {
+        double s = 0;
+        for (int e = 0; e < 3; ++e) s += Ji[ab[t][0]][e] * Ji[ab[t][1]][e];
+        plan.qmetric[((size_t)qq * 6 + t) * nc + k] = q.w[qq] * det * s;
+      }
+    }
+    // support points of the cell's DoFs under the curved map
+    for (int i = 0; i < nl; ++i) {
+      const int64_t dof = plan.cell_dofs[k * nl + i];
+      if (owner[dof] != k) continue;
+      for (int d = 0; d < 3; ++d) {
+        double s = 0;
+        for (int j = 0; j < 10; ++j) s += gv[(size_t)i * 10 + j] * X[j][d];
+        plan.dof_points[3 * dof + d] = s;
+      }
+    }
+  }
+  if (bad) throw std::runtime_error("curved cells: a quadrature point with non-positive Jacobian determinant");
+}
+
+}  // namespace smf

@@ -24,13 +24,19 @@ void element_row(const CGPlan& pl, int64_t k, int i, double* out) {
   const double* G = pl.shape.data();
   const double* W = G + (size_t)nl * nq * 3;
   double m[6];
-  for (int t = 0; t < 6; ++t) m[t] = pl.metric[(size_t)t * pl.n_cells + k];
+  if (!pl.curved)
+    for (int t = 0; t < 6; ++t) m[t] = pl.metric[(size_t)t * pl.n_cells + k];
   double g[64 * 3];  // M G_iq per quadrature point (n_q <= 64)
   for (int q = 0; q < nq; ++q) {
     const double* a = &G[(q * nl + i) * 3];
-    g[3 * q + 0] = W[q] * (m[0] * a[0] + m[1] * a[1] + m[2] * a[2]);
-    g[3 * q + 1] = W[q] * (m[1] * a[0] + m[3] * a[1] + m[4] * a[2]);
-    g[3 * q + 2] = W[q] * (m[2] * a[0] + m[4] * a[1] + m[5] * a[2]);
+    double w = W[q];
+    if (pl.curved) {  // per-point metric, weight included
+      for (int t = 0; t < 6; ++t) m[t] = pl.qmetric[((size_t)q * 6 + t) * pl.n_cells + k];
+      w = 1.0;
+    }
+    g[3 * q + 0] = w * (m[0] * a[0] + m[1] * a[1] + m[2] * a[2]);
+    g[3 * q + 1] = w * (m[1] * a[0] + m[3] * a[1] + m[4] * a[2]);
+    g[3 * q + 2] = w * (m[2] * a[0] + m[4] * a[1] + m[5] * a[2]);
   }
   for (int j = 0; j < nl; ++j) {
     double s = 0;

@@ -248,7 +248,8 @@ std::vector<double> dense_inverse(const Operator& op) {
   const double* W = G + nl * nq * 3;
   for (int64_t k = 0; k < pl.n_cells; ++k) {
     double m[6];
-    for (int t = 0; t < 6; ++t) m[t] = pl.metric[(size_t)t * pl.n_cells + k];
+    if (!pl.curved)
+      for (int t = 0; t < 6; ++t) m[t] = pl.metric[(size_t)t * pl.n_cells + k];
     for (int i = 0; i < nl; ++i) {
       const int64_t di = pl.cell_dofs[k * nl + i];
       if (di >= pl.n_free) continue;
@@ -259,7 +260,12 @@ std::vector<double> dense_inverse(const Operator& op) {
         for (int q = 0; q < nq; ++q) {
           const double* a = &G[(q * nl + i) * 3];
           const double* b = &G[(q * nl + j) * 3];
-          s += W[q] * (a[0] * (m[0] * b[0] + m[1] * b[1] + m[2] * b[2]) +
+          double w = W[q];
+          if (pl.curved) {  // per-point metric, weight included
+            for (int t = 0; t < 6; ++t) m[t] = pl.qmetric[((size_t)q * 6 + t) * pl.n_cells + k];
+            w = 1.0;
+          }
+          s += w * (a[0] * (m[0] * b[0] + m[1] * b[1] + m[2] * b[2]) +
                        a[1] * (m[1] * b[0] + m[3] * b[1] + m[4] * b[2]) +
                        a[2] * (m[2] * b[0] + m[4] * b[1] + m[5] * b[2]));
         }

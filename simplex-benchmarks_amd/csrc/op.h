@@ -97,6 +97,7 @@ struct CGPlan {
   std::vector<double> detj;          // n_cells
   int64_t n_ghost_total = 0;
   std::vector<double> shape;         // G (n_loc x n_q x 3) then w_q G
+  std::vector<double> quad_pts;      // 3 x n_q: the rule of shape (unit tetrahedron)
   // Nitsche boundary (weak Dirichlet): sparse rows of the boundary-face terms,
   // row bnd_rows[r] has entries [bnd_ptr[r], bnd_ptr[r+1]) (columns ascending)
   bool nitsche = false;
@@ -107,6 +108,11 @@ struct CGPlan {
   // in bmat (row-major, y_i += sum_j B_ij u_j), -1 without boundary faces
   std::vector<int32_t> bcell;
   std::vector<double> bmat;
+  // curved cells (P2 geometry map): qmetric[(q*6 + t)*n_cells + k] =
+  // w_q |det J_q| (J_q^-1 J_q^-T)_t at the points of the degree-2p rule,
+  // which shape then holds; metric keeps the straight-cell values
+  bool curved = false;
+  std::vector<double> qmetric;
 };
 
 // Host-side plan of a discontinuous-Galerkin (SIPG) operator.  Cell local
@@ -230,6 +236,8 @@ template <typename T>
 void launch_dg_epilogue(const Operator& op, const T* x, RowEpilogue<T> ep, hipStream_t s);
 
 void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan);
+// curved-cell geometry (opt.manifold != 0): qmetric and the mapped support points
+void build_curved_geometry(const Mesh& mesh, const mf_options& opt, CGPlan& plan);
 // boundary-face matrix of the Nitsche CG operator (plan.nitsche)
 void build_nitsche_boundary(const Mesh& mesh, CGPlan& plan, int penalty_kind);
 // SIPG penalty tau_K = (sum_bdry |F| + 1/2 sum_int |F|) / |K| per input cell

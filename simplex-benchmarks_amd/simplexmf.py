@@ -38,7 +38,8 @@ class Options(C.Structure):
     _fields_ = [("reorder", C.c_int32), ("patch_cells", C.c_int32),
                 ("mass_coefficient", C.c_double), ("laplace_coefficient", C.c_double),
                 ("penalty_kind", C.c_int32), ("host_only", C.c_int32), ("accumulate", C.c_int32),
-                ("dg_kernel", C.c_int32), ("boundary", C.c_int32), ("cells_per_wave", C.c_int32)]
+                ("dg_kernel", C.c_int32), ("boundary", C.c_int32), ("cells_per_wave", C.c_int32),
+                ("manifold", C.c_int32), ("manifold_center", C.c_double * 3)]
 
 
 class Info(C.Structure):
@@ -135,6 +136,7 @@ def lib():
         "mf_ls_train": (C.c_int, [vp, vp, C.POINTER(TrainOptions), C.POINTER(TrainStats), vp]),
         "mf_solve_fcg_learned": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, i32, C.POINTER(FcgOptions),
                                            C.POINTER(FcgStats), vp]),
+        "mf_get_quadrature": (C.c_int, [vp, vp, vp]),
         "mf_mesh_read_vtk": (C.c_int, [C.c_char_p, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), vp, vp]),
         "mf_get_system_matrix": (C.c_int, [vp, C.POINTER(vp)]),
         "mf_csr_vmult": (C.c_int, [vp, vp, vp, vp]),
@@ -234,7 +236,7 @@ _SIGNATURE_NAMES = [
     "mf_ls_default_train_options", "mf_ls_default_fcg_options", "mf_ls_create", "mf_ls_destroy",
     "mf_ls_set_params", "mf_ls_get_params", "mf_ls_save", "mf_ls_load", "mf_ls_apply", "mf_ls_forward",
     "mf_ls_features", "mf_ls_gradient", "mf_ls_train_step", "mf_ls_train", "mf_solve_fcg_learned",
-    "mf_mesh_read_vtk", "mf_get_system_matrix", "mf_csr_vmult", "mf_csr_get_info", "mf_csr_copy_to_host", "mf_csr_destroy",
+    "mf_get_quadrature", "mf_mesh_read_vtk", "mf_get_system_matrix", "mf_csr_vmult", "mf_csr_get_info", "mf_csr_copy_to_host", "mf_csr_destroy",
 ]
 
 
@@ -426,7 +428,7 @@ class Operator:
     def __init__(self, mesh: Mesh, degree: int, kind="cg", precision="f64", reorder=True,
                  patch_cells=0, mass_coefficient=1.0, laplace_coefficient=1.0,
                  penalty_kind=0, host_only=False, accumulate=0, dg_kernel=0, boundary="dirichlet",
-                 cells_per_wave=0):
+                 cells_per_wave=0, manifold="none", manifold_center=None):
         L = lib()
         opt = Options()
         L.mf_default_options(C.byref(opt))
@@ -440,6 +442,11 @@ class Operator:
         opt.dg_kernel = dg_kernel
         opt.boundary = {"dirichlet": 0, "nitsche": 1}[boundary] if isinstance(boundary, str) else int(boundary)
         opt.cells_per_wave = cells_per_wave
+        opt.manifold = {"none": 0, "spherical": 1, "straight": 2}[manifold] if isinstance(manifold, str) \
+            else int(manifold)
+        if manifold_center is not None:
+            for d in range(3):
+                opt.manifold_center[d] = float(manifold_center[d])
         self.mesh = mesh
         self.kind = KINDS[kind] if isinstance(kind, str) else int(kind)
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
@@ -522,6 +529,13 @@ class Operator:
         out = np.empty(self.n_scalar_dofs(), np.uint8)
         check(lib().mf_get_constrained(self.handle, out.ctypes.data))
         return out
+
+    def quadrature(self):
+        """(points (n_q, 3), weights (n_q,)) of the cell rule (CG operators)"""
+        nq = self.info["n_q_points"]
+        pts, w = np.empty((nq, 3)), np.empty(nq)
+        check(lib().mf_get_quadrature(self.handle, pts.ctypes.data, w.ctypes.data))
+        return pts, w
 
     def host_apply(self, src=None, mode=0):
         """Host execution of the kernels' schedule (table self-check, no GPU)."""
