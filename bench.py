@@ -57,6 +57,8 @@ def parse():
                     help="MiB copied device-to-device once before timing (scale for rocprof counters)")
     ap.add_argument("--precision", default="f64")
     ap.add_argument("--no-reorder", action="store_true")
+    ap.add_argument("--manifold", default="none", choices=["none", "spherical", "straight"],
+                    help="CG: curved cells on this manifold (P2 geometry map)")
     ap.add_argument("--patch-cells", type=int, default=0)
     ap.add_argument("--cells-per-wave", type=int, default=0, help="CG: cells per wave of a patch workgroup")
     ap.add_argument("--accumulate", type=int, default=0, help="CG LDS accumulation: 0 auto, 1 atomics, 2 read-add-write")
@@ -363,11 +365,14 @@ def main():
     head = measure(smf, args.kind, args.n, args.degree, args.precision, args.steps, args.warmup,
                    args.seed, reorder=not args.no_reorder, patch_cells=args.patch_cells,
                    cells_per_wave=args.cells_per_wave,
-                   accumulate=args.accumulate, keep=args.cpu_baseline and args.kind == "cg",
-                   dg_kernel=args.dg_kernel)
+                   accumulate=args.accumulate,
+                   keep=args.cpu_baseline and args.kind == "cg" and args.manifold == "none",
+                   dg_kernel=args.dg_kernel, manifold=args.manifold)
     info = head["info"]
     traffic = None
-    prof = os.path.join(ROOT, "profiles", f"traffic_{args.kind}_p{args.degree}_{args.precision}_n{args.n}.json")
+    curved = "" if args.manifold == "none" else f"_{args.manifold}"
+    prof = os.path.join(ROOT, "profiles",
+                        f"traffic_{args.kind}_p{args.degree}_{args.precision}_n{args.n}{curved}.json")
     if os.path.exists(prof):
         with open(prof) as f:
             traffic = json.load(f).get("bytes_per_vmult")
@@ -394,7 +399,7 @@ def main():
                              if k in ("kind", "degree", "precision", "n", "reorder", "manifold", "dofs", "ms_per_step",
                                       "gdofs", "achieved_gbs", "frac")})
     base = None
-    if args.cpu_baseline:
+    if args.cpu_baseline and args.manifold == "none":  # the oracle baseline covers affine cells
         log("cpu baseline ...")
         if args.kind == "cg":
             base = cpu_baseline(head["op"], head["x"], args.cpu_seconds)
@@ -416,7 +421,8 @@ def main():
         "data": "synthetic: seeded perturbed Kuhn mesh, x ~ U[-1,1]",
         "config": {
             "workload": f"{WORKLOAD[args.kind]} P{args.degree} vmult, {args.n}^3 x 6 perturbed "
-                        f"Kuhn tets ({head['cells']} cells, {head['dofs']} DoFs), {prec.upper()}",
+                        f"Kuhn tets ({head['cells']} cells, {head['dofs']} DoFs), {prec.upper()}"
+                        + ("" if args.manifold == "none" else f", curved cells ({args.manifold} manifold)"),
             "kind": args.kind, "degree": args.degree, "n": args.n, "cells": head["cells"],
             "dofs": head["dofs"], "reorder": not args.no_reorder, "parallelism": "single GPU",
         },
