@@ -40,6 +40,8 @@ void make_cg_shape(CGPlan& pl) {
       for (int d = 0; d < 3; ++d) pl.shape[(k * pl.n_loc + i) * 3 + d] = grads[i * 3 + d];
     pl.shape[pl.n_loc * pl.n_q * 3 + k] = q.w[k];
   }
+  // curved cells: the kernels also read the points, after the weights
+  if (pl.curved) pl.shape.insert(pl.shape.end(), pl.quad_pts.begin(), pl.quad_pts.end());
 }
 
 void upload_cg(Operator& op) {
@@ -54,8 +56,8 @@ void upload_cg(Operator& op) {
   op.d_gfix.upload(pl.gfix);
   op.d_lidx.upload(pl.lidx);
   op.d_partial.alloc(std::max<int64_t>(pl.n_ghost_total, 1) * op.ncomp * op.wbytes());
-  // curved cells: the kernels read the per-quadrature-point metric
-  const std::vector<double>& metric = pl.curved ? pl.qmetric : pl.metric;
+  // curved cells: the kernels read the 27 Jacobian coefficients per cell
+  const std::vector<double>& metric = pl.curved ? pl.jgeo : pl.metric;
   if (op.precision == MF_F64) {
     op.d_metric.upload(metric);
     op.d_shape.upload(pl.shape);

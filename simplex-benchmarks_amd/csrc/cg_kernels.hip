@@ -303,16 +303,19 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     // P1/P2: compile-time tables (literal constants after unrolling); P3
     // has too many distinct values for scalar registers and reads the table
     // from memory one quadrature point at a time.  Curved cells (FEAT bit 4)
-    // read the gradient table of the degree-2p rule the same way and the
-    // metric w_q |det J_q| J_q^-1 J_q^-T per point, the next point's metric
-    // loaded while the current one is applied (the general D_e branch,
-    // CG/cells_batched/application.cc:1361-1376).
+    // read the gradient table of the degree-2p rule the same way and form
+    // the metric w_q |det J_q| J_q^-1 J_q^-T per point in registers from the
+    // cell's 27 coefficients of J(xi) = C + sum_f xi_f D_f (a P2 map has an
+    // affine Jacobian): 216 B per cell instead of 48 B per point (the
+    // general D_e branch, CG/cells_batched/application.cc:1361-1376).
     if constexpr ((FEAT & 4) != 0) {
       constexpr int NQC = CGShape<P>::NQC;
-      const T* qm = metric + (c0 + min(lc, nc - 1));
-      T mq[6], mn[6];
+      const T* W = shape + NQC * NL * 3;  // weights, then the points (xi_0, xi_1, xi_2) per point
+      const T* XI = W + NQC;
+      const T* jg = metric + (c0 + min(lc, nc - 1));
+      T jc[27];
 #pragma unroll
-      for (int t = 0; t < 6; ++t) mq[t] = qm[(int64_t)t * ncells_total];
+      for (int t = 0; t < 27; ++t) jc[t] = jg[(int64_t)t * ncells_total];
       T u[NL];
       if (MODE == 0) {
 #pragma unroll
@@ -320,9 +323,26 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
       }
 #pragma unroll 1
       for (int q = 0; q < NQC; ++q) {
-        const int qn = q + 1 < NQC ? q + 1 : q;
+        const T x0 = XI[3 * q], x1 = XI[3 * q + 1], x2 = XI[3 * q + 2];
+        // J[d][e] = C_e + xi_e E_e + sum_{f != e} xi_f F_ef
+        T J[3][3];
 #pragma unroll
-        for (int t = 0; t < 6; ++t) mn[t] = qm[(int64_t)(qn * 6 + t) * ncells_total];
+        for (int d = 0; d < 3; ++d) {
+          J[d][0] = jc[d] + x0 * jc[9 + d] + x1 * jc[18 + d] + x2 * jc[21 + d];
+          J[d][1] = jc[3 + d] + x1 * jc[12 + d] + x0 * jc[18 + d] + x2 * jc[24 + d];
+          J[d][2] = jc[6 + d] + x2 * jc[15 + d] + x0 * jc[21 + d] + x1 * jc[24 + d];
+        }
+        // adj = det J^-1; metric w |det J| J^-1 J^-T = (w / det) adj adj^T
+        const T a00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], a01 = J[0][2] * J[2][1] - J[0][1] * J[2][2],
+                a02 = J[0][1] * J[1][2] - J[0][2] * J[1][1], a10 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                a11 = J[0][0] * J[2][2] - J[0][2] * J[2][0], a12 = J[0][2] * J[1][0] - J[0][0] * J[1][2],
+                a20 = J[1][0] * J[2][1] - J[1][1] * J[2][0], a21 = J[0][1] * J[2][0] - J[0][0] * J[2][1],
+                a22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        const T det = J[0][0] * a00 + J[0][1] * a10 + J[0][2] * a20;
+        const T sc = W[q] / det;
+        const T mq0 = sc * (a00 * a00 + a01 * a01 + a02 * a02), mq1 = sc * (a00 * a10 + a01 * a11 + a02 * a12),
+                mq2 = sc * (a00 * a20 + a01 * a21 + a02 * a22), mq3 = sc * (a10 * a10 + a11 * a11 + a12 * a12),
+                mq4 = sc * (a10 * a20 + a11 * a21 + a12 * a22), mq5 = sc * (a20 * a20 + a21 * a21 + a22 * a22);
         const T* G = shape + q * NL * 3;
         if (MODE == 0) {
           T gx = T(0), gy = T(0), gz = T(0);
@@ -332,21 +352,19 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
             gy += G[i * 3 + 1] * u[i];
             gz += G[i * 3 + 2] * u[i];
           }
-          const T tx = mq[0] * gx + mq[1] * gy + mq[2] * gz;
-          const T ty = mq[1] * gx + mq[3] * gy + mq[4] * gz;
-          const T tz = mq[2] * gx + mq[4] * gy + mq[5] * gz;
+          const T tx = mq0 * gx + mq1 * gy + mq2 * gz;
+          const T ty = mq1 * gx + mq3 * gy + mq4 * gz;
+          const T tz = mq2 * gx + mq4 * gy + mq5 * gz;
 #pragma unroll
           for (int i = 0; i < NL; ++i) y[i] += G[i * 3 + 0] * tx + G[i * 3 + 1] * ty + G[i * 3 + 2] * tz;
         } else {
 #pragma unroll
           for (int i = 0; i < NL; ++i) {
             const T gx = G[i * 3 + 0], gy = G[i * 3 + 1], gz = G[i * 3 + 2];
-            y[i] += gx * (mq[0] * gx + mq[1] * gy + mq[2] * gz) + gy * (mq[1] * gx + mq[3] * gy + mq[4] * gz) +
-                    gz * (mq[2] * gx + mq[4] * gy + mq[5] * gz);
+            y[i] += gx * (mq0 * gx + mq1 * gy + mq2 * gz) + gy * (mq1 * gx + mq3 * gy + mq4 * gz) +
+                    gz * (mq2 * gx + mq4 * gy + mq5 * gz);
           }
         }
-#pragma unroll
-        for (int t = 0; t < 6; ++t) mq[t] = mn[t];
       }
     } else if constexpr (P == 2 && MODE == 0) {
       T u[NL];

@@ -67,6 +67,7 @@ void build_curved_geometry(const Mesh& mesh, const mf_options& opt, CGPlan& plan
   }
   plan.curved = true;
   plan.qmetric.assign((size_t)nq * 6 * nc, 0.0);
+  plan.jgeo.assign((size_t)27 * nc, 0.0);
   const int ab[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
   // each support point is written by the first cell of its DoF, so the
   // result does not depend on the thread schedule
@@ -85,12 +86,55 @@ void build_curved_geometry(const Mesh& mesh, const mf_options& opt, CGPlan& plan
       for (int d = 0; d < 3; ++d) X[v][d] = mesh.x[3 * (int64_t)mesh.cells[4 * c + v] + d];
     for (int e = 0; e < 6; ++e)
       edge_node(X[kEdgeVerts[e][0]], X[kEdgeVerts[e][1]], opt.manifold, opt.manifold_center, X[4 + e]);
+    // J(xi) = C + sum_f xi_f D_f.  With lambda = (1 - xi_1 - xi_2 - xi_3,
+    // xi_1, xi_2, xi_3), dx/dlambda_a = 4 sum_b N_ab lambda_b - X_a with
+    // N_aa = X_a and N_ab the edge node of (a, b); J column e (vertex e + 1)
+    // is dx/dlambda_{e+1} - dx/dlambda_0, so
+    //   C_e  = 4 X_0e - 3 X_0 - X_e,
+    //   D_e column e = 4 (X_e + X_0 - 2 X_0e),
+    //   D_f column e = 4 (X_ef - X_0e - X_0f + X_0)  (e != f).
+    // Straight edges give C_e = X_e - X_0 and D = 0.
+    {
+      const int e0[3] = {4, 5, 6};                 // edge nodes (0,1) (0,2) (0,3)
+      const int pair[3][3] = {{0, 1, 7}, {0, 2, 8}, {1, 2, 9}};  // (e, f, node of edge (e+1, f+1))
+      for (int e = 0; e < 3; ++e)
+        for (int d = 0; d < 3; ++d) {
+          plan.jgeo[(size_t)(3 * e + d) * nc + k] = 4 * X[e0[e]][d] - 3 * X[0][d] - X[e + 1][d];
+          plan.jgeo[(size_t)(9 + 3 * e + d) * nc + k] = 4 * (X[e + 1][d] + X[0][d] - 2 * X[e0[e]][d]);
+        }
+      for (int i = 0; i < 3; ++i)
+        for (int d = 0; d < 3; ++d) {
+          const int e = pair[i][0], f = pair[i][1], ef = pair[i][2];
+          plan.jgeo[(size_t)(18 + 3 * i + d) * nc + k] = 4 * (X[ef][d] - X[e0[e]][d] - X[e0[f]][d] + X[0][d]);
+        }
+    }
     for (int qq = 0; qq < nq; ++qq) {
       // J[d][e] = sum_i X_i[d] d(phi_i)/d(xi_e)
       double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
       for (int i = 0; i < 10; ++i)
         for (int d = 0; d < 3; ++d)
           for (int e = 0; e < 3; ++e) J[d][e] += X[i][d] * gg[((size_t)qq * 10 + i) * 3 + e];
+      // set-up self-check of the kernels' affine form of J against the basis form
+      {
+        const double* xi = q.pts[qq].data();
+        double dev = 0, mag = 0;
+        for (int e = 0; e < 3; ++e)
+          for (int d = 0; d < 3; ++d) {
+            auto g = [&](int t) { return plan.jgeo[(size_t)t * nc + k]; };
+            double v = g(3 * e + d) + xi[e] * g(9 + 3 * e + d);
+            for (int i = 0; i < 3; ++i) {
+              const int a = i == 2 ? 1 : 0, b = i == 0 ? 1 : 2;  // pairs (0,1) (0,2) (1,2)
+              if (e == a) v += xi[b] * g(18 + 3 * i + d);
+              if (e == b) v += xi[a] * g(18 + 3 * i + d);
+            }
+            dev = std::fmax(dev, std::fabs(v - J[d][e]));
+            mag = std::fmax(mag, std::fabs(J[d][e]));
+          }
+        if (dev > 1e-12 * mag) {
+#pragma omp atomic write
+          bad = 2;
+        }
+      }
       const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
                    c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
       const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
@@ -126,6 +170,7 @@ void build_curved_geometry(const Mesh& mesh, const mf_options& opt, CGPlan& plan
       }
     }
   }
+  if (bad == 2) throw std::runtime_error("curved cells: affine Jacobian form disagrees with the P2 map");
   if (bad) throw std::runtime_error("curved cells: a quadrature point with non-positive Jacobian determinant");
 }
 

@@ -113,6 +113,11 @@ struct CGPlan {
   // which shape then holds; metric keeps the straight-cell values
   bool curved = false;
   std::vector<double> qmetric;
+  // the same geometry in the kernels' form: J(xi) = C + sum_f xi_f D_f is
+  // affine in xi for a P2 map; jgeo[t][n_cells], t < 27: C columns e = 0..2
+  // (t = 3e + d), then E_e = D_e column e (9 + 3e + d), then F_ef = D_f
+  // column e = D_e column f for (e, f) = (0,1), (0,2), (1,2) (18 + 3i + d)
+  std::vector<double> jgeo;
 };
 
 // Host-side plan of a discontinuous-Galerkin (SIPG) operator.  Cell local
