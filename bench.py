@@ -293,7 +293,7 @@ def matrix_compare(smf, n, degree, precision, steps, seed):
                     "the CSR matrices are not"}
 
 
-def external_compare(smf, n, steps, seed):
+def external_compare(smf, n, steps, seed, solve_n=0):
     """Operators on a mesh in an unstructured input order (the Kuhn cube with
     vertices and cells randomly permuted, as an external mesh would arrive),
     with the hierarchical reordering on and off: the reference's reorder_grid
@@ -313,8 +313,28 @@ def external_compare(smf, n, steps, seed):
                         "dofs": op.m(), "ms_per_step": round(t * 1e3, 5), "gdofs": round(op.m() / t / 1e9, 4),
                         "frac": round(op.info["algorithmic_bytes"] / t / 1e9 / HBM_PEAK_GBS, 4)})
             del op, src, dst
-    return {"mesh": f"{n}^3 x 6 perturbed Kuhn tets with vertices and cells in a seeded random order, "
-                    "local vertices rotated", "runs": out}
+    res = {"mesh": f"{n}^3 x 6 perturbed Kuhn tets with vertices and cells in a seeded random order, "
+                   "local vertices rotated", "runs": out}
+    if solve_n > 0:
+        # MG-CG to 1e-10 without a lattice: p-levels P2 -> P1 on the mesh itself,
+        # then the smoothed-aggregation AMG coarse solver on the P1 level
+        # (the reference's MGCoarseAMG, application_hp.cc:1684-1731)
+        emesh = smf.Mesh.kuhn_cube(solve_n, 0.2, seed).scrambled(seed=seed + 1)
+        op = smf.Operator(emesh, 2)
+        t0 = time.time()
+        mg = smf.Multigrid(op)
+        setup = time.time() - t0
+        b = smf.DeviceVector.from_numpy(smf.assemble_rhs(op, "grf", seed=seed, n_features=256, corr_length=0.1))
+        x = op.initialize_dof_vector()
+        stream = smf.Stream()
+        st, times = _time_solves(smf, lambda: smf.solve_cg(op, x, b, mg, rel_tol=1e-10, stream=stream), 5)
+        inf = mg.info()
+        res["solve"] = {"workload": f"MG-CG Poisson P2 on the scrambled {solve_n}^3 mesh, FP64 CG + FP32 V-cycle, "
+                                    "AMG coarse solve (one FP64 V-cycle), GRF right-hand side, stop at 1e-10",
+                        "dofs": op.m(), "levels": inf["n_dofs"], "amg_rows": inf["amg_rows"],
+                        "setup_s": round(setup, 3), "time_s": min(times), **_rates(st, min(times), op.m())}
+        log(f"external MG-CG {solve_n}^3: {st['iterations']} it, {min(times) * 1e3:.2f} ms, AMG {inf['amg_rows']}")
+    return res
 
 
 def cpu_baseline_dg(smf, degree, precision_unused, seconds, seed):
@@ -463,7 +483,8 @@ def main():
                                for deg, m in ((1, 2 * args.matrix_n), (2, args.matrix_n),
                                               (3, max(1, args.matrix_n // 2)))]
     if args.external_n > 0:
-        out["external_mesh"] = external_compare(smf, args.external_n, max(10, args.steps // 2), args.seed)
+        out["external_mesh"] = external_compare(smf, args.external_n, max(10, args.steps // 2), args.seed,
+                                                solve_n=args.solve_n)
     if args.solve_n > 0:
         out["solve"] = solve_compare(smf, args.solve_n, 2, args.seed, args.solves)
         out["solve_dg"] = solve_compare(smf, args.solve_n, 2, args.seed, args.solves, kind="dg")

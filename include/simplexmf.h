@@ -171,6 +171,11 @@ typedef struct mf_mg_options {
   int32_t fuse_steps;         /* 1 (default): scalar CG levels run the Chebyshev steps without a
                                  fix-up launch per step (interface rows completed by the next step);
                                  0: one operator application + fix-up per step */
+  int32_t coarse_solver;      /* 0 (default): FP64 dense inverse when the coarsest level has at most
+                                 max_coarse_dofs rows, else one smoothed-aggregation AMG V-cycle on its
+                                 assembled FP64 matrix (the reference's MGCoarseAMG, one ML V-cycle,
+                                 application_hp.cc:1684-1731; needed on meshes without a Kuhn lattice);
+                                 1: dense only; 2: AMG always */
 } mf_mg_options;
 
 #define MF_MAX_LEVELS 16
@@ -183,6 +188,8 @@ typedef struct mf_mg_info {
   int64_t vcycles;                      /* V-cycles applied so far */
   int32_t level_kind[MF_MAX_LEVELS];    /* MF_CG_POISSON or MF_DG_SIPG */
   int32_t nitsche[MF_MAX_LEVELS];       /* 1: CG level with the Nitsche boundary */
+  int32_t amg_levels;                   /* AMG coarse solver: its levels (0: dense coarse solve) */
+  int64_t amg_rows[MF_MAX_LEVELS];      /* rows per AMG level, [0] = the coarsest geometric level */
 } mf_mg_info;
 
 typedef struct mf_solve_stats {

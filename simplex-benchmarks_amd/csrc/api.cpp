@@ -6,6 +6,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "amg.h"
 #include "api_handles.h"
 #include "mg.h"
 #include "op.h"
@@ -479,6 +480,7 @@ void mf_mg_default_options(mf_mg_options* opt) {
   opt->level_precision = d.level_precision;
   opt->max_coarse_dofs = d.max_coarse_dofs;
   opt->fuse_steps = d.fuse_steps;
+  opt->coarse_solver = d.coarse_solver;
 }
 
 int mf_mg_create(mf_op_t fine, const mf_mg_options* opt, mf_mg_t* out) {
@@ -494,6 +496,8 @@ int mf_mg_create(mf_op_t fine, const mf_mg_options* opt, mf_mg_t* out) {
       o.level_precision = opt->level_precision;
       o.max_coarse_dofs = opt->max_coarse_dofs;
       o.fuse_steps = opt->fuse_steps;
+      o.coarse_solver = opt->coarse_solver;
+      if (o.coarse_solver < 0 || o.coarse_solver > 2) throw std::runtime_error("coarse_solver must be 0, 1 or 2");
     }
     if (o.smoother_degree < 1 || o.smoothing_range <= 1 || o.eig_cg_iterations < 1)
       throw std::runtime_error("invalid multigrid options");
@@ -527,6 +531,11 @@ int mf_mg_get_info(mf_mg_t mg, mf_mg_info* info) {
       info->nitsche[l] = !L[l].op->is_dg() && L[l].op->cg.nitsche;
     }
     info->vcycles = mg->mg->vcycles;
+    if (mg->mg->amg) {
+      const auto& lv = mg->mg->amg->lv;
+      info->amg_levels = (int32_t)std::min<size_t>(lv.size(), MF_MAX_LEVELS);
+      for (int l = 0; l < info->amg_levels; ++l) info->amg_rows[l] = lv[l].A.nrows;
+    }
   });
 }
 

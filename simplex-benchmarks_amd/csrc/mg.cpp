@@ -8,6 +8,9 @@
 #include <cstring>
 #include <stdexcept>
 
+#include "amg.h"
+#include "matrix.h"
+
 namespace smf {
 
 void launch_invert_diagonal(int precision, int64_t n, void* d, hipStream_t s);
@@ -24,6 +27,8 @@ void Csr::upload(int precision) {
     d_val.upload(v);
   }
 }
+
+std::vector<double> dense_inverse_spd(std::vector<double> A, int64_t n);
 
 namespace {
 
@@ -277,6 +282,13 @@ std::vector<double> dense_inverse(const Operator& op) {
   if (pl.nitsche)
     for (size_t r = 0; r < pl.bnd_rows.size(); ++r)
       for (int32_t k = pl.bnd_ptr[r]; k < pl.bnd_ptr[r + 1]; ++k) A[pl.bnd_rows[r] * n + pl.bnd_col[k]] += pl.bnd_val[k];
+  return dense_inverse_spd(std::move(A), n);
+}
+
+}  // namespace
+
+// inverse of a dense symmetric positive definite row-major n x n matrix
+std::vector<double> dense_inverse_spd(std::vector<double> A, int64_t n) {
   // Cholesky A = L L^T (in place, lower; rows of a column in parallel)
   for (int64_t j = 0; j < n; ++j) {
     double s = A[j * n + j];
@@ -318,6 +330,8 @@ std::vector<double> dense_inverse(const Operator& op) {
     for (int64_t i = 0; i < n; ++i) inv[i * n + c] = invt[c * n + i];
   return inv;
 }
+
+namespace {
 
 // largest eigenvalue of a symmetric tridiagonal matrix by Sturm bisection
 double tridiag_max_eig(const std::vector<double>& a, const std::vector<double>& b) {
@@ -487,6 +501,14 @@ void vcycle(Multigrid& mg, int l, hipStream_t s) {
   const int64_t n = L.op->m();
   T* x = L.x.as<T>();
   T* b = L.b.as<T>();
+  if (l == 0 && mg.amg) {
+    // one FP64 AMG V-cycle
+    Amg& amg = *mg.amg;
+    blas_convert<double, T>(n, b, amg.lv[0].b.as<double>(), s);
+    amg_vcycle(amg, s);
+    blas_convert<T, double>(n, amg.lv[0].x.as<double>(), x, s);
+    return;
+  }
   if (l == 0) {
     // FP64 dense coarse solve
     blas_convert<double, T>(n, b, mg.d_cb.as<double>(), s);
@@ -564,8 +586,10 @@ std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions
       kn.push_back(n);
     }
   }
-  if (ops.back()->m() > opt.max_coarse_dofs)
-    throw std::runtime_error("coarsest level too large for the dense coarse solver (needs a Kuhn mesh hierarchy)");
+  const bool use_amg = opt.coarse_solver == 2 || (opt.coarse_solver == 0 && ops.back()->m() > opt.max_coarse_dofs);
+  if (!use_amg && ops.back()->m() > opt.max_coarse_dofs)
+    throw std::runtime_error("coarsest level too large for the dense coarse solver (needs a Kuhn mesh hierarchy "
+                             "or the AMG coarse solver)");
   // assemble levels coarsest first
   const size_t nl = ops.size();
   mg->levels.resize(nl);
@@ -613,12 +637,28 @@ std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions
   }
   // coarse solver
   const Operator& c0 = *mg->levels[0].op;
+  mg->n_coarse = (int)c0.m();
+  mg->d_cb.alloc(std::max<int64_t>(c0.m(), 1) * sizeof(double));
+  mg->d_cx.alloc(std::max<int64_t>(c0.m(), 1) * sizeof(double));
+  if (use_amg) {
+    // one AMG V-cycle on the coarsest level's assembled FP64 matrix
+    if (c0.is_dg()) throw std::runtime_error("AMG coarse solver: the coarsest level must be a CG level");
+    SysMatrix sm;
+    assemble_cg_matrix(c0, sm);
+    Csr A0;
+    A0.nrows = A0.ncols = sm.n_rows;
+    A0.ptr = std::move(sm.ptr);
+    A0.col = std::move(sm.col);
+    A0.val = std::move(sm.val);
+    mg->amg = build_amg(A0, opt.max_coarse_dofs);
+    SMF_HIP(hipDeviceSynchronize());
+    return mg;
+  }
   std::vector<double> inv;
   {
     // the dense inverse needs the FP64 element data of the coarsest level
     inv = dense_inverse(c0);
   }
-  mg->n_coarse = (int)c0.m();
   // strong Dirichlet rows make A = [[A_ff, 0], [0, I]] (free DoFs first), so
   // only the leading block of the inverse is kept
   const int64_t nf = !c0.is_dg() && !c0.cg.nitsche ? c0.cg.n_free : c0.m();
@@ -630,8 +670,6 @@ std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions
     inv.resize(nf * nf);
   }
   mg->d_coarse_inv.upload(inv);
-  mg->d_cb.alloc(mg->n_coarse * sizeof(double));
-  mg->d_cx.alloc(mg->n_coarse * sizeof(double));
   SMF_HIP(hipDeviceSynchronize());
   return mg;
 }

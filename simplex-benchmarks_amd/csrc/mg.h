@@ -42,7 +42,13 @@ struct MGOptions {
   int level_precision = MF_F32;
   int max_coarse_dofs = 4096;
   int fuse_steps = 1;  // scalar CG levels: Chebyshev steps without per-step fix-up launches
+  // 0: dense FP64 inverse when the coarsest level has at most max_coarse_dofs
+  // rows, else one smoothed-aggregation AMG V-cycle on its assembled matrix;
+  // 1: dense only; 2: AMG always
+  int coarse_solver = 0;
 };
+
+struct Amg;
 
 struct Multigrid {
   MGOptions opt;
@@ -52,6 +58,7 @@ struct Multigrid {
   // coarse solver: dense FP64 inverse of the coarsest operator
   int n_coarse = 0, n_coarse_free = 0;  // dense block: the leading n_coarse_free rows (the rest: identity)
   DevBuf d_coarse_inv, d_cb, d_cx;
+  std::shared_ptr<Amg> amg;  // AMG coarse solver (instead of the dense inverse)
   DevBuf d_partial, d_scalar;  // reductions
   int64_t vcycles = 0;
 };

@@ -155,7 +155,7 @@ def lib():
 class MGOptions(C.Structure):
     _fields_ = [("smoother_degree", C.c_int32), ("smoothing_range", C.c_double),
                 ("eig_cg_iterations", C.c_int32), ("level_precision", C.c_int32),
-                ("max_coarse_dofs", C.c_int32), ("fuse_steps", C.c_int32)]
+                ("max_coarse_dofs", C.c_int32), ("fuse_steps", C.c_int32), ("coarse_solver", C.c_int32)]
 
 
 MF_MAX_LEVELS = 16
@@ -165,7 +165,8 @@ class MGInfo(C.Structure):
     _fields_ = [("n_levels", C.c_int32), ("degree", C.c_int32 * MF_MAX_LEVELS),
                 ("kuhn_n", C.c_int32 * MF_MAX_LEVELS), ("n_dofs", C.c_int64 * MF_MAX_LEVELS),
                 ("lambda_max", C.c_double * MF_MAX_LEVELS), ("vcycles", C.c_int64),
-                ("level_kind", C.c_int32 * MF_MAX_LEVELS), ("nitsche", C.c_int32 * MF_MAX_LEVELS)]
+                ("level_kind", C.c_int32 * MF_MAX_LEVELS), ("nitsche", C.c_int32 * MF_MAX_LEVELS),
+                ("amg_levels", C.c_int32), ("amg_rows", C.c_int64 * MF_MAX_LEVELS)]
 
 
 class SolveStats(C.Structure):
@@ -593,7 +594,7 @@ class Multigrid:
     (MultigridPreconditioner of CG/cells_batched/application_ph.cc:1754-2026)."""
 
     def __init__(self, fine: Operator, smoother_degree=5, smoothing_range=20.0, eig_cg_iterations=10,
-                 level_precision="f32", max_coarse_dofs=4096, fuse_steps=True):
+                 level_precision="f32", max_coarse_dofs=4096, fuse_steps=True, coarse_solver="auto"):
         L = lib()
         opt = MGOptions()
         L.mf_mg_default_options(C.byref(opt))
@@ -603,6 +604,8 @@ class Multigrid:
         opt.level_precision = PRECISIONS[level_precision] if isinstance(level_precision, str) else level_precision
         opt.max_coarse_dofs = max_coarse_dofs
         opt.fuse_steps = 1 if fuse_steps else 0
+        opt.coarse_solver = {"auto": 0, "dense": 1, "amg": 2}[coarse_solver] if isinstance(coarse_solver, str) \
+            else int(coarse_solver)
         self.level_dtype = NP_DTYPES[opt.level_precision]
         self.fine = fine
         h = C.c_void_p()
@@ -620,7 +623,8 @@ class Multigrid:
         n = inf.n_levels
         return {"n_levels": n, "degree": list(inf.degree[:n]), "kuhn_n": list(inf.kuhn_n[:n]),
                 "n_dofs": list(inf.n_dofs[:n]), "lambda_max": list(inf.lambda_max[:n]),
-                "vcycles": inf.vcycles, "level_kind": list(inf.level_kind[:n]), "nitsche": list(inf.nitsche[:n])}
+                "vcycles": inf.vcycles, "level_kind": list(inf.level_kind[:n]), "nitsche": list(inf.nitsche[:n]),
+                "amg_rows": list(inf.amg_rows[:inf.amg_levels])}
 
     def vmult(self, dst, src, stream=None):
         check(lib().mf_mg_vmult(self.handle, dst.ptr, src.ptr, stream.handle if stream else None))
