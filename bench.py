@@ -405,7 +405,7 @@ def main():
                 ("dg", 64, 2, "f32", True), ("dg", 64, 3, "f32", True), ("dg", 64, 2, "f64", False),
                 ("dg_helmholtz", 32, 2, "f64", True), ("dg_helmholtz", 32, 2, "f32", True),
                 ("cg", 128, 2, "f64", True), ("cg", 128, 2, "f32", True), ("cg_vector", 64, 2, "f64", True),
-                ("cg", 128, 2, "f64", "spherical")]
+                ("cg", 128, 2, "f64", "spherical"), ("cg", 16, 2, "f64", True), ("cg", 48, 2, "f64", True)]
         for kind, n, deg, prec, reorder in todo:
             # a string in the last slot: curved cells on that manifold (reordering on)
             manifold = reorder if isinstance(reorder, str) else "none"
@@ -417,13 +417,15 @@ def main():
                         manifold=manifold)
             r["reorder"] = reorder
             r["manifold"] = manifold
+            # operator data below the 256 MB Infinity Cache: not an HBM figure
+            r["cache_resident"] = r["algorithmic_bytes"] < 256e6
             if manifold != "none":
                 r["note"] = ("frac uses the reference data model for curved cells (J^-T + JxW per quadrature "
                              "point); the kernel forms them in registers from 27 values per cell, so it reads "
                              "far fewer bytes and frac can exceed 1")
             variants.append({k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()
                              if k in ("kind", "degree", "precision", "n", "reorder", "manifold", "dofs", "ms_per_step",
-                                      "gdofs", "achieved_gbs", "frac", "note")})
+                                      "gdofs", "achieved_gbs", "frac", "note", "cache_resident")})
     base = None
     if args.cpu_baseline and args.manifold == "none":  # the oracle baseline covers affine cells
         log("cpu baseline ...")
