@@ -18,8 +18,9 @@
 // Coarsening stops at max_dense rows (dense FP64 inverse) or when a level
 // shrinks by less than 10%.
 //
-// Solve on the device: a V-cycle with Chebyshev-Jacobi smoothing (degree 2,
-// range 30, the same recurrence as the geometric levels) and CSR products.
+// Solve on the device: a V-cycle with Chebyshev-Jacobi smoothing (the
+// geometric levels' degree, range 30, the same recurrence) and CSR-stream
+// products (row-per-thread CSR for a level whose rows exceed a row block).
 #include "amg.h"
 
 #include <algorithm>
@@ -201,7 +202,8 @@ void vcycle_level(Amg& amg, size_t l, hipStream_t s) {
     return;
   }
   auto apply = [&](double* y, const double* x) {
-    blas_csr<double>(n, L.A.d_ptr.as<int64_t>(), L.A.d_col.as<int32_t>(), L.A.d_val.as<double>(), x, y, false, s);
+    if (L.M.on_device) launch_spmv(L.M, y, x, s);
+    else blas_csr<double>(n, L.A.d_ptr.as<int64_t>(), L.A.d_col.as<int32_t>(), L.A.d_val.as<double>(), x, y, false, s);
   };
   AmgLevel& C = amg.lv[l + 1];
   cheby(amg, L, true, s, apply);
@@ -306,7 +308,13 @@ std::unique_ptr<Amg> build_amg(const Csr& A0, int max_dense) {
   // device data
   for (auto& L : amg->lv) {
     const int64_t n = L.A.nrows;
-    L.A.upload(MF_F64);
+    L.M.precision = MF_F64;
+    L.M.n_rows = n;
+    L.M.ptr = L.A.ptr;
+    L.M.col = L.A.col;
+    L.M.val = L.A.val;
+    if (n > 0 && make_row_blocks(L.M)) upload_matrix(L.M);
+    else L.A.upload(MF_F64);
     if (!L.P.ptr.empty()) {
       L.P.upload(MF_F64);
       L.R.upload(MF_F64);

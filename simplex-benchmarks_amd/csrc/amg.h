@@ -9,12 +9,14 @@
 #include <memory>
 #include <vector>
 
+#include "matrix.h"
 #include "mg.h"
 
 namespace smf {
 
 struct AmgLevel {
   Csr A;                     // this level's matrix (FP64)
+  SysMatrix M;               // the same for the CSR-stream product (when every row fits a row block)
   Csr P, R;                  // prolongation from the next coarser level and R = P^T
   DevBuf dinv, x, b, r, d, t;  // FP64 level vectors
   double lambda_max = 0;     // of D^-1 A (Chebyshev upper bound, 1.1 x estimate)
@@ -25,6 +27,7 @@ struct Amg {
   int n_dense = 0;           // the last level is solved with a dense FP64 inverse
   DevBuf d_inv;
   int smoother_degree = 2;   // Chebyshev-Jacobi steps before and after the coarse correction
+                             // (the multigrid sets its own smoother degree)
   double smoothing_range = 30.0;
 };
 

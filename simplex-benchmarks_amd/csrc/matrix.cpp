@@ -144,6 +144,7 @@ void assemble_cg_matrix(const Operator& op, SysMatrix& A) {
   }
 
   // components: block diagonal, component c in rows / columns [c n, (c+1) n)
+  // (FEM rows have at most a few hundred entries, far below a row block)
   A.precision = op.precision;
   A.n_rows = n * nc;
   A.ptr.assign(A.n_rows + 1, 0);
@@ -156,14 +157,21 @@ void assemble_cg_matrix(const Operator& op, SysMatrix& A) {
       A.val[c * snnz + e] = sval[e];
     }
   }
-  // row blocks of the product kernel: consecutive rows, greedily, up to
-  // kSpmvBlockEntries entries and kSpmvBlockRows rows
+  if (!make_row_blocks(A))
+    throw std::runtime_error("system matrix: a row has more than " + std::to_string(kSpmvBlockEntries) + " entries");
+}
+
+bool make_row_blocks(SysMatrix& A) {
+  // consecutive rows, greedily, up to kSpmvBlockEntries entries and
+  // kSpmvBlockRows rows per block
   A.blk.assign(1, 0);
   int64_t e = 0, rows = 0;
   for (int64_t r = 0; r < A.n_rows; ++r) {
     const int64_t len = A.ptr[r + 1] - A.ptr[r];
-    if (len > kSpmvBlockEntries)
-      throw std::runtime_error("system matrix: a row has more than " + std::to_string(kSpmvBlockEntries) + " entries");
+    if (len > kSpmvBlockEntries) {
+      A.blk.clear();
+      return false;
+    }
     if (e + len > kSpmvBlockEntries || rows == kSpmvBlockRows) {
       A.blk.push_back((int32_t)r);
       e = rows = 0;
@@ -172,6 +180,7 @@ void assemble_cg_matrix(const Operator& op, SysMatrix& A) {
     ++rows;
   }
   if (A.n_rows > 0) A.blk.push_back((int32_t)A.n_rows);
+  return true;
 }
 
 void upload_matrix(SysMatrix& A) {

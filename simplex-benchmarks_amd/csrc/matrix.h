@@ -33,6 +33,9 @@ struct SysMatrix {
 // boundary) in the operator's DoF numbering
 void assemble_cg_matrix(const Operator& op, SysMatrix& A);
 void upload_matrix(SysMatrix& A);
+// row blocks of the product kernel from A.ptr; false (and no blocks) when a
+// row has more than kSpmvBlockEntries entries
+bool make_row_blocks(SysMatrix& A);
 // dst = A src (device vectors in A.precision)
 void launch_spmv(const SysMatrix& A, void* dst, const void* src, hipStream_t s);
 
