@@ -70,3 +70,17 @@ def test_coarse_solver_options(gpu):
     assert abs(sa["iterations"] - sd["iterations"]) <= 3
     with pytest.raises(smf.MFError, match="dense coarse solver"):
         smf.Multigrid(smf.Operator(smf.Mesh.kuhn_cube(16, 0.2, 5).scrambled(seed=1), 2), coarse_solver="dense")
+
+
+def test_amg_coarse_under_dg(gpu):
+    """DG SIPG P2 on a scrambled mesh: DG -> CG (Nitsche) P2 -> P1, whose
+    Nitsche matrix (no identity rows) the AMG coarsens."""
+    smf = gpu
+    print("\ncase: AMG coarse solve under DG, scrambled 16^3 mesh, P2", flush=True)
+    op = smf.Operator(smf.Mesh.kuhn_cube(16, 0.2, 7).scrambled(seed=2), 2, kind="dg")
+    mg = smf.Multigrid(op)
+    info = mg.info()
+    assert info["nitsche"][0] == 1 and len(info["amg_rows"]) >= 2
+    st, res, _ = solve(smf, op, mg)
+    assert st["converged"] and st["iterations"] <= 30
+    assert res <= 2e-10
