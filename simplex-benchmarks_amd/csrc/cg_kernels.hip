@@ -303,7 +303,8 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     // P1/P2: compile-time tables (literal constants after unrolling); P3
     // has too many distinct values for scalar registers and reads the table
     // from memory one quadrature point at a time.  Curved cells (FEAT bit 4)
-    // read the gradient table of the degree-2p rule the same way and form
+    // read the gradient table of the degree-2p rule the same way (P2: the
+    // barycentric form at the point instead) and form
     // the metric w_q |det J_q| J_q^-1 J_q^-T per point in registers from the
     // cell's 27 coefficients of J(xi) = C + sum_f xi_f D_f (a P2 map has an
     // affine Jacobian): 216 B per cell instead of 48 B per point (the
@@ -344,7 +345,34 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
                 mq2 = sc * (a00 * a20 + a01 * a21 + a02 * a22), mq3 = sc * (a10 * a10 + a11 * a11 + a12 * a12),
                 mq4 = sc * (a10 * a20 + a11 * a21 + a12 * a22), mq5 = sc * (a20 * a20 + a21 * a21 + a22 * a22);
         const T* G = shape + q * NL * 3;
-        if (MODE == 0) {
+        if (P == 2 && MODE == 0) {
+          // barycentric P2 at a general point (lambda = (1 - x0 - x1 - x2, x0, x1, x2)):
+          // D_a = u_a (4 l_a - 1) + sum_{b != a} u_ab 4 l_b, reference gradient
+          // g_e = D_{e+1} - D_0; transposed with tau = (-(t0 + t1 + t2), t0, t1, t2):
+          // y_a += tau_a (4 l_a - 1), y_ab += tau_a 4 l_b + tau_b 4 l_a
+          const T l4[4] = {T(4) * (T(1) - x0 - x1 - x2), T(4) * x0, T(4) * x1, T(4) * x2};
+          constexpr int E[4][4] = {{-1, 4, 5, 6}, {4, -1, 7, 8}, {5, 7, -1, 9}, {6, 8, 9, -1}};  // node of edge (a, b)
+          T D[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            T s = u[a] * (l4[a] - T(1));
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+              if (b != a) s += u[E[a][b]] * l4[b];
+            D[a] = s;
+          }
+          const T gx = D[1] - D[0], gy = D[2] - D[0], gz = D[3] - D[0];
+          const T tx = mq0 * gx + mq1 * gy + mq2 * gz;
+          const T ty = mq1 * gx + mq3 * gy + mq4 * gz;
+          const T tz = mq2 * gx + mq4 * gy + mq5 * gz;
+          const T tau[4] = {-(tx + ty + tz), tx, ty, tz};
+#pragma unroll
+          for (int a = 0; a < 4; ++a) y[a] += tau[a] * (l4[a] - T(1));
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = a + 1; b < 4; ++b) y[E[a][b]] += tau[a] * l4[b] + tau[b] * l4[a];
+        } else if (MODE == 0) {
           T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
           for (int i = 0; i < NL; ++i) {
