@@ -341,9 +341,15 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
                 a22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
         const T det = J[0][0] * a00 + J[0][1] * a10 + J[0][2] * a20;
         const T sc = W[q] / det;
-        const T mq0 = sc * (a00 * a00 + a01 * a01 + a02 * a02), mq1 = sc * (a00 * a10 + a01 * a11 + a02 * a12),
-                mq2 = sc * (a00 * a20 + a01 * a21 + a02 * a22), mq3 = sc * (a10 * a10 + a11 * a11 + a12 * a12),
-                mq4 = sc * (a10 * a20 + a11 * a21 + a12 * a22), mq5 = sc * (a20 * a20 + a21 * a21 + a22 * a22);
+        // flux t = (w / det) adj (adj^T g) (vmult: the metric is not formed)
+        auto flux = [&](T gx, T gy, T gz, T& tx, T& ty, T& tz) {
+          const T h0 = a00 * gx + a10 * gy + a20 * gz;
+          const T h1 = a01 * gx + a11 * gy + a21 * gz;
+          const T h2 = a02 * gx + a12 * gy + a22 * gz;
+          tx = sc * (a00 * h0 + a01 * h1 + a02 * h2);
+          ty = sc * (a10 * h0 + a11 * h1 + a12 * h2);
+          tz = sc * (a20 * h0 + a21 * h1 + a22 * h2);
+        };
         const T* G = shape + q * NL * 3;
         if (P == 2 && MODE == 0) {
           // barycentric P2 at a general point (lambda = (1 - x0 - x1 - x2, x0, x1, x2)):
@@ -361,10 +367,8 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
               if (b != a) s += u[E[a][b]] * l4[b];
             D[a] = s;
           }
-          const T gx = D[1] - D[0], gy = D[2] - D[0], gz = D[3] - D[0];
-          const T tx = mq0 * gx + mq1 * gy + mq2 * gz;
-          const T ty = mq1 * gx + mq3 * gy + mq4 * gz;
-          const T tz = mq2 * gx + mq4 * gy + mq5 * gz;
+          T tx, ty, tz;
+          flux(D[1] - D[0], D[2] - D[0], D[3] - D[0], tx, ty, tz);
           const T tau[4] = {-(tx + ty + tz), tx, ty, tz};
 #pragma unroll
           for (int a = 0; a < 4; ++a) y[a] += tau[a] * (l4[a] - T(1));
@@ -380,12 +384,15 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
             gy += G[i * 3 + 1] * u[i];
             gz += G[i * 3 + 2] * u[i];
           }
-          const T tx = mq0 * gx + mq1 * gy + mq2 * gz;
-          const T ty = mq1 * gx + mq3 * gy + mq4 * gz;
-          const T tz = mq2 * gx + mq4 * gy + mq5 * gz;
+          T tx, ty, tz;
+          flux(gx, gy, gz, tx, ty, tz);
 #pragma unroll
           for (int i = 0; i < NL; ++i) y[i] += G[i * 3 + 0] * tx + G[i * 3 + 1] * ty + G[i * 3 + 2] * tz;
         } else {
+          // diagonal: the metric (w / det) adj adj^T
+          const T mq0 = sc * (a00 * a00 + a01 * a01 + a02 * a02), mq1 = sc * (a00 * a10 + a01 * a11 + a02 * a12),
+                  mq2 = sc * (a00 * a20 + a01 * a21 + a02 * a22), mq3 = sc * (a10 * a10 + a11 * a11 + a12 * a12),
+                  mq4 = sc * (a10 * a20 + a11 * a21 + a12 * a22), mq5 = sc * (a20 * a20 + a21 * a21 + a22 * a22);
 #pragma unroll
           for (int i = 0; i < NL; ++i) {
             const T gx = G[i * 3 + 0], gy = G[i * 3 + 1], gz = G[i * 3 + 2];
