@@ -38,7 +38,7 @@ def test_vmult_parity(gpu, orc, p, precision, reorder):
 
 
 @pytest.mark.parametrize("p", [1, 2, 3])
-def test_vmult_parity_cfg1(gpu, orc, p):
+def test_vmult_parity_16cube(gpu, orc, p):
     """16^3 x 6 perturbed tets (24,576 cells), x ~ U[-1,1] on all DoFs."""
     smf = gpu
     print(f"\ncase: CG vmult p={p} f64 n=16", flush=True)

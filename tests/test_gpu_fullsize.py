@@ -1,7 +1,6 @@
-"""Parity at the benchmark configurations' full sizes, through the C ABI:
-cfg0 CG P2 16^3, cfg1 DG SIPG 64^3 (P1, P2; P3 by size-independent
-properties), cfg2 Helmholtz-like vector DG P2 32^3, cfg3 MG-preconditioned
-CG on CG P2 48^3.  The oracle runs with OpenMP on the host (seconds to tens
+"""Parity at the benchmark workloads' full sizes, through the C ABI:
+CG P2 16^3, DG SIPG 64^3 (P1, P2; P3 by size-independent properties),
+Helmholtz-like vector DG P2 32^3, MG-preconditioned CG on CG P2 48^3.  The oracle runs with OpenMP on the host (seconds to tens
 of seconds per case)."""
 import numpy as np
 import pytest
@@ -18,7 +17,7 @@ def gpu_vmult(smf, op, x):
 
 
 @pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32", 1e-5)])
-def test_cfg0_cg_p2_16cube(gpu, orc, precision, tol):
+def test_cg_p2_16cube(gpu, orc, precision, tol):
     smf = gpu
     op = smf.Operator(smf.Mesh.kuhn_cube(16, 0.2, 7), 2, kind="cg", precision=precision)
     assert op.m() == 35937 and op.info["n_constrained"] == 6146
@@ -28,7 +27,7 @@ def test_cfg0_cg_p2_16cube(gpu, orc, precision, tol):
 
 
 @pytest.mark.parametrize("p", [1, 2])
-def test_cfg1_dg_64cube_vs_oracle(gpu, orc, p):
+def test_dg_64cube_vs_oracle(gpu, orc, p):
     smf = gpu
     print(f"\ncase: dg 64^3 p={p}", flush=True)
     op = smf.Operator(smf.Mesh.kuhn_cube(64, 0.2, 7), p, kind="dg", precision="f64")
@@ -38,7 +37,7 @@ def test_cfg1_dg_64cube_vs_oracle(gpu, orc, p):
     assert rel_err(y, want) < 1e-12
 
 
-def test_cfg1_dg_64cube_properties(gpu):
+def test_dg_64cube_properties(gpu):
     """Size-independent checks at 64^3 for P2 (FP64, FP32) and P3 (FP64):
     bitwise determinism, symmetry u.Av = v.Au, zero rows for a globally
     linear function on cells without a boundary face, and agreement of the
@@ -72,7 +71,7 @@ def test_cfg1_dg_64cube_properties(gpu):
 
 
 @pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32", 1e-5)])
-def test_cfg2_helmholtz_32cube(gpu, orc, precision, tol):
+def test_helmholtz_32cube(gpu, orc, precision, tol):
     smf = gpu
     print(f"\ncase: helmholtz 32^3 {precision}", flush=True)
     op = smf.Operator(smf.Mesh.kuhn_cube(32, 0.2, 7), 2, kind="dg_helmholtz", precision=precision,
@@ -83,7 +82,7 @@ def test_cfg2_helmholtz_32cube(gpu, orc, precision, tol):
     assert rel_err(gpu_vmult(smf, op, x), want) < tol
 
 
-def test_cfg3_mg_cg_48cube(gpu, orc):
+def test_mg_cg_48cube(gpu, orc):
     """Mixed-precision hybrid-MG CG on CG P2 48^3 (912,673 DoFs): the
     computed solution's residual with the oracle's operator."""
     smf = gpu

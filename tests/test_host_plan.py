@@ -55,7 +55,7 @@ def test_support_points_consistent(smf, orc, p):
     assert len(np.unique(cd)) == op.m()
 
 
-def test_cfg1_sizes(smf):
+def test_cg_p2_16cube_sizes(smf):
     op = smf.Operator(smf.Mesh.kuhn_cube(16, 0.2, 7), 2, host_only=True)
     assert op.info["n_cells"] == 24576 and op.m() == 35937
     assert op.info["n_constrained"] == 6146
