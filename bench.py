@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--n", type=int, default=64, help="hexes per cube edge")
     ap.add_argument("--degree", type=int, default=2)
     ap.add_argument("--kind", default="dg")
-    ap.add_argument("--variants", type=int, default=1, help="0 to skip the other configs' lines")
+    ap.add_argument("--variants", type=int, default=1, help="0 to skip the other workloads' lines")
     ap.add_argument("--external-n", type=int, default=64,
                     help="CG/DG P2 on an n^3 cube in a scrambled input order, reordering on and off (0 skips)")
     ap.add_argument("--matrix-n", type=int, default=64,
