@@ -147,14 +147,20 @@ void assemble_cg_matrix(const Operator& op, SysMatrix& A) {
   // (FEM rows have at most a few hundred entries, far below a row block)
   A.precision = op.precision;
   A.n_rows = n * nc;
-  A.ptr.assign(A.n_rows + 1, 0);
-  A.col.resize(snnz * nc);
-  A.val.resize(snnz * nc);
-  for (int c = 0; c < nc; ++c) {
-    for (int64_t r = 0; r <= n; ++r) A.ptr[c * n + r] = c * snnz + sptr[r];
-    for (int64_t e = 0; e < snnz; ++e) {
-      A.col[c * snnz + e] = (int32_t)(c * n + scol[e]);
-      A.val[c * snnz + e] = sval[e];
+  if (nc == 1) {  // scalar: the arrays as they are (no second copy)
+    A.ptr = std::move(sptr);
+    A.col = std::move(scol);
+    A.val = std::move(sval);
+  } else {
+    A.ptr.assign(A.n_rows + 1, 0);
+    A.col.resize(snnz * nc);
+    A.val.resize(snnz * nc);
+    for (int c = 0; c < nc; ++c) {
+      for (int64_t r = 0; r <= n; ++r) A.ptr[c * n + r] = c * snnz + sptr[r];
+      for (int64_t e = 0; e < snnz; ++e) {
+        A.col[c * snnz + e] = (int32_t)(c * n + scol[e]);
+        A.val[c * snnz + e] = sval[e];
+      }
     }
   }
   if (!make_row_blocks(A))
