@@ -56,6 +56,7 @@ void upload_cg(Operator& op) {
   op.d_pslot.upload(pl.pslot);
   op.d_gfix.upload(pl.gfix);
   op.d_lidx.upload(pl.lidx);
+  op.d_lidx12.upload(pl.lidx12);
   op.d_partial.alloc(std::max<int64_t>(pl.n_ghost_total, 1) * op.ncomp * op.wbytes());
   // curved cells: the kernels read the 27 Jacobian coefficients per cell
   const std::vector<double>& metric = pl.curved ? pl.jgeo : pl.metric;

@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ own_start,
     const int32_t* __restrict__ ghost_start, const int32_t* __restrict__ ghost_idx,
     const int32_t* __restrict__ pslot, const int32_t* __restrict__ gfix, const uint8_t* __restrict__ ccol,
-    const uint16_t* __restrict__ lidx,
+    const uint16_t* __restrict__ lidx, const uint32_t* __restrict__ lidx12,
     int64_t ncells_total,
     const T* __restrict__ metric, const T* __restrict__ shape, const T* src,
     T* __restrict__ dst, T* __restrict__ partial, int64_t cstride, int64_t pstride, RowEpilogue<T> ep,
@@ -256,14 +256,23 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
   }
   __syncthreads();
 
-  // per-round cell data, loaded one round ahead (software pipelining)
+  // per-round cell data, loaded one round ahead (software pipelining); the
+  // 12-bit packed indices (FEAT bit 8) are kept as words and decoded when
+  // the round starts, so the loads stay in flight across the round before
+  constexpr int NW = (FEAT & 8) ? (NL * 12 + 31) / 32 : 1;
   uint16_t li_n[NL];
+  uint32_t lw_n[NW];
   T m_n[6];
   int col_n, bc_n = -1;
   auto load_round = [&](int r0) {
     const int64_t cell = c0 + min(r0 + lane, nc - 1);
+    if constexpr ((FEAT & 8) != 0) {
 #pragma unroll
-    for (int i = 0; i < NL; ++i) li_n[i] = lidx[(int64_t)i * ncells_total + cell];
+      for (int w = 0; w < NW; ++w) lw_n[w] = lidx12[(int64_t)w * ncells_total + cell];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NL; ++i) li_n[i] = lidx[(int64_t)i * ncells_total + cell];
+    }
     if constexpr ((FEAT & 4) == 0) {
 #pragma unroll
       for (int t = 0; t < 6; ++t) m_n[t] = metric[t * ncells_total + cell];
@@ -277,7 +286,17 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const bool active = lc < nc;
     uint16_t li[NL];
 #pragma unroll
-    for (int i = 0; i < NL; ++i) li[i] = min((int)li_n[i], nloc);
+    for (int i = 0; i < NL; ++i) {
+      if constexpr ((FEAT & 8) != 0) {
+        // index i: bits 12 i .. 12 i + 11; 0xFFF (constrained) clamps to the dummy slot
+        const int b = 12 * i, w = b >> 5, sh = b & 31;
+        uint32_t v = lw_n[w] >> sh;
+        if (sh > 20) v |= lw_n[w + 1] << (32 - sh);
+        li[i] = min((int)(v & 0xFFFu), nloc);
+      } else {
+        li[i] = min((int)li_n[i], nloc);
+      }
+    }
     const T m0 = m_n[0], m1 = m_n[1], m2 = m_n[2], m3 = m_n[3], m4 = m_n[4], m5 = m_n[5];
     const int mycol = col_n, mybc = bc_n;
     if (r0 + nt < nc) load_round(r0 + nt);
@@ -572,7 +591,8 @@ void launch_cg_k(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpi
     hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_patches), dim3(kWave * pl.waves), shmem, s,
                        op.d_cell_start.as<int32_t>(), op.d_own_start.as<int32_t>(),
                        op.d_ghost_start.as<int32_t>(), op.d_ghost_idx.as<int32_t>(), op.d_pslot.as<int32_t>(),
-                       op.d_gfix.as<int32_t>(), op.d_ccol.as<uint8_t>(), op.d_lidx.as<uint16_t>(), pl.n_cells,
+                       op.d_gfix.as<int32_t>(), op.d_ccol.as<uint8_t>(), op.d_lidx.as<uint16_t>(),
+                       op.d_lidx12.as<uint32_t>(), pl.n_cells,
                        op.d_metric.as<T>(), op.d_shape.as<T>(), src, dst,
                        partial_out ? partial_out : op.d_partial.as<T>(), pl.n_dofs,
                        std::max<int64_t>(pl.n_ghost_total, 1), ep, pg,
@@ -595,6 +615,20 @@ void launch_cg_k(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpi
 template <int P, typename T, int MODE, int NC>
 void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpilogue<T> ep = {},
                  Pending<T> pg = {}, T* partial_out = nullptr, bool fixup = true, bool cell_boundary = false) {
+  // 12-bit packed local indices when the plan has them (P >= 2)
+  if constexpr (P >= 2) {
+    if (op.cg.lidx12_words > 0 && !op.cg.curved) {
+      if (op.cg.lidx12_words != (CGShape<P>::NL * 12 + 31) / 32)
+        throw std::runtime_error("CG kernel: packed index width does not match the degree");
+      if constexpr (MODE == 0 && NC == 1) {
+        if (partial_out && cell_boundary)
+          return launch_cg_k<P, T, MODE, NC, 11>(op, dst, src, s, ep, pg, partial_out, fixup, true);
+        if (partial_out) return launch_cg_k<P, T, MODE, NC, 9>(op, dst, src, s, ep, pg, partial_out, fixup, false);
+      }
+      if (pg.on || cell_boundary) throw std::runtime_error("CG kernel: pending staging / cell boundary need a Chebyshev run");
+      return launch_cg_k<P, T, MODE, NC, 8>(op, dst, src, s, ep, pg, partial_out, fixup, false);
+    }
+  }
   if (op.cg.curved) {
     if (pg.on || cell_boundary || partial_out) throw std::runtime_error("CG kernel: curved cells take no Chebyshev run");
     if (op.cg.n_q != CGShape<P>::NQC) throw std::runtime_error("CG kernel: curved-cell rule does not match the kernel");

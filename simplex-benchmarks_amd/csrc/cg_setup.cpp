@@ -287,6 +287,33 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   }
   if (plan.nitsche) build_nitsche_boundary(mesh, plan, opt.penalty_kind);
   if (opt.manifold != 0) build_curved_geometry(mesh, opt, plan);
+
+  // ---- 12-bit packed local indices (fewer and smaller loads per cell) ----
+  plan.lidx12.clear();
+  plan.lidx12_words = 0;
+  if (p >= 2 && plan.max_local < 0xFFF) {
+    const int nw = (nl * 12 + 31) / 32;
+    plan.lidx12.assign((size_t)nw * nc, 0u);
+    for (int64_t k = 0; k < nc; ++k)
+      for (int i = 0; i < nl; ++i) {
+        const uint16_t li = plan.lidx[(size_t)i * nc + k];
+        const uint64_t v = li == 0xFFFF ? 0xFFFu : li;
+        const int b = 12 * i, w = b >> 5, s = b & 31;
+        plan.lidx12[(size_t)w * nc + k] |= (uint32_t)(v << s);
+        if (s > 20) plan.lidx12[(size_t)(w + 1) * nc + k] |= (uint32_t)(v >> (32 - s));
+      }
+    // set-up self-check: the packed form decodes to the unpacked one
+    for (int64_t k = 0; k < nc; ++k)
+      for (int i = 0; i < nl; ++i) {
+        const int b = 12 * i, w = b >> 5, s = b & 31;
+        uint32_t v = plan.lidx12[(size_t)w * nc + k] >> s;
+        if (s > 20) v |= plan.lidx12[(size_t)(w + 1) * nc + k] << (32 - s);
+        v &= 0xFFFu;
+        const uint16_t li = plan.lidx[(size_t)i * nc + k];
+        if (v != (li == 0xFFFF ? 0xFFFu : li)) throw std::runtime_error("packed local indices do not decode");
+      }
+    plan.lidx12_words = nw;
+  }
 }
 
 namespace {

@@ -92,6 +92,11 @@ struct CGPlan {
   // per ghost entry: (position of its patch in the DoF's partial list) << 16 | list length
   std::vector<int32_t> gfix;
   std::vector<uint16_t> lidx;        // n_loc x n_cells patch-local index (0xFFFF = constrained)
+  // the same packed to 12 bits (P >= 2, every patch below 4095 local DoFs;
+  // 0xFFF = constrained): lidx12[w][n_cells], index i in bits 12 i .. 12 i + 11
+  // of the little-endian word sequence; lidx12_words = 0 when not packed
+  std::vector<uint32_t> lidx12;
+  int lidx12_words = 0;
   std::vector<double> metric;        // 6 x n_cells: |det J| J^-1 J^-T (xx xy xz yy yz zz)
   std::vector<double> jit;           // 9 x n_cells: J^-T row-major
   std::vector<double> detj;          // n_cells
@@ -166,7 +171,7 @@ struct Operator {
   bool is_dg() const { return kind == MF_DG_SIPG || kind == MF_DG_VECTOR || kind == MF_DG_HELMHOLTZ; }
   // device data
   DevBuf d_cell_start, d_own_start, d_ghost_start, d_ghost_idx, d_ccol, d_fix_ptr,
-      d_pslot, d_gfix, d_lidx, d_metric, d_shape, d_partial;
+      d_pslot, d_gfix, d_lidx, d_lidx12, d_metric, d_shape, d_partial;
   DevBuf d_bnd_rows, d_bnd_ptr, d_bnd_col, d_bnd_val, d_bnd_dense, d_bcell, d_bmat;
   DevBuf d_dg_geo, d_dg_nbr, d_dg_nface, d_dg_cell_tab, d_dg_face_tab;
   bool on_device = false;
