@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
   uint16_t li_n[NL];
   uint32_t lw_n[NW];
   T m_n[6];
-  int col_n, bc_n = -1;
+  int col_n = 0, bc_n = -1;
   auto load_round = [&](int r0) {
     const int64_t cell = c0 + min(r0 + lane, nc - 1);
     if constexpr ((FEAT & 8) != 0) {
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
 #pragma unroll
       for (int t = 0; t < 6; ++t) m_n[t] = metric[t * ncells_total + cell];
     }
-    col_n = ccol[cell];
+    if constexpr ((FEAT & 8) == 0) col_n = ccol[cell];  // packed: decoded at the round start
     if ((FEAT & 2) && bcell) bc_n = bcell[cell];
   };
   if (nc > 0) load_round(0);
@@ -298,7 +298,10 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
       }
     }
     const T m0 = m_n[0], m1 = m_n[1], m2 = m_n[2], m3 = m_n[3], m4 = m_n[4], m5 = m_n[5];
-    const int mycol = col_n, mybc = bc_n;
+    // packed: the colour follows the indices (bits 12 NL .. 12 NL + 7)
+    int mycol = col_n;
+    if constexpr ((FEAT & 8) != 0) mycol = (int)((lw_n[(NL * 12) >> 5] >> ((NL * 12) & 31)) & 0xFFu);
+    const int mybc = bc_n;
     if (r0 + nt < nc) load_round(r0 + nt);
 #pragma unroll 1
     for (int cc = 0; cc < NC; ++cc) {

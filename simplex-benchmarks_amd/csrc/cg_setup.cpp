@@ -291,10 +291,14 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
   // ---- 12-bit packed local indices (fewer and smaller loads per cell) ----
   plan.lidx12.clear();
   plan.lidx12_words = 0;
+  // the cell's colour takes the 8 bits after the indices (P2: bits 120-127,
+  // P3: 240-247), so the kernels need no separate colour load
   if (p >= 2 && plan.max_local < 0xFFF) {
     const int nw = (nl * 12 + 31) / 32;
+    const int cb = nl * 12, cw = cb >> 5, cs = cb & 31;
+    if (cs > 24) throw std::runtime_error("packed local indices: no room for the colour");
     plan.lidx12.assign((size_t)nw * nc, 0u);
-    for (int64_t k = 0; k < nc; ++k)
+    for (int64_t k = 0; k < nc; ++k) {
       for (int i = 0; i < nl; ++i) {
         const uint16_t li = plan.lidx[(size_t)i * nc + k];
         const uint64_t v = li == 0xFFFF ? 0xFFFu : li;
@@ -302,6 +306,8 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
         plan.lidx12[(size_t)w * nc + k] |= (uint32_t)(v << s);
         if (s > 20) plan.lidx12[(size_t)(w + 1) * nc + k] |= (uint32_t)(v >> (32 - s));
       }
+      plan.lidx12[(size_t)cw * nc + k] |= (uint32_t)plan.ccol[k] << cs;
+    }
     // set-up self-check: the packed form decodes to the unpacked one
     for (int64_t k = 0; k < nc; ++k)
       for (int i = 0; i < nl; ++i) {
@@ -312,6 +318,9 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
         const uint16_t li = plan.lidx[(size_t)i * nc + k];
         if (v != (li == 0xFFFF ? 0xFFFu : li)) throw std::runtime_error("packed local indices do not decode");
       }
+    for (int64_t k = 0; k < nc; ++k)
+      if (((plan.lidx12[(size_t)cw * nc + k] >> cs) & 0xFFu) != plan.ccol[k])
+        throw std::runtime_error("packed cell colour does not decode");
     plan.lidx12_words = nw;
   }
 }

@@ -94,7 +94,8 @@ struct CGPlan {
   std::vector<uint16_t> lidx;        // n_loc x n_cells patch-local index (0xFFFF = constrained)
   // the same packed to 12 bits (P >= 2, every patch below 4095 local DoFs;
   // 0xFFF = constrained): lidx12[w][n_cells], index i in bits 12 i .. 12 i + 11
-  // of the little-endian word sequence; lidx12_words = 0 when not packed
+  // of the little-endian word sequence, then the cell colour (ccol) in the
+  // next 8 bits; lidx12_words = 0 when not packed
   std::vector<uint32_t> lidx12;
   int lidx12_words = 0;
   std::vector<double> metric;        // 6 x n_cells: |det J| J^-1 J^-T (xx xy xz yy yz zz)
