@@ -176,6 +176,8 @@ typedef struct mf_mg_options {
                                  assembled FP64 matrix (the reference's MGCoarseAMG, one ML V-cycle,
                                  application_hp.cc:1684-1731; needed on meshes without a Kuhn lattice);
                                  1: dense only; 2: AMG always */
+  double amg_threshold;       /* AMG aggregation threshold theta (default 0.05): coupling (i, j) is strong
+                                 when |a_ij| >= theta sqrt(|a_ii a_jj|) (ML's "aggregation: threshold") */
 } mf_mg_options;
 
 #define MF_MAX_LEVELS 16

@@ -275,8 +275,8 @@ def curved_vmult(p, vertices, cells, cell_dofs, constrained, pts, w, x, center=(
 # ---- smoothed-aggregation AMG (numpy restatement) ----------------------------
 # The coarse solver in place of the reference's ML V-cycle (MGCoarseAMG,
 # CG/cells_batched/application_hp.cc:1684-1731): smoothed aggregation
-# (Vanek, Mandel, Brezina 1996) with the build's stated choices -- every
-# nonzero coupling strong, three aggregation passes in row order, unit-norm
+# (Vanek, Mandel, Brezina 1996) with the build's stated choices -- strong
+# couplings |a_ij| >= theta sqrt(|a_ii a_jj|), three aggregation passes in row order, unit-norm
 # aggregate indicators, P = (I - 4/(3 lambda) D^-1 A) T with lambda from 30
 # power iterations on D^-1 A from v_i = 1 + sin(0.7 i)/2, Galerkin A_c, a
 # dense inverse at <= max_dense rows, Chebyshev-Jacobi smoothing (degree,
@@ -287,7 +287,7 @@ def _csr_rows(ptr, col, val):
     return [(col[ptr[i]:ptr[i + 1]], val[ptr[i]:ptr[i + 1]]) for i in range(len(ptr) - 1)]
 
 
-def amg_hierarchy(ptr, col, val, max_dense=4096):
+def amg_hierarchy(ptr, col, val, max_dense=4096, theta=0.0):
     """levels: list of dicts with A (dense-free scipy CSR), P, lam; last level has inv"""
     import scipy.sparse as sp
     n = len(ptr) - 1
@@ -305,13 +305,17 @@ def amg_hierarchy(ptr, col, val, max_dense=4096):
             lam = v @ w
             v = w
         rows = _csr_rows(A.indptr, A.indices, A.data)
-        iso = np.array([not np.any((c != i) & (x != 0)) for i, (c, x) in enumerate(rows)])
+        # strong couplings: |a_ij| >= theta sqrt(|a_ii a_jj|), off the diagonal, nonzero
+        rows = [(c[(c != i) & (x != 0) & (np.abs(x) >= theta * np.sqrt(np.abs(d[i] * d[c])))],
+                 x[(c != i) & (x != 0) & (np.abs(x) >= theta * np.sqrt(np.abs(d[i] * d[c])))])
+                for i, (c, x) in enumerate(rows)]
+        iso = np.array([len(c) == 0 for c, _ in rows])
         agg = -np.ones(n, np.int64)
         na = 0
         for i, (c, x) in enumerate(rows):  # pass 1
             if agg[i] >= 0 or iso[i]:
                 continue
-            nb = c[(c != i) & (x != 0)]
+            nb = c
             if np.any(agg[nb] >= 0):
                 continue
             agg[i] = na
@@ -323,7 +327,7 @@ def amg_hierarchy(ptr, col, val, max_dense=4096):
                 continue
             best, bv = -1, 0.0
             for j, a in zip(c, x):
-                if j != i and agg1[j] >= 0 and abs(a) > bv:
+                if agg1[j] >= 0 and abs(a) > bv:
                     best, bv = agg1[j], abs(a)
             agg[i] = best
         for i, (c, x) in enumerate(rows):  # pass 3
@@ -331,7 +335,7 @@ def amg_hierarchy(ptr, col, val, max_dense=4096):
                 continue
             agg[i] = na
             for j, a in zip(c, x):
-                if j != i and a != 0 and agg[j] < 0 and not iso[j]:
+                if agg[j] < 0 and not iso[j]:
                     agg[j] = na
             na += 1
         if na == 0 or na > 0.9 * n:

@@ -4,13 +4,15 @@
 // ML V-cycle (CG/cells_batched/application_hp.cc:1684-1731, 1918-1930).
 //
 // Set-up on the host, per level:
-//   * every nonzero coupling is strong (strength threshold 0); rows without
-//     off-diagonal entries (identity rows of constrained DoFs) are isolated
-//     and belong to no aggregate;
+//   * coupling (i, j) is strong when |a_ij| >= theta sqrt(|a_ii a_jj|)
+//     (ML's aggregation threshold); rows without strong couplings (identity
+//     rows of constrained DoFs among them) are isolated and belong to no
+//     aggregate;
 //   * aggregation in three passes over the rows in order: (1) a row whose
-//     neighbours are all unaggregated starts an aggregate with them; (2) a
-//     remaining row joins the pass-1 aggregate of its strongest neighbour;
-//     (3) the rest start aggregates with their unaggregated neighbours;
+//     strong neighbours are all unaggregated starts an aggregate with them;
+//     (2) a remaining row joins the pass-1 aggregate of its strongest strong
+//     neighbour; (3) the rest start aggregates with their unaggregated strong
+//     neighbours;
 //   * tentative prolongator T: column j = indicator of aggregate j scaled to
 //     unit norm (the constants, the near-null space of the Laplacian);
 //   * P = (I - omega D^-1 A) T with omega = 4 / (3 lambda_max(D^-1 A));
@@ -121,24 +123,32 @@ double lambda_max_jacobi(const Csr& A, const std::vector<double>& d) {
   return lam;
 }
 
-// aggregate of every row (-1: isolated); returns the number of aggregates
-int32_t aggregate(const Csr& A, std::vector<int32_t>& agg) {
+// aggregate of every row (-1: isolated); returns the number of aggregates.
+// Coupling (i, j) is strong when |a_ij| >= theta sqrt(|a_ii a_jj|) (nonzero
+// for theta = 0); a row without strong couplings is isolated.
+int32_t aggregate(const Csr& A, const std::vector<double>& d, double theta, std::vector<int32_t>& agg) {
   const int64_t n = A.nrows;
   agg.assign(n, -1);
+  std::vector<uint8_t> strong(A.col.size(), 0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
+      const int32_t j = A.col[k];
+      strong[k] = j != i && A.val[k] != 0.0 && std::fabs(A.val[k]) >= theta * std::sqrt(std::fabs(d[i] * d[j]));
+    }
   std::vector<uint8_t> iso(n, 1);
   for (int64_t i = 0; i < n; ++i)
     for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
-      if (A.col[k] != i && A.val[k] != 0.0) iso[i] = 0;
+      if (strong[k]) iso[i] = 0;
   int32_t na = 0;
   for (int64_t i = 0; i < n; ++i) {  // pass 1
     if (agg[i] >= 0 || iso[i]) continue;
     bool fresh = true;
     for (int64_t k = A.ptr[i]; k < A.ptr[i + 1] && fresh; ++k)
-      if (A.col[k] != i && A.val[k] != 0.0 && agg[A.col[k]] >= 0) fresh = false;
+      if (strong[k] && agg[A.col[k]] >= 0) fresh = false;
     if (!fresh) continue;
     agg[i] = na;
     for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
-      if (A.col[k] != i && A.val[k] != 0.0) agg[A.col[k]] = na;
+      if (strong[k]) agg[A.col[k]] = na;
     ++na;
   }
   const std::vector<int32_t> agg1 = agg;
@@ -148,7 +158,7 @@ int32_t aggregate(const Csr& A, std::vector<int32_t>& agg) {
     double bv = 0;
     for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
       const int32_t j = A.col[k];
-      if (j != i && agg1[j] >= 0 && std::fabs(A.val[k]) > bv) {
+      if (strong[k] && agg1[j] >= 0 && std::fabs(A.val[k]) > bv) {
         bv = std::fabs(A.val[k]);
         best = agg1[j];
       }
@@ -160,7 +170,7 @@ int32_t aggregate(const Csr& A, std::vector<int32_t>& agg) {
     agg[i] = na;
     for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
       const int32_t j = A.col[k];
-      if (j != i && A.val[k] != 0.0 && agg[j] < 0 && !iso[j]) agg[j] = na;
+      if (strong[k] && agg[j] < 0 && !iso[j]) agg[j] = na;
     }
     ++na;
   }
@@ -219,8 +229,9 @@ void vcycle_level(Amg& amg, size_t l, hipStream_t s) {
 
 }  // namespace
 
-std::unique_ptr<Amg> build_amg(const Csr& A0, int max_dense) {
+std::unique_ptr<Amg> build_amg(const Csr& A0, int max_dense, double strength) {
   auto amg = std::make_unique<Amg>();
+  amg->strength = strength;
   amg->lv.emplace_back();
   amg->lv.back().A.nrows = A0.nrows;
   amg->lv.back().A.ncols = A0.ncols;
@@ -235,7 +246,7 @@ std::unique_ptr<Amg> build_amg(const Csr& A0, int max_dense) {
       if (!(v > 0)) throw std::runtime_error("AMG: matrix with a non-positive diagonal entry");
     L.lambda_max = lambda_max_jacobi(A, d);
     std::vector<int32_t> agg;
-    const int32_t na = aggregate(A, agg);
+    const int32_t na = aggregate(A, d, amg->strength, agg);
     if (na == 0 || na > 0.9 * (double)A.nrows) break;
     // tentative prolongator: unit-norm aggregate indicators
     std::vector<int64_t> size(na, 0);

@@ -29,12 +29,13 @@ struct Amg {
   int smoother_degree = 2;   // Chebyshev-Jacobi steps before and after the coarse correction
                              // (the multigrid sets its own smoother degree)
   double smoothing_range = 30.0;
+  double strength = 0.0;     // aggregation threshold theta
 };
 
 // A0: symmetric positive definite FP64 CSR, columns ascending (identity rows
 // of constrained DoFs allowed: they stay out of the aggregates); levels are
 // added until one has at most max_dense rows
-std::unique_ptr<Amg> build_amg(const Csr& A0, int max_dense);
+std::unique_ptr<Amg> build_amg(const Csr& A0, int max_dense, double strength);
 // lv[0].x = one V-cycle applied to lv[0].b, from x = 0 (device, stream-ordered)
 void amg_vcycle(Amg& amg, hipStream_t s);
 

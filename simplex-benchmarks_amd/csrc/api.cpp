@@ -482,6 +482,7 @@ void mf_mg_default_options(mf_mg_options* opt) {
   opt->max_coarse_dofs = d.max_coarse_dofs;
   opt->fuse_steps = d.fuse_steps;
   opt->coarse_solver = d.coarse_solver;
+  opt->amg_threshold = d.amg_threshold;
 }
 
 int mf_mg_create(mf_op_t fine, const mf_mg_options* opt, mf_mg_t* out) {
@@ -499,6 +500,8 @@ int mf_mg_create(mf_op_t fine, const mf_mg_options* opt, mf_mg_t* out) {
       o.fuse_steps = opt->fuse_steps;
       o.coarse_solver = opt->coarse_solver;
       if (o.coarse_solver < 0 || o.coarse_solver > 2) throw std::runtime_error("coarse_solver must be 0, 1 or 2");
+      o.amg_threshold = opt->amg_threshold;
+      if (!(o.amg_threshold >= 0.0 && o.amg_threshold < 1.0)) throw std::runtime_error("amg_threshold must be in [0, 1)");
     }
     if (o.smoother_degree < 1 || o.smoothing_range <= 1 || o.eig_cg_iterations < 1)
       throw std::runtime_error("invalid multigrid options");

@@ -650,7 +650,7 @@ std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions
     A0.ptr = std::move(sm.ptr);
     A0.col = std::move(sm.col);
     A0.val = std::move(sm.val);
-    mg->amg = build_amg(A0, opt.max_coarse_dofs);
+    mg->amg = build_amg(A0, opt.max_coarse_dofs, opt.amg_threshold);
     mg->amg->smoother_degree = opt.smoother_degree;  // its finest level, as on the geometric levels
     SMF_HIP(hipDeviceSynchronize());
     return mg;

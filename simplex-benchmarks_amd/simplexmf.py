@@ -155,7 +155,8 @@ def lib():
 class MGOptions(C.Structure):
     _fields_ = [("smoother_degree", C.c_int32), ("smoothing_range", C.c_double),
                 ("eig_cg_iterations", C.c_int32), ("level_precision", C.c_int32),
-                ("max_coarse_dofs", C.c_int32), ("fuse_steps", C.c_int32), ("coarse_solver", C.c_int32)]
+                ("max_coarse_dofs", C.c_int32), ("fuse_steps", C.c_int32), ("coarse_solver", C.c_int32),
+                ("amg_threshold", C.c_double)]
 
 
 MF_MAX_LEVELS = 16
@@ -594,7 +595,8 @@ class Multigrid:
     (MultigridPreconditioner of CG/cells_batched/application_ph.cc:1754-2026)."""
 
     def __init__(self, fine: Operator, smoother_degree=5, smoothing_range=20.0, eig_cg_iterations=10,
-                 level_precision="f32", max_coarse_dofs=4096, fuse_steps=True, coarse_solver="auto"):
+                 level_precision="f32", max_coarse_dofs=4096, fuse_steps=True, coarse_solver="auto",
+                 amg_threshold=None):
         L = lib()
         opt = MGOptions()
         L.mf_mg_default_options(C.byref(opt))
@@ -606,6 +608,8 @@ class Multigrid:
         opt.fuse_steps = 1 if fuse_steps else 0
         opt.coarse_solver = {"auto": 0, "dense": 1, "amg": 2}[coarse_solver] if isinstance(coarse_solver, str) \
             else int(coarse_solver)
+        if amg_threshold is not None:
+            opt.amg_threshold = float(amg_threshold)
         self.level_dtype = NP_DTYPES[opt.level_precision]
         self.fine = fine
         h = C.c_void_p()

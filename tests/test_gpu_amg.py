@@ -97,7 +97,7 @@ def test_amg_vcycle_matches_oracle_restatement(gpu, orc):
     mg = smf.Multigrid(op, level_precision="f64")
     assert mg.info()["n_levels"] == 1
     ptr, col, val = smf.SystemMatrix(op).to_host()
-    levels = orc.amg_hierarchy(ptr, col, val, max_dense=4096)
+    levels = orc.amg_hierarchy(ptr, col, val, max_dense=4096, theta=0.05)  # the default threshold
     assert [L["A"].shape[0] for L in levels] == mg.info()["amg_rows"]
     r = np.random.default_rng(9).uniform(-1, 1, op.m())
     z = op.initialize_dof_vector()

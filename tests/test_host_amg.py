@@ -36,9 +36,10 @@ def pcg(A, b, prec, tol=1e-10, maxit=500):
     return x, maxit
 
 
-def test_amg_restatement_structure(smf, orc):
+@pytest.mark.parametrize("theta", [0.0, 0.05])
+def test_amg_restatement_structure(smf, orc, theta):
     op, (ptr, col, val) = p1_matrix(smf, 12)
-    levels = orc.amg_hierarchy(ptr, col, val, max_dense=200)
+    levels = orc.amg_hierarchy(ptr, col, val, max_dense=200, theta=theta)
     assert len(levels) >= 2 and levels[-1]["A"].shape[0] <= 200
     con = op.constrained().astype(bool)
     # constrained rows get no coarse correction, every free row some
@@ -51,9 +52,10 @@ def test_amg_restatement_structure(smf, orc):
         assert np.linalg.eigvalsh(A[np.ix_(free, free)]).min() > 0
 
 
-def test_amg_restatement_preconditions_cg(smf, orc):
+@pytest.mark.parametrize("theta", [0.0, 0.05])
+def test_amg_restatement_preconditions_cg(smf, orc, theta):
     op, (ptr, col, val) = p1_matrix(smf, 14)
-    levels = orc.amg_hierarchy(ptr, col, val, max_dense=300)
+    levels = orc.amg_hierarchy(ptr, col, val, max_dense=300, theta=theta)
     A = levels[0]["A"]
     b = np.random.default_rng(2).uniform(-1, 1, A.shape[0])
     V = lambda r: orc.amg_vcycle(levels, r)
