@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--mode", default="vmult", choices=["vmult", "solve"])
     ap.add_argument("--solves", type=int, default=5, help="repeated solves of the solver comparison")
+    ap.add_argument("--solve-compare", type=int, default=1,
+                    help="0 skips the learned-vs-plain comparison (the external-mesh solve still runs)")
     ap.add_argument("--solve-n", type=int, default=48, help="mesh of the solver comparison (0: skip it)")
     ap.add_argument("--solve-kind", default="cg", choices=["cg", "dg"], help="operator of --mode solve")
     return ap.parse_args()
@@ -487,7 +489,7 @@ def main():
     if args.external_n > 0:
         out["external_mesh"] = external_compare(smf, args.external_n, max(10, args.steps // 2), args.seed,
                                                 solve_n=args.solve_n)
-    if args.solve_n > 0:
+    if args.solve_n > 0 and args.solve_compare:
         out["solve"] = solve_compare(smf, args.solve_n, 2, args.seed, args.solves)
         out["solve_dg"] = solve_compare(smf, args.solve_n, 2, args.seed, args.solves, kind="dg")
     print(json.dumps(out), flush=True)

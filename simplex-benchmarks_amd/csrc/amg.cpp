@@ -219,8 +219,11 @@ void vcycle_level(Amg& amg, size_t l, hipStream_t s) {
   cheby(amg, L, true, s, apply);
   apply(L.r.as<double>(), L.x.as<double>());
   blas_axpby<double>(n, 1.0, L.b.as<double>(), -1.0, L.r.as<double>(), s);  // r = b - A x
-  blas_csr<double>(L.R.nrows, L.R.d_ptr.as<int64_t>(), L.R.d_col.as<int32_t>(), L.R.d_val.as<double>(),
-                   L.r.as<double>(), C.b.as<double>(), false, s);
+  if (L.Rm.on_device)
+    launch_spmv(L.Rm, C.b.as<double>(), L.r.as<double>(), s);
+  else
+    blas_csr<double>(L.R.nrows, L.R.d_ptr.as<int64_t>(), L.R.d_col.as<int32_t>(), L.R.d_val.as<double>(),
+                     L.r.as<double>(), C.b.as<double>(), false, s);
   vcycle_level(amg, l + 1, s);
   blas_csr<double>(L.P.nrows, L.P.d_ptr.as<int64_t>(), L.P.d_col.as<int32_t>(), L.P.d_val.as<double>(),
                    C.x.as<double>(), L.x.as<double>(), true, s);
@@ -328,7 +331,13 @@ std::unique_ptr<Amg> build_amg(const Csr& A0, int max_dense, double strength) {
     else L.A.upload(MF_F64);
     if (!L.P.ptr.empty()) {
       L.P.upload(MF_F64);
-      L.R.upload(MF_F64);
+      L.Rm.precision = MF_F64;
+      L.Rm.n_rows = L.R.nrows;
+      L.Rm.ptr = L.R.ptr;
+      L.Rm.col = L.R.col;
+      L.Rm.val = L.R.val;
+      if (L.R.nrows > 0 && make_row_blocks(L.Rm)) upload_matrix(L.Rm);
+      else L.R.upload(MF_F64);
     }
     std::vector<double> dinv = diagonal(L.A);
     for (auto& v : dinv) v = 1.0 / v;

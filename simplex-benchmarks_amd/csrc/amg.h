@@ -18,6 +18,7 @@ struct AmgLevel {
   Csr A;                     // this level's matrix (FP64)
   SysMatrix M;               // the same for the CSR-stream product (when every row fits a row block)
   Csr P, R;                  // prolongation from the next coarser level and R = P^T
+  SysMatrix Rm;              // R for the CSR-stream product (few rows of many entries)
   DevBuf dinv, x, b, r, d, t;  // FP64 level vectors
   double lambda_max = 0;     // of D^-1 A (Chebyshev upper bound, 1.1 x estimate)
 };
