@@ -71,6 +71,9 @@ int orc_dg_diagonal(const orc_mesh* m, int p, const int64_t* cell_dofs, int64_t 
                     double mass, double laplace, double* diag);
 /* penalty variant for the DG functions: 0 reference (default), 1 (p+1)(p+2)/h_F, h_F = |K|/|F| */
 int orc_dg_set_penalty_kind(int kind);
+/* planted mistakes for tests (0 none, 1 min for max penalty, 2 average
+ * without 1/2, 3 lumped mass); vmult only */
+int orc_dg_set_plant(int plant);
 /* per-cell penalty tau_K (n_cells) */
 int orc_dg_tau(const orc_mesh* m, double* tau);
 

@@ -33,6 +33,16 @@ int orc_dg_set_penalty_kind(int kind) {
   return 0;
 }
 
+/* planted mistakes, for the tests that show a check catches them:
+ * 1 min(tau-, tau+) instead of max; 2 average of the normal derivatives
+ * without the factor 1/2; 3 lumped (row-sum) mass matrix */
+static int g_plant = 0;
+int orc_dg_set_plant(int plant) {
+  if (plant < 0 || plant > 3) return -1;
+  g_plant = plant;
+  return 0;
+}
+
 typedef struct geo {
   double x0[3], J[3][3], Jinv[3][3], det;
 } geo;
@@ -213,7 +223,7 @@ static int dg_apply(const dgctx* c, const double* src, double* dst, double* diag
           }
           for (int i = 0; i < nl; ++i)
             dst[off + dofs[i]] += jxw * (c->laplace * (gu[0] * grad[i][0] + gu[1] * grad[i][1] + gu[2] * grad[i][2]) +
-                                         c->mass * u * val[i]);
+                                         c->mass * (g_plant == 3 ? src[off + dofs[i]] : u) * val[i]);
         } else {
           for (int i = 0; i < nl; ++i)
             diag[off + dofs[i]] += jxw * (c->laplace * (grad[i][0] * grad[i][0] + grad[i][1] * grad[i][1] + grad[i][2] * grad[i][2]) +
@@ -265,7 +275,8 @@ static int dg_apply(const dgctx* c, const double* src, double* dst, double* diag
       const double hm = fabs(gm.det) / 6.0 / area, hp = interior ? fabs(gp.det) / 6.0 / area : hm;
       sigma = (c->p + 1.0) * (c->p + 2.0) * fmax(1.0 / hm, 1.0 / hp);
     } else {
-      sigma = interior ? c->factor * fmax(c->tau[cm], c->tau[cp]) : c->factor * c->tau[cm];
+      sigma = interior ? c->factor * (g_plant == 1 ? fmin(c->tau[cm], c->tau[cp]) : fmax(c->tau[cm], c->tau[cp]))
+                       : c->factor * c->tau[cm];
     }
     const int64_t* dm = c->cd + cm * nl;
     const int64_t* dp = interior ? c->cd + cp * nl : NULL;
@@ -299,7 +310,7 @@ static int dg_apply(const dgctx* c, const double* src, double* dst, double* diag
             up = -um; /* mirror principle */
             dup = dum;
           }
-          const double jump = um - up, avg = 0.5 * (dum + dup);
+          const double jump = um - up, avg = (g_plant == 2 ? 1.0 : 0.5) * (dum + dup);
           const double tv = (sigma * jump - avg) * jxw * c->laplace;
           const double tg = -0.5 * jump * jxw * c->laplace;
           for (int i = 0; i < nl; ++i) {

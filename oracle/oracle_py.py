@@ -444,3 +444,212 @@ def adam(p, g, m, v, lr, step, b1=0.9, b2=0.999, eps=1e-8):
     g = _f64(g)
     _mlp_lib().orc_adam(p.ctypes.data, g.ctypes.data, m.ctypes.data, v.ctypes.data, p.size, lr, b1, b2, eps,
                         step)
+
+
+# ---- planted mistakes (tests only) -------------------------------------------
+class planted:
+    """Context manager that switches a planted mistake into the DG oracle
+    (1 min for max penalty, 2 average without 1/2, 3 lumped mass), so a test
+    can show that its check fails on it."""
+
+    def __init__(self, code):
+        self.code = int(code)
+
+    def __enter__(self):
+        L = lib()
+        L.orc_dg_set_plant.argtypes = [C.c_int]
+        if L.orc_dg_set_plant(self.code):
+            raise ValueError("unknown plant")
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_dg_set_plant(0)
+        return False
+
+
+# ---- Gaussian random field right-hand side (numpy restatement) ---------------
+# f(x) = sqrt(2/M) sum_m cos(omega_m . x + phase_m), omega_m ~ N(0, I/l^2)
+# (Box-Muller), phase_m ~ U[0, 2 pi), drawn from splitmix64(seed) in the order
+# (u1, u2) per omega component, then the phase (Rahimi & Recht's random
+# Fourier features of the squared-exponential covariance exp(-|d|^2/(2 l^2))).
+# The reference's right-hand side pattern b_i = int f phi_i is
+# CG/cells_batched/application_hp.cc:1441-1465; its f (:77-93) is replaced by
+# this field.
+
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(seed, n):
+    """first n outputs of splitmix64 (Steele, Lea, Flood 2014) as Python ints"""
+    s = seed & _M64
+    out = []
+    for _ in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & _M64
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def grf_features(seed, n_features, corr_length):
+    """(M, 3) frequencies and (M,) phases of the seeded field"""
+    r = splitmix64(seed, 7 * n_features)
+    u = [((v >> 11) + 0.5) * 2.0 ** -53 for v in r]
+    om = np.empty((n_features, 3))
+    ph = np.empty(n_features)
+    k = 0
+    for f in range(n_features):
+        for a in range(3):
+            u1, u2 = u[k], u[k + 1]
+            k += 2
+            om[f, a] = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2) / corr_length
+        ph[f] = 2.0 * np.pi * u[k]
+        k += 1
+    return om, ph
+
+
+def grf_eval(om, ph, X):
+    """field at points X (n, 3)"""
+    X = np.atleast_2d(X)
+    return np.sqrt(2.0 / len(ph)) * np.cos(X @ om.T + ph).sum(axis=1)
+
+
+def rhs_quadrature(p, vertices, cells, cell_dofs, n_dofs, f, n_rule=10):
+    """b_i = sum_K int_K f phi_i with the oracle's collapsed Gauss rule of
+    n_rule points per direction (exact to degree 2 n_rule - 3) and its own
+    basis; f maps physical points (n, 3) to values"""
+    pts, w = tet_rule(n_rule)
+    vals = np.array([basis(p, x)[0] for x in pts])          # (nq, nl)
+    V = np.asarray(vertices, np.float64)
+    Cc = np.asarray(cells)
+    x0 = V[Cc[:, 0]]
+    J = np.stack([V[Cc[:, k]] - x0 for k in (1, 2, 3)], axis=2)   # (nc, 3, 3), columns = edges
+    det = np.abs(np.linalg.det(J))
+    X = x0[:, None, :] + np.einsum("kde,qe->kqd", J, pts)          # (nc, nq, 3)
+    fx = f(X.reshape(-1, 3)).reshape(X.shape[:2])
+    loc = det[:, None] * np.einsum("kq,q,qi->ki", fx, w, vals)
+    b = np.zeros(n_dofs)
+    np.add.at(b, np.asarray(cell_dofs).ravel(), loc.ravel())
+    return b
+
+
+# ---- Chebyshev-Jacobi smoother, its eigenvalue estimate and the V-cycle -------
+# Restates the multigrid of CG/cells_batched/application_ph.cc:1754-2026 /
+# application_hp.cc:1893-1948 with the build's stated choices (deal.II's
+# PreconditionChebyshev internals are not in the reference):
+#  * lambda_max of D^-1 A from `iters` CG iterations on A x = D 1 ... started
+#    from r = 1, z = D^-1 r (Lanczos tridiagonal from the CG coefficients:
+#    T_kk = 1/alpha_k + beta_{k-1}/alpha_{k-1}, T_k,k+1 = sqrt(beta_k)/alpha_k);
+#  * Chebyshev interval [1.2 lambda / range, 1.2 lambda], Saad's Algorithm
+#    12.1 on the Jacobi-preconditioned system, `degree` residual polynomial
+#    degree, the first step from x = 0 skipping the product;
+#  * V-cycle: pre-smooth from 0, residual, restrict R = P^T, recurse, prolongate
+#    and add, post-smooth; the coarsest level is solved exactly on its free
+#    rows (constrained rows of the coarse solution are 0).
+
+
+def lanczos_lambda_max(A, dinv, iters):
+    n = A.shape[0]
+    r = np.ones(n)
+    z = dinv * r
+    p = z.copy()
+    rz = r @ z
+    al, be = [], []
+    for _ in range(iters):
+        if not rz > 0:
+            break
+        q = A @ p
+        a = rz / (p @ q)
+        r = r - a * q
+        z = dinv * r
+        rzn = r @ z
+        b = rzn / rz
+        al.append(a)
+        be.append(b)
+        p = z + b * p
+        rz = rzn
+    m = len(al)
+    if m == 0:
+        return 1.0
+    T = np.zeros((m, m))
+    for k in range(m):
+        T[k, k] = 1.0 / al[k] + (be[k - 1] / al[k - 1] if k > 0 else 0.0)
+        if k + 1 < m:
+            T[k, k + 1] = T[k + 1, k] = np.sqrt(be[k]) / al[k]
+    return float(np.linalg.eigvalsh(T).max())
+
+
+def inverse_diagonal(d):
+    """1/d where |d| > 1e-10, else 1 (CG/cells_batched/application_hp.cc:1404-1407)"""
+    d = np.asarray(d, np.float64)
+    big = np.abs(d) > 1e-10
+    return np.where(big, 1.0 / np.where(big, d, 1.0), 1.0)
+
+
+def chebyshev(A, dinv, b, x, degree, lmax, lmin):
+    """`degree` Chebyshev-Jacobi steps on A x = b; x None = start from zero"""
+    theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    sigma = theta / delta
+    rho = 1.0 / sigma
+    d = None
+    for k in range(degree):
+        if k == 0:
+            c1, c2 = 0.0, 1.0 / theta
+            if x is None:
+                d = c2 * dinv * b
+                x = d.copy()
+                continue
+        else:
+            rn = 1.0 / (2.0 * sigma - rho)
+            c1, c2 = rn * rho, 2.0 * rn / delta
+            rho = rn
+        r = b - A @ x
+        d = c2 * dinv * r if k == 0 else c1 * d + c2 * dinv * r
+        x = x + d
+    return x
+
+
+def chebyshev_error_factor(lam, degree, lmax, lmin):
+    """error reduction of `degree` steps from x = 0 on an eigenvector of
+    D^-1 A with eigenvalue lam: T_d((theta - lam)/delta) / T_d(theta/delta)"""
+    theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    T = np.polynomial.chebyshev.Chebyshev.basis(degree)
+    return T((theta - lam) / delta) / T(theta / delta)
+
+
+def mg_levels(As, Ps, constrained, degree=5, smoothing_range=20.0, eig_iters=10):
+    """level dicts, [0] = coarsest: As[l] dense level matrices, Ps[l] (l >= 1)
+    the prolongation from level l-1 to l, constrained[l] boolean masks"""
+    levels = []
+    for l, A in enumerate(As):
+        L = {"A": A, "dinv": inverse_diagonal(np.diag(A)), "con": np.asarray(constrained[l], bool)}
+        if l == 0:
+            free = ~L["con"]
+            L["inv"] = np.linalg.inv(A[np.ix_(free, free)])
+        else:
+            L["P"] = Ps[l]
+            lam = lanczos_lambda_max(A, L["dinv"], eig_iters)
+            L["lmax"] = 1.2 * lam
+            L["lmin"] = L["lmax"] / smoothing_range
+            L["lam"] = lam
+        L["degree"] = degree
+        levels.append(L)
+    return levels
+
+
+def mg_vcycle(levels, b, l=None):
+    if l is None:
+        l = len(levels) - 1
+    L = levels[l]
+    if l == 0:
+        x = np.zeros(len(b))
+        free = ~L["con"]
+        x[free] = L["inv"] @ b[free]
+        return x
+    A, dinv = L["A"], L["dinv"]
+    x = chebyshev(A, dinv, b, None, L["degree"], L["lmax"], L["lmin"])
+    r = b - A @ x
+    xc = mg_vcycle(levels, L["P"].T @ r, l - 1)
+    x = x + L["P"] @ xc
+    return chebyshev(A, dinv, b, x, L["degree"], L["lmax"], L["lmin"])
