@@ -77,6 +77,22 @@ int orc_dg_set_plant(int plant);
 /* per-cell penalty tau_K (n_cells) */
 int orc_dg_tau(const orc_mesh* m, double* tau);
 
+/* ---- CPU baseline (cpu_baseline.cpp): the reference's optimised DG SIPG
+ * vmult restated for CPUs -- 8-lane vectors of cells or faces, batches of 4
+ * vectors per shape matrix, OpenMP over cell batches and coloured face
+ * batches.  Cells must list their vertices in ascending order; DoF (k, i)
+ * is row k*n_loc + i in orc_basis node order.  nthreads <= 0: all threads. */
+void* orc_dgb_create(const orc_mesh* m, int p, int nthreads);
+int orc_dgb_vmult(void* plan, const double* src, double* dst);
+/* reps repetitions of n back-to-back products, seconds per repetition */
+int orc_dgb_time(void* plan, const double* src, double* dst, int reps, int n, double* seconds);
+/* out[6]: cell points, face points, face groups, colours, threads, vector bits */
+int orc_dgb_info(void* plan, int64_t* out);
+/* the rules used: kind 0 tetrahedron (4 barycentric coordinates per point),
+ * 1 triangle (3 per point); returns the number of points */
+int orc_dgb_rule(int kind, int p, double* bc, double* w);
+void orc_dgb_destroy(void* plan);
+
 /* ---- learned-smoother network (oracle_mlp.c) ---------------------------- */
 int orc_mlp_n_params(void);
 uint16_t orc_bf16(float f);

@@ -304,40 +304,12 @@ def amg_hierarchy(ptr, col, val, max_dense=4096, theta=0.0):
             w = (A @ v) / d
             lam = v @ w
             v = w
-        rows = _csr_rows(A.indptr, A.indices, A.data)
-        # strong couplings: |a_ij| >= theta sqrt(|a_ii a_jj|), off the diagonal, nonzero
-        rows = [(c[(c != i) & (x != 0) & (np.abs(x) >= theta * np.sqrt(np.abs(d[i] * d[c])))],
-                 x[(c != i) & (x != 0) & (np.abs(x) >= theta * np.sqrt(np.abs(d[i] * d[c])))])
-                for i, (c, x) in enumerate(rows)]
-        iso = np.array([len(c) == 0 for c, _ in rows])
-        agg = -np.ones(n, np.int64)
-        na = 0
-        for i, (c, x) in enumerate(rows):  # pass 1
-            if agg[i] >= 0 or iso[i]:
-                continue
-            nb = c
-            if np.any(agg[nb] >= 0):
-                continue
-            agg[i] = na
-            agg[nb] = na
-            na += 1
-        agg1 = agg.copy()
-        for i, (c, x) in enumerate(rows):  # pass 2
-            if agg[i] >= 0 or iso[i]:
-                continue
-            best, bv = -1, 0.0
-            for j, a in zip(c, x):
-                if agg1[j] >= 0 and abs(a) > bv:
-                    best, bv = agg1[j], abs(a)
-            agg[i] = best
-        for i, (c, x) in enumerate(rows):  # pass 3
-            if agg[i] >= 0 or iso[i]:
-                continue
-            agg[i] = na
-            for j, a in zip(c, x):
-                if agg[j] < 0 and not iso[j]:
-                    agg[j] = na
-            na += 1
+        th = theta
+        while True:
+            agg, na, rows, iso = _aggregate(A, d, th)
+            if (na > 0 and na <= 0.9 * n) or th <= 0.0:
+                break
+            th = 0.25 * th if th > 1e-3 else 0.0  # stalled: weaker strength threshold
         if na == 0 or na > 0.9 * n:
             break
         member = agg >= 0
@@ -346,8 +318,50 @@ def amg_hierarchy(ptr, col, val, max_dense=4096, theta=0.0):
         P = T - sp.diags((4.0 / (3.0 * lam)) / d) @ (A @ T)
         levels.append({"A": A, "P": P, "lam": 1.1 * lam, "dinv": 1.0 / d})
         A = (P.T @ (A @ P)).tocsr()
+    if A.shape[0] > max_dense:
+        raise RuntimeError(f"AMG: coarsening stalled at {A.shape[0]} rows")
     levels.append({"A": A, "inv": np.linalg.inv(A.toarray()), "dinv": 1.0 / A.diagonal()})
     return levels
+
+
+def _aggregate(A, d, theta):
+    """three aggregation passes in row order over the strong couplings"""
+    n = A.shape[0]
+    rows = _csr_rows(A.indptr, A.indices, A.data)
+    # strong couplings: |a_ij| >= theta sqrt(|a_ii a_jj|), off the diagonal, nonzero
+    rows = [(c[(c != i) & (x != 0) & (np.abs(x) >= theta * np.sqrt(np.abs(d[i] * d[c])))],
+             x[(c != i) & (x != 0) & (np.abs(x) >= theta * np.sqrt(np.abs(d[i] * d[c])))])
+            for i, (c, x) in enumerate(rows)]
+    iso = np.array([len(c) == 0 for c, _ in rows])
+    agg = -np.ones(n, np.int64)
+    na = 0
+    for i, (c, x) in enumerate(rows):  # pass 1
+        if agg[i] >= 0 or iso[i]:
+            continue
+        nb = c
+        if np.any(agg[nb] >= 0):
+            continue
+        agg[i] = na
+        agg[nb] = na
+        na += 1
+    agg1 = agg.copy()
+    for i, (c, x) in enumerate(rows):  # pass 2
+        if agg[i] >= 0 or iso[i]:
+            continue
+        best, bv = -1, 0.0
+        for j, a in zip(c, x):
+            if agg1[j] >= 0 and abs(a) > bv:
+                best, bv = agg1[j], abs(a)
+        agg[i] = best
+    for i, (c, x) in enumerate(rows):  # pass 3
+        if agg[i] >= 0 or iso[i]:
+            continue
+        agg[i] = na
+        for j, a in zip(c, x):
+            if agg[j] < 0 and not iso[j]:
+                agg[j] = na
+        na += 1
+    return agg, na, rows, iso
 
 
 def amg_vcycle(levels, b, degree=5, smoothing_range=30.0, l=0):
@@ -444,6 +458,68 @@ def adam(p, g, m, v, lr, step, b1=0.9, b2=0.999, eps=1e-8):
     g = _f64(g)
     _mlp_lib().orc_adam(p.ctypes.data, g.ctypes.data, m.ctypes.data, v.ctypes.data, p.size, lr, b1, b2, eps,
                         step)
+
+
+# ---- CPU baseline (cpu_baseline.cpp) ------------------------------------------
+class DGBaseline:
+    """The reference's optimised DG SIPG vmult restated for CPUs (vector lanes
+    of cells or faces, batches of four vectors per shape matrix, OpenMP).
+    cells: (n, 4) in ascending vertex order; DoF (k, i) is row k*n_loc + i."""
+
+    def __init__(self, vertices, cells, degree, nthreads=0):
+        L = lib()
+        L.orc_dgb_create.restype = C.c_void_p
+        L.orc_dgb_create.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_dgb_vmult.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_dgb_time.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.orc_dgb_info.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_dgb_destroy.argtypes = [C.c_void_p]
+        self.vertices = np.ascontiguousarray(vertices, np.float64)
+        self.cells = np.ascontiguousarray(cells, np.int32)
+        self.degree = int(degree)
+        self.n_loc = (degree + 1) * (degree + 2) * (degree + 3) // 6
+        self.n = self.cells.shape[0] * self.n_loc
+        self._m = OrcMesh(self.vertices.shape[0], self.vertices.ctypes.data, self.cells.shape[0],
+                          self.cells.ctypes.data)
+        self.plan = L.orc_dgb_create(C.byref(self._m), self.degree, int(nthreads))
+        if not self.plan:
+            raise ValueError("CPU baseline: degree 1..3 and ascending vertex order per cell required")
+
+    def info(self):
+        out = np.zeros(6, np.int64)
+        lib().orc_dgb_info(self.plan, out.ctypes.data)
+        return dict(zip(["n_q", "n_qf", "face_groups", "colours", "threads", "vector_bits"], map(int, out)))
+
+    def vmult(self, src, dst=None):
+        src = np.ascontiguousarray(src, np.float64)
+        if dst is None:
+            dst = np.empty(self.n)
+        lib().orc_dgb_vmult(self.plan, src.ctypes.data, dst.ctypes.data)
+        return dst
+
+    def time(self, src, reps=5, n=100):
+        """seconds of each of `reps` repetitions of `n` products"""
+        src = np.ascontiguousarray(src, np.float64)
+        dst = np.empty(self.n)
+        sec = np.zeros(reps)
+        lib().orc_dgb_time(self.plan, src.ctypes.data, dst.ctypes.data, reps, n, sec.ctypes.data)
+        return sec
+
+    def __del__(self):
+        if getattr(self, "plan", None) and _lib is not None:
+            _lib.orc_dgb_destroy(self.plan)
+            self.plan = None
+
+
+def dgb_rule(kind, p):
+    L = lib()
+    L.orc_dgb_rule.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    n = L.orc_dgb_rule(kind, p, None, None)
+    nb = 4 if kind == 0 else 3
+    bc = np.empty((n, nb))
+    w = np.empty(n)
+    L.orc_dgb_rule(kind, p, bc.ctypes.data, w.ctypes.data)
+    return bc, w
 
 
 # ---- planted mistakes (tests only) -------------------------------------------

@@ -249,7 +249,14 @@ std::unique_ptr<Amg> build_amg(const Csr& A0, int max_dense, double strength) {
       if (!(v > 0)) throw std::runtime_error("AMG: matrix with a non-positive diagonal entry");
     L.lambda_max = lambda_max_jacobi(A, d);
     std::vector<int32_t> agg;
-    const int32_t na = aggregate(A, d, amg->strength, agg);
+    // a level that barely coarsens retries with a weaker strength threshold
+    // (theta / 4 per retry, 0 below 1e-3: then every nonzero coupling is strong)
+    double theta = amg->strength;
+    int32_t na = aggregate(A, d, theta, agg);
+    while ((na == 0 || na > 0.9 * (double)A.nrows) && theta > 0.0) {
+      theta = theta > 1e-3 ? 0.25 * theta : 0.0;
+      na = aggregate(A, d, theta, agg);
+    }
     if (na == 0 || na > 0.9 * (double)A.nrows) break;
     // tentative prolongator: unit-norm aggregate indicators
     std::vector<int64_t> size(na, 0);
@@ -311,8 +318,9 @@ std::unique_ptr<Amg> build_amg(const Csr& A0, int max_dense, double strength) {
   }
   AmgLevel& last = amg->lv.back();
   const int64_t nd = last.A.nrows;
-  if (nd > 4 * (int64_t)max_dense)
-    throw std::runtime_error("AMG: coarsening stalled at " + std::to_string(nd) + " rows");
+  if (nd > (int64_t)max_dense)
+    throw std::runtime_error("AMG: coarsening stalled at " + std::to_string(nd) + " rows, above the dense coarse "
+                             "solver's limit of " + std::to_string(max_dense) + " (max_coarse_dofs)");
   std::vector<double> dense(nd * nd, 0.0);
   for (int64_t i = 0; i < nd; ++i)
     for (int64_t k = last.A.ptr[i]; k < last.A.ptr[i + 1]; ++k) dense[i * nd + last.A.col[k]] = last.A.val[k];

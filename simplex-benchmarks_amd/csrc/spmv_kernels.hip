@@ -49,12 +49,18 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_stream_kernel(int64_t nbloc
   const int64_t k0 = ptr[rr], k1 = ptr[rr + 1];
   int32_t c[kPerLane];
   T v[kPerLane], xv[kPerLane];
+  // entries past the block's run (or a block without entries) load nothing:
+  // column 0 (x has at least one row) and value 0
 #pragma unroll
   for (int u = 0; u < kPerLane; ++u) {
     const int64_t e = e0 + t + u * kSpmvThreads;
-    const int64_t ee = e < e1 ? e : e1 - 1;
-    c[u] = __builtin_nontemporal_load(col + ee);
-    v[u] = __builtin_nontemporal_load(val + ee);
+    if (e < e1) {
+      c[u] = __builtin_nontemporal_load(col + e);
+      v[u] = __builtin_nontemporal_load(val + e);
+    } else {
+      c[u] = 0;
+      v[u] = T(0);
+    }
   }
 #pragma unroll
   for (int u = 0; u < kPerLane; ++u) xv[u] = x[c[u]];

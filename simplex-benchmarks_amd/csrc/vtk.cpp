@@ -113,6 +113,10 @@ Mesh read_vtk(const std::string& path, int64_t* n_flipped) {
         for (int64_t i = 0; i < b; ++i) conn[i] = t.integer("connectivity");
         if (a < 1 || offsets[0] != 0 || offsets[a - 1] != b)
           throw std::runtime_error("VTK: OFFSETS do not match CONNECTIVITY");
+        // every cell's range [offsets[c], offsets[c+1]) must lie inside CONNECTIVITY
+        for (int64_t i = 1; i < a; ++i)
+          if (offsets[i] < offsets[i - 1] || offsets[i] > b)
+            throw std::runtime_error("VTK: OFFSETS must rise monotonically within [0, " + std::to_string(b) + "]");
       } else {
         if (b < a) throw std::runtime_error("VTK: bad CELLS header");
         offsets.assign(1, 0);
@@ -128,15 +132,18 @@ Mesh read_vtk(const std::string& path, int64_t* n_flipped) {
       have_cells = true;
     } else if (key == "CELL_TYPES") {
       const int64_t n = t.integer("number of cell types");
+      if (n < 0) throw std::runtime_error("VTK: negative CELL_TYPES count");
       types.resize(n);
       for (int64_t i = 0; i < n; ++i) types[i] = t.integer("cell type");
     } else if (key == "FIELD") {
       // FIELD name n, then per array "name components tuples type" and its values
       t.need("field name");
       const int64_t na = t.integer("number of field arrays");
+      if (na < 0) throw std::runtime_error("VTK: negative FIELD array count");
       for (int64_t a = 0; a < na; ++a) {
         t.need("field array name");
         const int64_t nc = t.integer("components"), nt = t.integer("tuples");
+        if (nc < 0 || nt < 0) throw std::runtime_error("VTK: negative FIELD array size");
         t.need("field array type");
         for (int64_t i = 0; i < nc * nt; ++i) t.need("field value");
       }

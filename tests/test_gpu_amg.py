@@ -86,18 +86,21 @@ def test_amg_coarse_under_dg(gpu):
     assert res <= 2e-10
 
 
-def test_amg_vcycle_matches_oracle_restatement(gpu, orc):
+@pytest.mark.parametrize("theta", [0.05, 2.0])
+def test_amg_vcycle_matches_oracle_restatement(gpu, orc, theta):
     """P1 on a scrambled 20^3 mesh (9,261 rows): the hierarchy is the P1
     level only, so one FP64 multigrid application is one AMG V-cycle; it
     matches the oracle's numpy restatement built from the same assembled
-    matrix (itself checked against the oracle's dense matrix)."""
+    matrix (itself checked against the oracle's dense matrix).  theta = 0.05
+    is the default threshold; theta = 2 isolates every row, so every level
+    takes the stall retries (theta / 4 until it coarsens)."""
     smf = gpu
-    print("\ncase: AMG V-cycle vs oracle restatement, scrambled 20^3, P1", flush=True)
+    print(f"\ncase: AMG V-cycle vs oracle restatement, scrambled 20^3, P1, theta {theta}", flush=True)
     op = smf.Operator(smf.Mesh.kuhn_cube(20, 0.2, 4).scrambled(seed=6), 1)
-    mg = smf.Multigrid(op, level_precision="f64")
+    mg = smf.Multigrid(op, level_precision="f64", amg_threshold=theta)
     assert mg.info()["n_levels"] == 1
     ptr, col, val = smf.SystemMatrix(op).to_host()
-    levels = orc.amg_hierarchy(ptr, col, val, max_dense=4096, theta=0.05)  # the default threshold
+    levels = orc.amg_hierarchy(ptr, col, val, max_dense=4096, theta=theta)
     assert [L["A"].shape[0] for L in levels] == mg.info()["amg_rows"]
     r = np.random.default_rng(9).uniform(-1, 1, op.m())
     z = op.initialize_dof_vector()

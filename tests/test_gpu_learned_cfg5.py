@@ -5,7 +5,9 @@ train_smoother.py on 4096 right-hand sides).
 
 Checks, for an unseen right-hand side:
   * the learned step stays on (no fallback) and the solve converges to 1e-10;
-  * it takes no more iterations than the same V-cycle without the step;
+  * CG: it takes fewer iterations than the same V-cycle without the step;
+    DG: no more (the DG network gains nothing at 48^3: 15 against 15, see
+    its training metadata and the design notes);
   * its error against an FP64 reference solution is within 2x of the plain
     solver's, in the energy norm.  The reference solution is a 1e-13 plain
     multigrid solve whose residual is checked with the oracle's operator (one
@@ -59,5 +61,8 @@ def test_committed_network_at_48cube(gpu, orc, kind, name):
           f"{sl['mean_omega']:.3f}, energy error plain {ep:.3e} learned {el:.3e}", flush=True)
     assert sp["converged"] and sl["converged"]
     assert sl["fallback"] == 0 and sl["learned_steps"] == sl["iterations"]
-    assert sl["iterations"] <= sp["iterations"]
+    if kind == "cg":
+        assert sl["iterations"] < sp["iterations"]
+    else:
+        assert sl["iterations"] <= sp["iterations"]
     assert el <= 2.0 * ep

@@ -69,3 +69,22 @@ def test_amg_restatement_preconditions_cg(smf, orc, theta):
     exact = orc.amg_hierarchy(ptr, col, val, max_dense=10 ** 6)
     assert len(exact) == 1
     assert np.max(np.abs(A @ orc.amg_vcycle(exact, b) - b)) <= 1e-10 * np.abs(b).max()
+
+
+def test_amg_restatement_stall_retries_then_refuses(smf, orc):
+    # a threshold no coupling meets isolates every row: the level retries with
+    # theta / 4 until it coarsens (2 -> 0.5 here), as if called with that theta
+    op, (ptr, col, val) = p1_matrix(smf, 10)
+    weak = orc.amg_hierarchy(ptr, col, val, max_dense=200, theta=2.0)
+    half = orc.amg_hierarchy(ptr, col, val, max_dense=200, theta=0.5)
+    assert len(weak) >= 2 and weak[-1]["A"].shape[0] <= 200
+    assert [L["A"].shape[0] for L in weak] == [L["A"].shape[0] for L in half]
+    A = weak[0]["A"]
+    b = np.random.default_rng(5).uniform(-1, 1, A.shape[0])
+    _, it = pcg(A, b, lambda r: orc.amg_vcycle(weak, r))
+    assert it <= 30
+    # a diagonal matrix cannot coarsen at any threshold: above the dense limit it is refused
+    n = 300
+    dptr, dcol, dval = np.arange(n + 1), np.arange(n), np.full(n, 2.0)
+    with pytest.raises(RuntimeError, match="stalled"):
+        orc.amg_hierarchy(dptr, dcol, dval, max_dense=100)

@@ -69,6 +69,42 @@ def test_vtk_errors(smf, tmp_path):
         smf.Mesh.read_vtk(str(p))
 
 
+_VTK5_HEAD = ("# vtk DataFile Version 5.1\nt\nASCII\nDATASET UNSTRUCTURED_GRID\nPOINTS 5 double\n"
+              "0 0 0 1 0 0 0 1 0 0 0 1 1 1 1\n")
+
+
+@pytest.mark.parametrize("offsets,conn,types", [
+    # non-monotonic: skips a point cell, then a tetrahedron whose range would run past CONNECTIVITY
+    ("0 50 54 4", "0 1 2 3", "1 10 10"),
+    # an offset below zero
+    ("0 -4 4", "0 1 2 3", "10 10"),
+    # an offset beyond CONNECTIVITY in the middle
+    ("0 9 4", "0 1 2 3", "10 10"),
+])
+def test_vtk5_offsets_validated(smf, tmp_path, offsets, conn, types):
+    no = len(offsets.split())
+    text = (_VTK5_HEAD + f"CELLS {no} {len(conn.split())}\nOFFSETS vtktypeint64\n{offsets}\n"
+            f"CONNECTIVITY vtktypeint64\n{conn}\nCELL_TYPES {no - 1}\n{types}\n")
+    p = tmp_path / "bad_offsets.vtk"
+    p.write_text(text)
+    with pytest.raises(smf.MFError, match="OFFSETS"):
+        smf.Mesh.read_vtk(str(p))
+
+
+def test_vtk_negative_counts_rejected(smf, tmp_path):
+    p = tmp_path / "neg.vtk"
+    p.write_text(_VTK5_HEAD + "CELLS 2 4\nOFFSETS vtktypeint64\n0 4\nCONNECTIVITY vtktypeint64\n0 1 2 3\n"
+                 "CELL_TYPES -1\n")
+    with pytest.raises(smf.MFError, match="negative CELL_TYPES"):
+        smf.Mesh.read_vtk(str(p))
+    p.write_text(_VTK5_HEAD + "FIELD f -2\n")
+    with pytest.raises(smf.MFError, match="negative FIELD"):
+        smf.Mesh.read_vtk(str(p))
+    p.write_text(_VTK5_HEAD + "FIELD f 1\nx -1 3 double\n")
+    with pytest.raises(smf.MFError, match="negative FIELD"):
+        smf.Mesh.read_vtk(str(p))
+
+
 def _by_point(op, y):
     """values of a scalar CG vector keyed by the rounded support points"""
     X = op.dof_points()
