@@ -214,6 +214,16 @@ int mf_mg_destroy(mf_mg_t mg);
  * NULL; flexible != 0 uses Notay's flexible CG.  Synchronises the stream. */
 int mf_solve_cg(mf_op_t A, mf_mg_t mg, void* x, const void* b, double rel_tol, int32_t max_iter,
                 int32_t flexible, mf_solve_stats* stats, void* stream);
+/* mf_solve_cg that allocates no device memory.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The solver's work
+ * vectors, reduction partials and Krylov scalars live in a static device
+ * array of the library (32 MiB per GPU, held until the process ends);
+ * operators with more than 1048576 rows are refused.  All calls on one GPU
+ * share that array: calls that overlap on one GPU race.  Results equal
+ * mf_solve_cg's bitwise (same kernels, same order). */
+int mf_solve_cg_noalloc(mf_op_t A, mf_mg_t mg, void* x, const void* b, double rel_tol, int32_t max_iter,
+                        int32_t flexible, mf_solve_stats* stats, void* stream);
 /* Host right-hand side b_i = int f phi_i (Dirichlet rows 0) of a CG operator:
  * kind 0 f = 3 (3 pi)^2 prod sin(3 pi x_d); kind 1 seeded Gaussian random
  * field (random Fourier features, n_features, correlation length). */

@@ -701,18 +701,42 @@ void mg_vmult(Multigrid& mg, void* z, const void* r, hipStream_t s) {
   mg.vcycles++;
 }
 
+int64_t cg_workspace_doubles(int64_t n) {
+  const int64_t nv = (n + 31) / 32 * 32;  // 256-byte aligned vectors
+  return 4 * nv + (blas_reduction_parts() + 31) / 32 * 32 + 32 + kS_N;
+}
+
+CgWorkspace cg_workspace_at(double* base, int64_t n) {
+  const int64_t nv = (n + 31) / 32 * 32;
+  CgWorkspace ws;
+  ws.r = base;
+  ws.z = base + nv;
+  ws.p = base + 2 * nv;
+  ws.q = base + 3 * nv;
+  ws.partial = base + 4 * nv;
+  ws.scalar = ws.partial + (blas_reduction_parts() + 31) / 32 * 32;
+  ws.S = ws.scalar + 32;
+  return ws;
+}
+
 SolveStats solve_cg(const Operator& A, Multigrid* mg, double* x, const double* b, double rel_tol, int max_iter,
                     bool flexible, hipStream_t s) {
   if (A.precision != MF_F64) throw std::runtime_error("the outer solver runs in FP64");
   const int64_t n = A.m();
-  DevBuf r, z, p, q, zold, partial, scalar;
-  for (DevBuf* v : {&r, &z, &p, &q}) v->alloc(n * 8);
-  partial.alloc(blas_reduction_parts() * sizeof(double));
-  scalar.alloc(sizeof(double));
+  DevBuf work;
+  work.alloc(cg_workspace_doubles(n) * sizeof(double));
+  return solve_cg_ws(A, mg, x, b, rel_tol, max_iter, flexible, cg_workspace_at(work.as<double>(), n), s);
+}
+
+SolveStats solve_cg_ws(const Operator& A, Multigrid* mg, double* x, const double* b, double rel_tol, int max_iter,
+                       bool flexible, const CgWorkspace& ws, hipStream_t s) {
+  if (A.precision != MF_F64) throw std::runtime_error("the outer solver runs in FP64");
+  const int64_t n = A.m();
+  double *r = ws.r, *z = ws.z, *p = ws.p, *q = ws.q, *partial = ws.partial, *scalar = ws.scalar;
   auto dot = [&](const double* u, const double* v) {
-    blas_dot<double>(n, u, v, partial.as<double>(), scalar.as<double>(), s);
+    blas_dot<double>(n, u, v, partial, scalar, s);
     double h;
-    SMF_HIP(hipMemcpyAsync(&h, scalar.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    SMF_HIP(hipMemcpyAsync(&h, scalar, sizeof(double), hipMemcpyDeviceToHost, s));
     SMF_HIP(hipStreamSynchronize(s));
     return h;
   };
@@ -721,9 +745,8 @@ SolveStats solve_cg(const Operator& A, Multigrid* mg, double* x, const double* b
     else SMF_HIP(hipMemcpyAsync(zz, rr, n * 8, hipMemcpyDeviceToDevice, s));
   };
   SolveStats st;
-  (void)zold;
   blas_fill<double>(n, 0.0, x, s);
-  SMF_HIP(hipMemcpyAsync(r.p, b, n * 8, hipMemcpyDeviceToDevice, s));
+  SMF_HIP(hipMemcpyAsync(r, b, n * 8, hipMemcpyDeviceToDevice, s));
   st.b_norm = std::sqrt(dot(b, b));
   st.r0 = st.b_norm;
   st.rn = st.r0;
@@ -734,19 +757,18 @@ SolveStats solve_cg(const Operator& A, Multigrid* mg, double* x, const double* b
   }
   // alpha, beta and the dot products stay in device memory (S); the host
   // reads |r|^2 once per iteration for the stopping test
-  DevBuf S;
-  S.alloc(kS_N * sizeof(double));
-  double* Sd = S.as<double>();
-  precond(z.as<double>(), r.as<double>());
-  SMF_HIP(hipMemcpyAsync(p.p, z.p, n * 8, hipMemcpyDeviceToDevice, s));
-  blas_dot<double>(n, r.as<double>(), z.as<double>(), partial.as<double>(), Sd + kS_RZ, s);
+  double* Sd = ws.S;
+  blas_fill<double>(kS_N, 0.0, Sd, s);  // a fill kernel (no memset node under graph capture)
+  precond(z, r);
+  SMF_HIP(hipMemcpyAsync(p, z, n * 8, hipMemcpyDeviceToDevice, s));
+  blas_dot<double>(n, r, z, partial, Sd + kS_RZ, s);
   for (int it = 1; it <= max_iter; ++it) {
-    launch_op(A, 0, q.p, p.p, s);
-    blas_dot<double>(n, p.as<double>(), q.as<double>(), partial.as<double>(), Sd + kS_PQ, s);
-    blas_krylov_scalars(Sd, 0, s);                                        // alpha = rz / pq
-    blas_axpby_dev(n, Sd, kS_ALPHA, 1.0, p.as<double>(), -1, x, s);               // x += alpha p
-    blas_axpby_dev(n, Sd, kS_ALPHA, -1.0, q.as<double>(), -1, r.as<double>(), s);  // r -= alpha q
-    blas_dot<double>(n, r.as<double>(), r.as<double>(), partial.as<double>(), Sd + kS_RR, s);
+    launch_op(A, 0, q, p, s);
+    blas_dot<double>(n, p, q, partial, Sd + kS_PQ, s);
+    blas_krylov_scalars(Sd, 0, s);                             // alpha = rz / pq
+    blas_axpby_dev(n, Sd, kS_ALPHA, 1.0, p, -1, x, s);         // x += alpha p
+    blas_axpby_dev(n, Sd, kS_ALPHA, -1.0, q, -1, r, s);        // r -= alpha q
+    blas_dot<double>(n, r, r, partial, Sd + kS_RR, s);
     double rr;
     SMF_HIP(hipMemcpyAsync(&rr, Sd + kS_RR, sizeof(double), hipMemcpyDeviceToHost, s));
     SMF_HIP(hipStreamSynchronize(s));
@@ -756,13 +778,13 @@ SolveStats solve_cg(const Operator& A, Multigrid* mg, double* x, const double* b
       st.converged = st.rn <= tol;
       break;
     }
-    precond(z.as<double>(), r.as<double>());
-    blas_dot<double>(n, r.as<double>(), z.as<double>(), partial.as<double>(), Sd + kS_RZN, s);
+    precond(z, r);
+    blas_dot<double>(n, r, z, partial, Sd + kS_RZN, s);
     // flexible CG (Notay 2000): beta = z_new.(r_new - r_old)/rz_old with
     // r_new - r_old = -alpha q
-    if (flexible) blas_dot<double>(n, z.as<double>(), q.as<double>(), partial.as<double>(), Sd + kS_ZQ, s);
+    if (flexible) blas_dot<double>(n, z, q, partial, Sd + kS_ZQ, s);
     blas_krylov_scalars(Sd, flexible ? 2 : 1, s);
-    blas_axpby_dev(n, Sd, -1, 1.0, z.as<double>(), kS_BETA, p.as<double>(), s);  // p = z + beta p
+    blas_axpby_dev(n, Sd, -1, 1.0, z, kS_BETA, p, s);  // p = z + beta p
   }
   SMF_HIP(hipStreamSynchronize(s));
   return st;

@@ -80,6 +80,19 @@ struct SolveStats {
 // application_hp.cc:1950-1960); mg may be null (unpreconditioned)
 SolveStats solve_cg(const Operator& A, Multigrid* mg, double* x, const double* b, double rel_tol,
                     int max_iter, bool flexible, hipStream_t s);
+// device work memory of solve_cg: four vectors of A.m() doubles, the
+// reduction partials (blas_reduction_parts() doubles), one scalar and the
+// Krylov scalar block (kS_N doubles); every entry is written before it is read
+struct CgWorkspace {
+  double *r, *z, *p, *q, *partial, *scalar, *S;
+};
+// doubles solve_cg_ws needs for n rows
+int64_t cg_workspace_doubles(int64_t n);
+// CgWorkspace carved out of `base` (cg_workspace_doubles(n) doubles, 256-byte aligned)
+CgWorkspace cg_workspace_at(double* base, int64_t n);
+// solve_cg on caller-provided work memory: allocates nothing on the device
+SolveStats solve_cg_ws(const Operator& A, Multigrid* mg, double* x, const double* b, double rel_tol,
+                       int max_iter, bool flexible, const CgWorkspace& ws, hipStream_t s);
 
 // b_i = int f phi_i on a CG operator (Dirichlet rows 0): kind 0 f = 3(3pi)^2
 // prod sin(3 pi x_d) (exact solution prod sin(3 pi x_d)); kind 1 a seeded

@@ -118,6 +118,7 @@ def lib():
         "mf_mg_transfer": (C.c_int, [vp, i32, i32, vp, vp, vp]),
         "mf_mg_level_dofs": (C.c_int, [vp, i32, vp, vp]),
         "mf_solve_cg": (C.c_int, [vp, vp, vp, vp, C.c_double, i32, i32, C.POINTER(SolveStats), vp]),
+        "mf_solve_cg_noalloc": (C.c_int, [vp, vp, vp, vp, C.c_double, i32, i32, C.POINTER(SolveStats), vp]),
         "mf_assemble_rhs": (C.c_int, [vp, i32, u64, i32, C.c_double, vp]),
         "mf_exact_solution": (C.c_int, [vp, vp]),
         "mf_ls_default_train_options": (None, [C.POINTER(TrainOptions)]),
@@ -234,7 +235,7 @@ _SIGNATURE_NAMES = [
     "mf_synchronize", "mf_stream_create", "mf_stream_destroy", "mf_stream_synchronize",
     "mf_event_create", "mf_event_destroy", "mf_event_record", "mf_event_elapsed_ms",
     "mf_mg_default_options", "mf_mg_create", "mf_mg_vmult", "mf_mg_get_info", "mf_mg_destroy",
-    "mf_mg_transfer", "mf_mg_level_dofs", "mf_solve_cg", "mf_assemble_rhs", "mf_exact_solution",
+    "mf_mg_transfer", "mf_mg_level_dofs", "mf_solve_cg", "mf_solve_cg_noalloc", "mf_assemble_rhs", "mf_exact_solution",
     "mf_ls_default_train_options", "mf_ls_default_fcg_options", "mf_ls_create", "mf_ls_destroy",
     "mf_ls_set_params", "mf_ls_get_params", "mf_ls_save", "mf_ls_load", "mf_ls_apply", "mf_ls_forward",
     "mf_ls_features", "mf_ls_gradient", "mf_ls_train_step", "mf_ls_train", "mf_solve_fcg_learned",
@@ -650,6 +651,17 @@ def solve_cg(A: Operator, x: DeviceVector, b: DeviceVector, mg: Multigrid | None
     st = SolveStats()
     check(lib().mf_solve_cg(A.handle, mg.handle if mg is not None else None, x.ptr, b.ptr, rel_tol,
                             max_iter, 1 if flexible else 0, C.byref(st), stream.handle if stream else None))
+    return st.as_dict()
+
+
+def solve_cg_noalloc(A: Operator, x: DeviceVector, b: DeviceVector, mg: Multigrid | None = None, rel_tol=1e-10,
+                     max_iter=1000, flexible=False, stream=None):
+    """solve_cg with the work memory in the library's static device array
+    (no allocation; operators up to 1048576 rows; calls on one GPU must not
+    overlap)"""
+    st = SolveStats()
+    check(lib().mf_solve_cg_noalloc(A.handle, mg.handle if mg is not None else None, x.ptr, b.ptr, rel_tol,
+                                    max_iter, 1 if flexible else 0, C.byref(st), stream.handle if stream else None))
     return st.as_dict()
 
 
