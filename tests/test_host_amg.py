@@ -73,10 +73,10 @@ def test_amg_restatement_preconditions_cg(smf, orc, theta):
 
 def test_amg_restatement_stall_retries_then_refuses(smf, orc):
     # a threshold no coupling meets isolates every row: the level retries with
-    # theta / 4 until it coarsens (2 -> 0.5 here), as if called with that theta
+    # theta / 4 until it coarsens (0.9 -> 0.225 here), as if called with that theta
     op, (ptr, col, val) = p1_matrix(smf, 10)
-    weak = orc.amg_hierarchy(ptr, col, val, max_dense=200, theta=2.0)
-    half = orc.amg_hierarchy(ptr, col, val, max_dense=200, theta=0.5)
+    weak = orc.amg_hierarchy(ptr, col, val, max_dense=200, theta=0.9)
+    half = orc.amg_hierarchy(ptr, col, val, max_dense=200, theta=0.225)
     assert len(weak) >= 2 and weak[-1]["A"].shape[0] <= 200
     assert [L["A"].shape[0] for L in weak] == [L["A"].shape[0] for L in half]
     A = weak[0]["A"]
