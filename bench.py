@@ -428,7 +428,8 @@ def main():
                 ("dg", 64, 2, "f32", True), ("dg", 64, 3, "f32", True), ("dg", 64, 2, "f64", False),
                 ("dg_helmholtz", 32, 2, "f64", True), ("dg_helmholtz", 32, 2, "f32", True),
                 ("cg", 128, 2, "f64", True), ("cg", 128, 2, "f32", True), ("cg_vector", 64, 2, "f64", True),
-                ("cg", 128, 2, "f64", "spherical"), ("cg", 16, 2, "f64", True), ("cg", 48, 2, "f64", True)]
+                ("cg", 128, 2, "f64", "spherical"), ("dg", 64, 2, "f64", "spherical"),
+                ("cg", 16, 2, "f64", True), ("cg", 48, 2, "f64", True)]
         for kind, n, deg, prec, reorder in todo:
             # a string in the last slot: curved cells on that manifold (reordering on)
             manifold = reorder if isinstance(reorder, str) else "none"
@@ -447,13 +448,18 @@ def main():
                 # local DoF) and the 27 FP64 values of the P2 geometry map per cell (216 B);
                 # it forms J, adj J and det J per point in registers (FP64-VALU-bound)
                 w = 8 if prec == "f64" else 4
-                own = 2 * w * r["dofs"] + 4 * r["info"]["dofs_per_cell"] * r["cells"] + 216 * r["cells"]
+                if kind == "cg":
+                    own = 2 * w * r["dofs"] + 4 * r["info"]["dofs_per_cell"] * r["cells"] + 27 * w * r["cells"]
+                    what = "4 B per local DoF + 27 w of P2 geometry per cell"
+                else:
+                    own = 2 * w * r["dofs"] + (28 * w + 20) * r["cells"]
+                    what = "27 w of P2 geometry + tau + 4 neighbour ids + face numbers per cell"
                 r["algorithmic_bytes"] = own
                 r["achieved_gbs"] = own / (r["ms_per_step"] * 1e-3) / 1e9
                 r["frac"] = r["achieved_gbs"] / HBM_PEAK_GBS
-                r["note"] = ("bytes: the curved kernel's own data model (2 w per DoF + 4 B per local DoF + "
-                             "216 B of P2 geometry per cell); the reference stores J^-T and JxW per "
-                             "quadrature point instead")
+                r["note"] = (f"bytes: the curved kernel's own data model (2 w per DoF + {what}); it forms J, "
+                             "cof J and det J per quadrature point in registers (FP64-VALU-bound); the "
+                             "reference stores J^-T and JxW per quadrature point instead")
             variants.append({k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()
                              if k in ("kind", "degree", "precision", "n", "reorder", "manifold", "dofs", "ms_per_step",
                                       "gdofs", "achieved_gbs", "frac", "note", "cache_resident",

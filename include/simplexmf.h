@@ -63,13 +63,15 @@ typedef struct mf_options {
                                multigrid, DG/CG c-transfer, application_phc.cc:1424-1456, 1798-1832) */
   int32_t cells_per_wave;   /* CG: cells per wavefront of a patch workgroup (0: 1024 P1, 512 P2, 128 P3);
                                waves per patch = patch_cells / cells_per_wave, at most 4 */
-  int32_t manifold;         /* CG, Dirichlet boundary: 0 affine cells (default); 1 curved cells: a P2
+  int32_t manifold;         /* CG (Dirichlet boundary) and DG: 0 affine cells (default); 1 curved cells: a P2
                                geometry map whose edge nodes lie on the spherical manifold around
                                manifold_center (MappingFE(FE_SimplexP(2)) + SphericalManifold,
                                CG/cells_batched/application.cc:1613, 1668-1677, 1763-1767), integrated
                                per quadrature point (the general D_e branch, :1361-1376) with a rule of
                                degree 2p; 2 the same P2 map with straight-edge midpoints (affine cells
-                               through the curved-cell path) */
+                               through the curved-cell path).  DG: curved faces with per-point normals
+                               and J^-1 n on both sides (DG/cells_batched/application.cc:2191-2238), tau
+                               from the curved volumes and face areas; no multigrid below it */
   double manifold_center[3];  /* centre of the spherical manifold (default (5, 0, 0), as the reference) */
 } mf_options;
 
@@ -145,11 +147,19 @@ int mf_get_cell_vertices(mf_op_t op, int32_t* verts);
 int mf_get_dof_points(mf_op_t op, double* xyz);
 /* 1 for Dirichlet-constrained scalar DoFs (n_scalar_dofs bytes) */
 int mf_get_constrained(mf_op_t op, uint8_t* mask);
-/* the cell quadrature rule of a CG operator (deal.II's QGaussSimplex(p+1),
+/* the cell quadrature rule of an operator (deal.II's QGaussSimplex(p+1),
  * CG/cells_batched/application.cc:1614, is this library's own symmetric rule
- * of degree 2p-2, or 2p for curved cells): n_q_points points on the unit
- * tetrahedron (3 doubles each) and weights summing to 1/6 */
+ * of degree 2p-2, or 2p for curved cells and the Helmholtz mass term):
+ * n_q_points points on the unit tetrahedron (3 doubles each) and weights
+ * summing to 1/6 */
 int mf_get_quadrature(mf_op_t op, double* points, double* weights);
+/* the face rule of a DG operator (degree 2p): *n_points points (s, t) of the
+ * unit triangle (2 doubles each), weights summing to 1/2.  On the face of a
+ * cell with local vertices a < b < c (ascending global ids) the point is
+ * x_a + s (x_b - x_a) + t (x_c - x_a) in reference coordinates; both cells of
+ * a face therefore evaluate it at the same physical point.  Call with
+ * points == weights == NULL for the count. */
+int mf_get_face_quadrature(mf_op_t op, int32_t* n_points, double* points, double* weights);
 
 const char* mf_last_error(void);
 

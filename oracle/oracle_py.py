@@ -729,3 +729,139 @@ def mg_vcycle(levels, b, l=None):
     xc = mg_vcycle(levels, L["P"].T @ r, l - 1)
     x = x + L["P"] @ xc
     return chebyshev(A, dinv, b, x, L["degree"], L["lmax"], L["lmin"])
+
+
+# ---- curved DG cells (numpy restatement) ---------------------------------------
+# Follows DG/cells_batched/application.cc:1614-1666 (general D_e:
+# g <- JxW_q jac_q^T jac_q g) and :2191-2238 (curvilinear SIPG face flux with
+# per-point normal x jacobian on both sides), with the penalty of :1193-1245,
+# 2120-2123 evaluated on the curved geometry.  Geometry: each cell's P2 map
+# through its vertices (ascending global order) and the manifold's edge
+# midpoints (curved_nodes).  Independent of the product's code: the oracle's
+# own P2 and P_p bases (orc_basis), the Jacobians from the basis gradients,
+# the face area element as |dx/ds x dx/dt| of the face parametrisation, and
+# an explicit check that both sides' points coincide.  The rules are inputs
+# (the operator's, exactness checked separately).
+
+_REF_V = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+_FACE_V = [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]  # face f: opposite vertex f, ascending
+
+
+def _p2_jac(nodes_k, xi):
+    """J (3x3) of the P2 map with nodes (10, 3) at reference point xi"""
+    g = basis(2, xi)[1]                    # (10, 3) gradients of the geometry basis
+    return np.einsum("id,ie->de", nodes_k, g)
+
+
+def _p2_map(nodes_k, xi):
+    return basis(2, xi)[0] @ nodes_k
+
+
+def dg_curved_matrix(p, vertices, cells, pts, w, fpts, fw, ncomp=1, mass=0.0, laplace=1.0,
+                     center=(5.0, 0.0, 0.0), straight=False):
+    """sparse matrix of the curved-cell DG SIPG operator; rows
+    (k*ncomp + c)*n_loc + i; cells with vertices in ascending order"""
+    import scipy.sparse as sp
+    cells = np.asarray(cells)
+    nc = cells.shape[0]
+    nl = (p + 1) * (p + 2) * (p + 3) // 6
+    nodes = curved_nodes(vertices, cells, center, straight)
+    factor = (p + 1) * (p + 3) / 3.0
+    # faces
+    faces = {}
+    for k in range(nc):
+        for f, fv in enumerate(_FACE_V):
+            faces.setdefault(tuple(sorted(cells[k, list(fv)])), []).append((k, f))
+
+    def face_point(f, s, t):
+        a, b, c = _FACE_V[f]
+        return _REF_V[a] + s * (_REF_V[b] - _REF_V[a]) + t * (_REF_V[c] - _REF_V[a])
+
+    def face_geom(k, f, s, t):
+        """physical point, area element, outward unit normal at face point (s, t)"""
+        a, b, c = _FACE_V[f]
+        xi = face_point(f, s, t)
+        J = _p2_jac(nodes[k], xi)
+        ts, tt = J @ (_REF_V[b] - _REF_V[a]), J @ (_REF_V[c] - _REF_V[a])
+        A = np.cross(ts, tt)
+        # outward: n is normal to the face tangents J e1, J e2, so J^T n is
+        # normal to the reference face; it must point out of the reference cell
+        xq = _p2_map(nodes[k], xi)
+        n = A / np.linalg.norm(A)
+        out_ref = _REF_V[a] - _REF_V[f]     # from the opposite vertex to the face
+        if (J.T @ n) @ out_ref < 0:
+            n = -n
+        return xq, np.linalg.norm(A), n, J, xi
+
+    # volumes and surfaces for tau
+    vol = np.zeros(nc)
+    for k in range(nc):
+        for x, wq in zip(pts, w):
+            vol[k] += wq * abs(np.linalg.det(_p2_jac(nodes[k], x)))
+    surf = np.zeros(nc)
+    for key, sides in faces.items():
+        for k, f in sides:
+            area = sum(wq * face_geom(k, f, s, t)[1] for (s, t), wq in zip(fpts, fw))
+            surf[k] += area * (1.0 if len(sides) == 1 else 0.5)
+    tau = surf / vol
+    n = nc * nl
+    rows, cols, vals = [], [], []
+
+    def add(block, ka, kb):
+        ia, ib = np.arange(nl) + ka * nl, np.arange(nl) + kb * nl
+        rows.append(np.repeat(ia, nl))
+        cols.append(np.tile(ib, nl))
+        vals.append(block.ravel())
+
+    for k in range(nc):
+        Ke = np.zeros((nl, nl))
+        for x, wq in zip(pts, w):
+            J = _p2_jac(nodes[k], x)
+            v, g = basis(p, x)
+            gp = g @ np.linalg.inv(J)          # physical gradients (nl, 3)
+            dj = abs(np.linalg.det(J))
+            Ke += wq * dj * (laplace * gp @ gp.T + mass * np.outer(v, v))
+        add(Ke, k, k)
+    for key, sides in faces.items():
+        (km, fm) = sides[0]
+        Bmm, Bmp, Bpm, Bpp = (np.zeros((nl, nl)) for _ in range(4))
+        sig = factor * (max(tau[km], tau[sides[1][0]]) if len(sides) == 2 else tau[km])
+        for (s, t), wq in zip(fpts, fw):
+            xq, da, nrm, J, xi = face_geom(km, fm, s, t)
+            vm, gm = basis(p, xi)
+            dm = gm @ (np.linalg.inv(J) @ nrm)
+            jw = wq * da * laplace
+            if len(sides) == 2:
+                kp, fp = sides[1]
+                xqp, _, _, Jp, xip = face_geom(kp, fp, s, t)
+                assert np.max(np.abs(xqp - xq)) < 1e-11, "the two sides' face points differ"
+                vp, gpp = basis(p, xip)
+                dp = gpp @ (np.linalg.inv(Jp) @ nrm)
+                Jv = np.concatenate([vm, -vp])
+                Av = 0.5 * np.concatenate([dm, dp])
+                B = jw * (sig * np.outer(Jv, Jv) - np.outer(Jv, Av) - np.outer(Av, Jv))
+                Bmm += B[:nl, :nl]
+                Bmp += B[:nl, nl:]
+                Bpm += B[nl:, :nl]
+                Bpp += B[nl:, nl:]
+            else:
+                Bmm += jw * (2 * sig * np.outer(vm, vm) - np.outer(vm, dm) - np.outer(dm, vm))
+        add(Bmm, km, km)
+        if len(sides) == 2:
+            kp = sides[1][0]
+            add(Bmp, km, kp)
+            add(Bpm, kp, km)
+            add(Bpp, kp, kp)
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    if ncomp == 1:
+        return A
+    # components share the geometry: row (k ncomp + c) nl + i
+    A3 = sp.kron(A, sp.eye(1)).tocoo()
+    r, c_, v = [], [], []
+    for comp in range(ncomp):
+        kr, ir = np.divmod(A3.row, nl)
+        kc, ic = np.divmod(A3.col, nl)
+        r.append((kr * ncomp + comp) * nl + ir)
+        c_.append((kc * ncomp + comp) * nl + ic)
+        v.append(A3.data)
+    return sp.csr_matrix((np.concatenate(v), (np.concatenate(r), np.concatenate(c_))), shape=(n * ncomp, n * ncomp))

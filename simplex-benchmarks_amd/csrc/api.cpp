@@ -94,10 +94,12 @@ void upload_dg(Operator& op) {
     op.d_dg_geo.upload(pl.geo);
     op.d_dg_cell_tab.upload(pl.cell_tab);
     op.d_dg_face_tab.upload(pl.face_tab);
+    if (pl.curved) op.d_dg_jgeo.upload(pl.jgeo);
   } else {
     op.d_dg_geo.upload(convert<float>(pl.geo));
     op.d_dg_cell_tab.upload(convert<float>(pl.cell_tab));
     op.d_dg_face_tab.upload(convert<float>(pl.face_tab));
+    if (pl.curved) op.d_dg_jgeo.upload(convert<float>(pl.jgeo));
   }
   op.on_device = true;
 }
@@ -240,10 +242,39 @@ int mf_mesh_kuhn_cube(int32_t n, double perturb, uint64_t seed, double* vertices
 int mf_get_quadrature(mf_op_t h, double* points, double* weights) {
   MF_TRY({
     if (!h || !points || !weights) throw std::runtime_error("null argument");
-    if (h->op.is_dg()) throw std::runtime_error("quadrature query: CG operators only");
+    if (h->op.is_dg()) {
+      // cell_tab: gradients [q][i][d], weights [q], values [q][i], points [q][3]
+      const DGPlan& pl = h->op.dg;
+      const size_t w0 = (size_t)pl.n_q * pl.n_loc * 3;
+      const size_t p0 = w0 + pl.n_q + (size_t)pl.n_q * pl.n_loc;
+      for (int q = 0; q < pl.n_q; ++q) {
+        weights[q] = pl.cell_tab[w0 + q];
+        for (int d = 0; d < 3; ++d) points[3 * q + d] = pl.cell_tab[p0 + 3 * q + d];
+      }
+      return 0;
+    }
     const CGPlan& pl = h->op.cg;
     std::memcpy(points, pl.quad_pts.data(), pl.quad_pts.size() * sizeof(double));
     for (int q = 0; q < pl.n_q; ++q) weights[q] = pl.shape[(size_t)pl.n_loc * pl.n_q * 3 + q];
+  });
+}
+
+int mf_get_face_quadrature(mf_op_t h, int32_t* n_points, double* points, double* weights) {
+  MF_TRY({
+    if (!h || !n_points) throw std::runtime_error("null argument");
+    if (!h->op.is_dg()) throw std::runtime_error("face quadrature query: DG operators only");
+    const DGPlan& pl = h->op.dg;
+    *n_points = pl.n_qf;
+    if (!points || !weights) return 0;
+    // face_tab: per face values and gradients (stride n_qf n_loc 4), weights [q], points [f][q][3]
+    const size_t w0 = (size_t)4 * pl.n_qf * pl.n_loc * 4;
+    const size_t p0 = w0 + pl.n_qf;
+    for (int q = 0; q < pl.n_qf; ++q) {
+      weights[q] = pl.face_tab[w0 + q];
+      // (s, t) of face 3, whose vertices (0, 1, 2) put v_0 + s (v_1 - v_0) + t (v_2 - v_0) at (s, t, 0)
+      points[2 * q] = pl.face_tab[p0 + (3 * pl.n_qf + q) * 3];
+      points[2 * q + 1] = pl.face_tab[p0 + (3 * pl.n_qf + q) * 3 + 1];
+    }
   });
 }
 
@@ -278,8 +309,10 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
     if (opt && (opt->dg_kernel < 0 || opt->dg_kernel > 3)) throw std::runtime_error("dg_kernel must be 0..3");
     if (opt && (opt->penalty_kind < 0 || opt->penalty_kind > 1)) throw std::runtime_error("penalty_kind must be 0 or 1");
     if (opt && (opt->manifold < 0 || opt->manifold > 2)) throw std::runtime_error("manifold must be 0, 1 or 2");
-    if (opt && opt->manifold != 0 && ((kind != MF_CG_POISSON && kind != MF_CG_VECTOR) || opt->boundary != 0))
-      throw std::runtime_error("curved cells (manifold != 0) are available for the CG operators with the Dirichlet boundary");
+    if (opt && opt->manifold != 0 && opt->boundary != 0)
+      throw std::runtime_error("curved cells (manifold != 0): CG operators with the Dirichlet boundary, DG operators");
+    if (opt && opt->manifold != 0 && kind != MF_CG_POISSON && kind != MF_CG_VECTOR && opt->penalty_kind != 0)
+      throw std::runtime_error("curved DG cells take the reference penalty (penalty_kind 0)");
     if (opt && opt->boundary == 1 && kind != MF_CG_POISSON)
       throw std::runtime_error("Nitsche boundary (boundary = 1) is available for the scalar CG operator");
     if (precision != MF_F64 && precision != MF_F32) throw std::runtime_error("precision must be MF_F64 or MF_F32");

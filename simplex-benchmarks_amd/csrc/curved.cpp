@@ -52,6 +52,43 @@ void edge_node(const double* xa, const double* xb, int manifold, const double* c
 
 }  // namespace
 
+void p2_geometry_nodes(const double* const xv[4], int manifold, const double* center, double X[10][3]) {
+  for (int v = 0; v < 4; ++v)
+    for (int d = 0; d < 3; ++d) X[v][d] = xv[v][d];
+  for (int e = 0; e < 6; ++e) edge_node(X[kEdgeVerts[e][0]], X[kEdgeVerts[e][1]], manifold, center, X[4 + e]);
+}
+
+void p2_jacobian_coeffs(const double X[10][3], double jc[27]) {
+  // J(xi) = C + sum_f xi_f D_f.  With lambda = (1 - xi_1 - xi_2 - xi_3,
+  // xi_1, xi_2, xi_3), dx/dlambda_a = 4 sum_b N_ab lambda_b - X_a with
+  // N_aa = X_a and N_ab the edge node of (a, b); J column e (vertex e + 1)
+  // is dx/dlambda_{e+1} - dx/dlambda_0, so
+  //   C_e  = 4 X_0e - 3 X_0 - X_e,
+  //   D_e column e = 4 (X_e + X_0 - 2 X_0e),
+  //   D_f column e = 4 (X_ef - X_0e - X_0f + X_0)  (e != f).
+  // Straight edges give C_e = X_e - X_0 and D = 0.
+  const int e0[3] = {4, 5, 6};                                 // edge nodes (0,1) (0,2) (0,3)
+  const int pair[3][3] = {{0, 1, 7}, {0, 2, 8}, {1, 2, 9}};  // (e, f, node of edge (e+1, f+1))
+  for (int e = 0; e < 3; ++e)
+    for (int d = 0; d < 3; ++d) {
+      jc[3 * e + d] = 4 * X[e0[e]][d] - 3 * X[0][d] - X[e + 1][d];
+      jc[9 + 3 * e + d] = 4 * (X[e + 1][d] + X[0][d] - 2 * X[e0[e]][d]);
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int d = 0; d < 3; ++d) {
+      const int e = pair[i][0], f = pair[i][1], ef = pair[i][2];
+      jc[18 + 3 * i + d] = 4 * (X[ef][d] - X[e0[e]][d] - X[e0[f]][d] + X[0][d]);
+    }
+}
+
+void p2_jacobian_at(const double jc[27], const double* xi, double J[3][3]) {
+  for (int d = 0; d < 3; ++d) {
+    J[d][0] = jc[d] + xi[0] * jc[9 + d] + xi[1] * jc[18 + d] + xi[2] * jc[21 + d];
+    J[d][1] = jc[3 + d] + xi[1] * jc[12 + d] + xi[0] * jc[18 + d] + xi[2] * jc[24 + d];
+    J[d][2] = jc[6 + d] + xi[2] * jc[15 + d] + xi[0] * jc[21 + d] + xi[1] * jc[24 + d];
+  }
+}
+
 void build_curved_geometry(const Mesh& mesh, const mf_options& opt, CGPlan& plan) {
   const int64_t nc = plan.n_cells;
   const int nl = plan.n_loc;
@@ -82,31 +119,13 @@ void build_curved_geometry(const Mesh& mesh, const mf_options& opt, CGPlan& plan
   for (int64_t k = 0; k < nc; ++k) {
     const int64_t c = plan.cell_perm[k];
     double X[10][3];
-    for (int v = 0; v < 4; ++v)
-      for (int d = 0; d < 3; ++d) X[v][d] = mesh.x[3 * (int64_t)mesh.cells[4 * c + v] + d];
-    for (int e = 0; e < 6; ++e)
-      edge_node(X[kEdgeVerts[e][0]], X[kEdgeVerts[e][1]], opt.manifold, opt.manifold_center, X[4 + e]);
-    // J(xi) = C + sum_f xi_f D_f.  With lambda = (1 - xi_1 - xi_2 - xi_3,
-    // xi_1, xi_2, xi_3), dx/dlambda_a = 4 sum_b N_ab lambda_b - X_a with
-    // N_aa = X_a and N_ab the edge node of (a, b); J column e (vertex e + 1)
-    // is dx/dlambda_{e+1} - dx/dlambda_0, so
-    //   C_e  = 4 X_0e - 3 X_0 - X_e,
-    //   D_e column e = 4 (X_e + X_0 - 2 X_0e),
-    //   D_f column e = 4 (X_ef - X_0e - X_0f + X_0)  (e != f).
-    // Straight edges give C_e = X_e - X_0 and D = 0.
     {
-      const int e0[3] = {4, 5, 6};                 // edge nodes (0,1) (0,2) (0,3)
-      const int pair[3][3] = {{0, 1, 7}, {0, 2, 8}, {1, 2, 9}};  // (e, f, node of edge (e+1, f+1))
-      for (int e = 0; e < 3; ++e)
-        for (int d = 0; d < 3; ++d) {
-          plan.jgeo[(size_t)(3 * e + d) * nc + k] = 4 * X[e0[e]][d] - 3 * X[0][d] - X[e + 1][d];
-          plan.jgeo[(size_t)(9 + 3 * e + d) * nc + k] = 4 * (X[e + 1][d] + X[0][d] - 2 * X[e0[e]][d]);
-        }
-      for (int i = 0; i < 3; ++i)
-        for (int d = 0; d < 3; ++d) {
-          const int e = pair[i][0], f = pair[i][1], ef = pair[i][2];
-          plan.jgeo[(size_t)(18 + 3 * i + d) * nc + k] = 4 * (X[ef][d] - X[e0[e]][d] - X[e0[f]][d] + X[0][d]);
-        }
+      const double* xv[4];
+      for (int v = 0; v < 4; ++v) xv[v] = &mesh.x[3 * (int64_t)mesh.cells[4 * c + v]];
+      p2_geometry_nodes(xv, opt.manifold, opt.manifold_center, X);
+      double jc[27];
+      p2_jacobian_coeffs(X, jc);
+      for (int t = 0; t < 27; ++t) plan.jgeo[(size_t)t * nc + k] = jc[t];
     }
     for (int qq = 0; qq < nq; ++qq) {
       // J[d][e] = sum_i X_i[d] d(phi_i)/d(xi_e)

@@ -503,25 +503,27 @@ SMF_HDI int lane_slot4(int g, int t, int f) {
                       LI.j[g][i0 + 2 < NL ? i0 + 2 : 0], LI.j[g][i0 + 3 < NL ? i0 + 3 : 0]);
 }
 
-// Lane f of cell k, component c: the cell term's share (above) and the SIPG
-// flux of face f, evaluated with the cell's nodes renumbered so that face f
-// is face 3 (the neighbour's frame of apply_cell), so every table is the
-// compile-time face-3 table.  yrow[j] receives the lane's contribution to the
-// row of its frame node j (cell node perm[f][j]; make_lane_inv inverts it)
-// (MODE 0) or to that diagonal entry (MODE 1); the four lanes' rows add up to
-// apply_cell's result.  Storing in the lane frame keeps the permutation out
-// of the registers live across the lane's work.
-template <int P, typename T, int NC, bool MASS, int MODE>
-SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __restrict__ geo,
-                            const int32_t* __restrict__ nbr, const uint32_t* __restrict__ nface, T factor, T mass,
-                            T lap, const T* __restrict__ src, T* __restrict__ yrow, int pkind = 0) {
-  constexpr DGCellTab<P, MASS> CT = make_cell_tab<P, MASS>();
-  constexpr DGCellShare<P, MASS> CS = make_cell_share<P, MASS>();
-  constexpr DGFaceTab<P> FT = make_face_tab<P>();
-  constexpr int NL = nloc(P), NQ = DGCellTab<P, MASS>::NQ, NQF = DGFaceTab<P>::NQF;
-  constexpr int NFN = DGFaceTab<P>::NFN, NFG = DGFaceTab<P>::NFG;
-  using std::sqrt;
+// Geometry of lane f of cell k (shared by all components of the cell):
+// the face normal and its J^-1 n in the lane frame (nJ), the cell metric in
+// the lane frame, the face's |F|-scaled Jacobian factor, and the neighbour
+// (kn, its local face fb, its geometry).
+// The neighbour's values (jb, taub, adetb) are first used in the face
+// section, so their loads stay in flight across the cell term.
+template <typename T>
+struct LaneGeo {
+  T nJ[3], M[6], area2, adet, tau, len, n[3];
+  T jb[9], adetb, taub;
+  int64_t kn;
+  int fb;
+  bool interior;
+};
 
+template <int P, typename T>
+SMF_HDI LaneGeo<T> lane_geometry(int64_t k, int f, int64_t ncells, const T* __restrict__ geo,
+                                 const int32_t* __restrict__ nbr, const uint32_t* __restrict__ nface, T lap) {
+  constexpr DGFaceTab<P> FT = make_face_tab<P>();
+  using std::sqrt;
+  LaneGeo<T> g;
   // barycentric gradients grad lambda_a: a = 1..3 are the columns of J^-T
   T gl[4][3];
   for (int d = 0; d < 3; ++d) {
@@ -531,31 +533,69 @@ SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* _
     gl[0][d] = -(gl[1][d] + gl[2][d] + gl[3][d]);
   }
   const T adet = geo[9 * ncells + k], tau = geo[10 * ncells + k];
+  g.adet = adet;
+  // neighbour geometry issued early so that its latency overlaps the rest
+  const int32_t kb = nbr[f * ncells + k];
+  g.interior = kb >= 0;
+  g.kn = g.interior ? kb : k;
+  g.fb = g.interior ? (int)((nface[k] >> (8 * f)) & 3) : 3;
+#pragma unroll
+  for (int t = 0; t < 9; ++t) g.jb[t] = geo[t * ncells + g.kn];
+  g.adetb = geo[9 * ncells + g.kn];
+  g.taub = geo[10 * ncells + g.kn];
+  g.tau = tau;
   // own face f: J^-T nhat_f = -grad lambda_f; outward unit normal, area
   T gf[3];
   for (int d = 0; d < 3; ++d) gf[d] = select4<T>(f, gl[0][d], gl[1][d], gl[2][d], gl[3][d]);
   const T len = sqrt(gf[0] * gf[0] + gf[1] * gf[1] + gf[2] * gf[2]);
   const T il = T(1) / len;
   const T n0 = -gf[0] * il, n1 = -gf[1] * il, n2 = -gf[2] * il;
-  const T area2 = adet * len * lap;
+  g.n[0] = n0;
+  g.n[1] = n1;
+  g.n[2] = n2;
+  g.len = len;
+  g.area2 = adet * len * lap;
   T bo[4];
   for (int a = 0; a < 4; ++a) bo[a] = gl[a][0] * n0 + gl[a][1] * n1 + gl[a][2] * n2;
   // lane frame: new vertex j <- old vertex sigma[f][j]; reference direction
   // d (1..3) is new vertex d
-  T nJ[3], G[3][3];
+  T G[3][3];
   for (int d = 0; d < 3; ++d) {
-    nJ[d] = select4<T>(f, bo[FT.sigma[0][d + 1]], bo[FT.sigma[1][d + 1]], bo[FT.sigma[2][d + 1]],
-                       bo[FT.sigma[3][d + 1]]);
+    g.nJ[d] = select4<T>(f, bo[FT.sigma[0][d + 1]], bo[FT.sigma[1][d + 1]], bo[FT.sigma[2][d + 1]],
+                         bo[FT.sigma[3][d + 1]]);
     for (int e = 0; e < 3; ++e)
       G[d][e] = select4<T>(f, gl[FT.sigma[0][d + 1]][e], gl[FT.sigma[1][d + 1]][e], gl[FT.sigma[2][d + 1]][e],
                            gl[FT.sigma[3][d + 1]][e]);
   }
-  const T M0 = adet * (G[0][0] * G[0][0] + G[0][1] * G[0][1] + G[0][2] * G[0][2]);
-  const T M1 = adet * (G[0][0] * G[1][0] + G[0][1] * G[1][1] + G[0][2] * G[1][2]);
-  const T M2 = adet * (G[0][0] * G[2][0] + G[0][1] * G[2][1] + G[0][2] * G[2][2]);
-  const T M3 = adet * (G[1][0] * G[1][0] + G[1][1] * G[1][1] + G[1][2] * G[1][2]);
-  const T M4 = adet * (G[1][0] * G[2][0] + G[1][1] * G[2][1] + G[1][2] * G[2][2]);
-  const T M5 = adet * (G[2][0] * G[2][0] + G[2][1] * G[2][1] + G[2][2] * G[2][2]);
+  g.M[0] = adet * (G[0][0] * G[0][0] + G[0][1] * G[0][1] + G[0][2] * G[0][2]);
+  g.M[1] = adet * (G[0][0] * G[1][0] + G[0][1] * G[1][1] + G[0][2] * G[1][2]);
+  g.M[2] = adet * (G[0][0] * G[2][0] + G[0][1] * G[2][1] + G[0][2] * G[2][2]);
+  g.M[3] = adet * (G[1][0] * G[1][0] + G[1][1] * G[1][1] + G[1][2] * G[1][2]);
+  g.M[4] = adet * (G[1][0] * G[2][0] + G[1][1] * G[2][1] + G[1][2] * G[2][2]);
+  g.M[5] = adet * (G[2][0] * G[2][0] + G[2][1] * G[2][1] + G[2][2] * G[2][2]);
+  return g;
+}
+
+// Lane f of cell k, component c, from the lane's geometry: the cell term's
+// share (above) and the SIPG flux of face f, evaluated with the cell's nodes
+// renumbered so that face f is face 3 (the neighbour's frame of apply_cell),
+// so every table is the compile-time face-3 table.  yrow[j] receives the
+// lane's contribution to the row of its frame node j (cell node perm[f][j];
+// make_lane_inv inverts it) (MODE 0) or to that diagonal entry (MODE 1); the
+// four lanes' rows add up to apply_cell's result.  Storing in the lane frame
+// keeps the permutation out of the registers live across the lane's work.
+template <int P, typename T, int NC, bool MASS, int MODE>
+SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T factor, T mass, T lap,
+                            const T* __restrict__ src, T* __restrict__ yrow, int pkind) {
+  constexpr DGCellTab<P, MASS> CT = make_cell_tab<P, MASS>();
+  constexpr DGCellShare<P, MASS> CS = make_cell_share<P, MASS>();
+  constexpr DGFaceTab<P> FT = make_face_tab<P>();
+  constexpr int NL = nloc(P), NQ = DGCellTab<P, MASS>::NQ, NQF = DGFaceTab<P>::NQF;
+  constexpr int NFN = DGFaceTab<P>::NFN, NFG = DGFaceTab<P>::NFG;
+  const T M0 = g.M[0], M1 = g.M[1], M2 = g.M[2], M3 = g.M[3], M4 = g.M[4], M5 = g.M[5];
+  const T* nJ = g.nJ;
+  const T adet = g.adet, area2 = g.area2;
+  const bool interior = g.interior;
   int pf[NL];
   for (int i = 0; i < NL; ++i) pf[i] = select4<int>(f, FT.perm[0][i], FT.perm[1][i], FT.perm[2][i], FT.perm[3][i]);
 
@@ -566,22 +606,15 @@ SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* _
     u[i] = MODE == 0 ? src[base + pf[i]] : T(0);
     y[i] = T(0);
   }
-  // neighbour data issued before the cell term so that its latency overlaps
-  // it (a boundary face reads its own cell; those values are not used)
-  const int32_t kb = nbr[f * ncells + k];
-  const bool interior = kb >= 0;
-  const int64_t kn = interior ? kb : k;
-  const int fb = interior ? (int)((nface[k] >> (8 * f)) & 3) : 3;
-  T ub[NL], jb[9];
+  // neighbour values issued before the cell term so that their latency
+  // overlaps it (a boundary face reads its own cell; those values are not used)
+  T ub[NL];
   if (MODE == 0) {
-    const int64_t nbase = (kn * NC + c) * NL;
+    const int64_t nbase = (g.kn * NC + c) * NL;
 #pragma unroll
     for (int i = 0; i < NL; ++i)
-      ub[i] = src[nbase + select4<int>(fb, FT.perm[0][i], FT.perm[1][i], FT.perm[2][i], FT.perm[3][i])];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) jb[t] = geo[t * ncells + kn];
+      ub[i] = src[nbase + select4<int>(g.fb, FT.perm[0][i], FT.perm[1][i], FT.perm[2][i], FT.perm[3][i])];
   }
-  const T adetb = geo[9 * ncells + kn], taub = geo[10 * ncells + kn];
   // ---- the lane's share of the cell integral ----
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
@@ -619,10 +652,10 @@ SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* _
   // ---- face f (= face 3 of the lane frame) ----
   T sigma;
   if (pkind == 1) {  // (p+1)(p+2) |F| / min |K|: factor * max(len, len |det J| / |det J+|)
-    const T lb = len * adet / adetb;
-    sigma = factor * (len > lb ? len : lb);
+    const T lb = g.len * adet / g.adetb;
+    sigma = factor * (g.len > lb ? g.len : lb);
   } else {
-    sigma = factor * (interior ? (tau > taub ? tau : taub) : tau);
+    sigma = factor * (interior ? (g.tau > g.taub ? g.tau : g.taub) : g.tau);
   }
   if (MODE == 0) {
     T gn[NFG];
@@ -641,17 +674,16 @@ SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* _
 #pragma unroll
     for (int j = 0; j < NFG; ++j) gb[j] = T(0);
     if (interior) {
+      // the neighbour's J^-1 n in the frame of its face fb
       T bb[4];
-      {
-        bb[1] = jb[0] * n0 + jb[3] * n1 + jb[6] * n2;
-        bb[2] = jb[1] * n0 + jb[4] * n1 + jb[7] * n2;
-        bb[3] = jb[2] * n0 + jb[5] * n1 + jb[8] * n2;
-        bb[0] = -(bb[1] + bb[2] + bb[3]);
-      }
+      bb[1] = g.jb[0] * g.n[0] + g.jb[3] * g.n[1] + g.jb[6] * g.n[2];
+      bb[2] = g.jb[1] * g.n[0] + g.jb[4] * g.n[1] + g.jb[7] * g.n[2];
+      bb[3] = g.jb[2] * g.n[0] + g.jb[5] * g.n[1] + g.jb[8] * g.n[2];
+      bb[0] = -(bb[1] + bb[2] + bb[3]);
       T nb[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d)
-        nb[d] = select4<T>(fb, bb[FT.sigma[0][d + 1]], bb[FT.sigma[1][d + 1]], bb[FT.sigma[2][d + 1]],
+        nb[d] = select4<T>(g.fb, bb[FT.sigma[0][d + 1]], bb[FT.sigma[1][d + 1]], bb[FT.sigma[2][d + 1]],
                            bb[FT.sigma[3][d + 1]]);
 #pragma unroll
       for (int j = 0; j < NFG; ++j) {
@@ -672,24 +704,24 @@ SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* _
     constexpr DGFaceMass<P> FM = make_face_mass<P>();
     T J[NFN], A[NFG];
 #pragma unroll
-    for (int k = 0; k < NFN; ++k) J[k] = interior ? u[FT.fn[3][k]] - ub[FT.fn[3][k]] : T(2) * u[FT.fn[3][k]];
+    for (int kk = 0; kk < NFN; ++kk) J[kk] = interior ? u[FT.fn[3][kk]] - ub[FT.fn[3][kk]] : T(2) * u[FT.fn[3][kk]];
 #pragma unroll
     for (int j = 0; j < NFG; ++j) A[j] = interior ? T(0.5) * (gn[j] + gb[j]) : gn[j];
 #pragma unroll
-    for (int k = 0; k < NFN; ++k) {
+    for (int kk = 0; kk < NFN; ++kk) {
       T sj = T(0), sa = T(0);
 #pragma unroll
-      for (int l = 0; l < NFN; ++l) sj += T(FM.vv[k][l]) * J[l];
+      for (int l = 0; l < NFN; ++l) sj += T(FM.vv[kk][l]) * J[l];
 #pragma unroll
-      for (int j = 0; j < NFG; ++j) sa += T(FM.vg[k][j]) * A[j];
-      y[FT.fn[3][k]] += area2 * (sigma * sj - sa);
+      for (int j = 0; j < NFG; ++j) sa += T(FM.vg[kk][j]) * A[j];
+      y[FT.fn[3][kk]] += area2 * (sigma * sj - sa);
     }
     T tgj[NFG];
 #pragma unroll
     for (int j = 0; j < NFG; ++j) {
       T sj = T(0);
 #pragma unroll
-      for (int k = 0; k < NFN; ++k) sj += T(FM.vg[k][j]) * J[k];
+      for (int kk = 0; kk < NFN; ++kk) sj += T(FM.vg[kk][j]) * J[kk];
       tgj[j] = T(-0.5) * area2 * sj;
     }
 #pragma unroll
@@ -726,6 +758,15 @@ SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* _
   }
 #pragma unroll
   for (int i = 0; i < NL; ++i) yrow[i] = y[i];
+}
+
+// Lane f of cell k, component c (geometry and component together)
+template <int P, typename T, int NC, bool MASS, int MODE>
+SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __restrict__ geo,
+                             const int32_t* __restrict__ nbr, const uint32_t* __restrict__ nface, T factor, T mass,
+                             T lap, const T* __restrict__ src, T* __restrict__ yrow, int pkind = 0) {
+  const LaneGeo<T> g = lane_geometry<P, T>(k, f, ncells, geo, nbr, nface, lap);
+  lane_component<P, T, NC, MASS, MODE>(g, k, f, c, factor, mass, lap, src, yrow, pkind);
 }
 
 }  // namespace dgt

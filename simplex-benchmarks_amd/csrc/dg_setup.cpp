@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
+#include <vector>
 
 #include "dg_cell.h"
 #include "op.h"
@@ -21,7 +22,92 @@ namespace smf {
 
 namespace {
 const double kNhat[4][3] = {{1, 1, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+
+// Curved cells (the reference's MappingFE(FE_SimplexP(2)) + manifold,
+// CG/cells_batched/application.cc:1613, 1668-1677, with the DG operator of
+// DG/cells_batched/application.cc:1614-1666 (general D_e) and 2191-2238
+// (curvilinear face flux)): each cell's P2 geometry map in the ascending
+// vertex order (its edge nodes are symmetric in the two ends, so both cells
+// of a face map it identically), tau_K = (sum_bdry |F| + 1/2 sum_int |F|)/|K|
+// with |K| = sum_q w_q |det J(xi_q)| and |F| = sum_q w_q |cof J(xi_q) N_f| by
+// the operator's rules (N_f the reference face normal scaled by the
+// reference face's area over 1/2), and the support points under the map.
+void build_dg_curved(const Mesh& mesh, const mf_options& opt, const Quadrature& qc, const Quadrature& qf,
+                     const std::vector<uint8_t>& is_bdry, DGPlan& plan) {
+  const int64_t nc = plan.n_cells;
+  const int nl = plan.n_loc;
+  const Lagrange geo2(2), fe(plan.p);
+  plan.curved = true;
+  plan.jgeo.assign((size_t)27 * nc, 0.0);
+  std::vector<double> gv((size_t)nl * 10);
+  for (int i = 0; i < nl; ++i) {
+    const auto xr = fe.node_point(i);
+    geo2.eval(xr.data(), &gv[(size_t)i * 10], nullptr);
+  }
+  int bad = 0;
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < nc; ++k) {
+    const double* xv[4];
+    for (int v = 0; v < 4; ++v) xv[v] = &mesh.x[3 * (int64_t)plan.cell_verts[4 * k + v]];
+    double X[10][3], jc[27];
+    p2_geometry_nodes(xv, opt.manifold, opt.manifold_center, X);
+    p2_jacobian_coeffs(X, jc);
+    for (int t = 0; t < 27; ++t) plan.jgeo[(size_t)t * nc + k] = jc[t];
+    auto cof = [&](const double* xi, double C[3][3], double* det) {
+      double J[3][3];
+      p2_jacobian_at(jc, xi, J);
+      // cof(J)[d][e] = cofactor of J[d][e]
+      C[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      C[0][1] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      C[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      C[1][0] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+      C[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+      C[1][2] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+      C[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+      C[2][1] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+      C[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      *det = J[0][0] * C[0][0] + J[0][1] * C[0][1] + J[0][2] * C[0][2];
+    };
+    // ascending vertex order may reverse the orientation: det J keeps one
+    // sign over a valid cell, |det J| weights the integrals
+    double vol = 0, sgn = 0;
+    for (int q = 0; q < qc.size(); ++q) {
+      double C[3][3], det;
+      cof(qc.pts[q].data(), C, &det);
+      if (q == 0) sgn = det > 0 ? 1.0 : -1.0;
+      if (!(det * sgn > 0)) {
+#pragma omp atomic write
+        bad = 1;
+      }
+      vol += qc.w[q] * std::fabs(det);
+    }
+    double surf = 0;
+    for (int f = 0; f < 4; ++f) {
+      double area = 0;
+      for (int q = 0; q < qf.size(); ++q) {
+        const auto xi = face_to_cell(f, qf.pts[q][0], qf.pts[q][1]);
+        double C[3][3], det;
+        cof(xi.data(), C, &det);
+        double a2 = 0;
+        for (int d = 0; d < 3; ++d) {
+          const double a = C[d][0] * kNhat[f][0] + C[d][1] * kNhat[f][1] + C[d][2] * kNhat[f][2];
+          a2 += a * a;
+        }
+        area += qf.w[q] * std::sqrt(a2);
+      }
+      surf += (is_bdry[4 * k + f] ? 1.0 : 0.5) * area;
+    }
+    plan.geo[(size_t)10 * nc + k] = surf / vol;
+    for (int i = 0; i < nl; ++i)
+      for (int d = 0; d < 3; ++d) {
+        double sxy = 0;
+        for (int j = 0; j < 10; ++j) sxy += gv[(size_t)i * 10 + j] * X[j][d];
+        plan.dof_points[3 * (k * nl + i) + d] = sxy;
+      }
+  }
+  if (bad) throw std::runtime_error("curved cells: the Jacobian determinant changes sign inside a cell");
 }
+}  // namespace
 
 int cg_quadrature_degree(int p);
 
@@ -136,8 +222,10 @@ void build_dg_plan(const Mesh& mesh, int p, int ncomp, const mf_options& opt, bo
     }
   }
 
-  // tables
-  const Quadrature qc = tet_rule(helmholtz ? 2 * p : cg_quadrature_degree(p));
+  // tables (curved cells: the cell rule of degree 2p, as the integrands are
+  // no longer polynomials)
+  const bool curved = opt.manifold != 0;
+  const Quadrature qc = tet_rule(helmholtz || curved ? 2 * p : cg_quadrature_degree(p));
   const Quadrature qf = tri_rule(2 * p);
   plan.n_q = qc.size();
   plan.n_qf = qf.size();
@@ -166,6 +254,16 @@ void build_dg_plan(const Mesh& mesh, int p, int ncomp, const mf_options& opt, bo
       }
     }
   for (int q = 0; q < nqf; ++q) plan.face_tab[4 * fstride + q] = qf.w[q];
+  // the rule points (read by the curved-cell kernel): cell points [q][3],
+  // face points in cell coordinates [f][q][3]
+  for (int q = 0; q < plan.n_q; ++q)
+    for (int d = 0; d < 3; ++d) plan.cell_tab.push_back(qc.pts[q][d]);
+  for (int f = 0; f < 4; ++f)
+    for (int q = 0; q < nqf; ++q) {
+      const auto x = face_to_cell(f, qf.pts[q][0], qf.pts[q][1]);
+      for (int d = 0; d < 3; ++d) plan.face_tab.push_back(x[d]);
+    }
+  if (curved) build_dg_curved(mesh, opt, qc, qf, is_bdry, plan);
 }
 
 namespace {
@@ -223,6 +321,7 @@ void host_dg_t(const Operator& op, int mode, T* dst, const T* src) {
 }  // namespace
 
 void host_apply_dg(const Operator& op, int mode, void* dst, const void* src) {
+  if (op.dg.curved) throw std::runtime_error("host schedule: affine DG cells only");
   if (op.precision == MF_F64)
     host_dg_t<double>(op, mode, static_cast<double*>(dst), static_cast<const double*>(src));
   else

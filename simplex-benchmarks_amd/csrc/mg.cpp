@@ -550,6 +550,9 @@ std::unique_ptr<Multigrid> build_multigrid(const Operator& fine, const MGOptions
     throw std::runtime_error("multigrid needs a scalar CG Poisson or DG SIPG fine operator");
   if (fine.kind == MF_CG_POISSON && fine.cg.nitsche)
     throw std::runtime_error("multigrid below a Nitsche CG operator: build it from the DG operator");
+  if (dg && fine.dg.curved)
+    throw std::runtime_error("multigrid below a curved DG operator is not built (its CG levels would need curved "
+                             "Nitsche boundaries)");
   auto mg = std::make_unique<Multigrid>();
   mg->opt = opt;
   mg->fine_precision = fine.precision;
@@ -794,6 +797,7 @@ void assemble_rhs(const Operator& op, int kind, uint64_t seed, int n_features, d
                   std::vector<double>& b) {
   const bool dg = op.kind == MF_DG_SIPG;
   if (op.kind != MF_CG_POISSON && !dg) throw std::runtime_error("right-hand side assembly is for scalar CG and DG operators");
+  if (dg && op.dg.curved) throw std::runtime_error("right-hand side assembly: affine DG cells only");
   const CGPlan& pl = op.cg;
   const Mesh& m = op.mesh;
   const int nl = dg ? op.dg.n_loc : pl.n_loc;

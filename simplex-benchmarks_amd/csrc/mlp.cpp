@@ -46,6 +46,7 @@ std::unique_ptr<LearnedSmoother> make_learned_smoother(const Operator& op, uint6
   const bool dg = op.kind == MF_DG_SIPG;
   if (!((op.kind == MF_CG_POISSON || dg) && op.degree == 2 && op.ncomp == 1))
     throw std::runtime_error("the learned smoother is defined for the scalar P2 CG and DG operators");
+  if (dg && op.dg.curved) throw std::runtime_error("learned smoother: affine cells only");
   if (op.precision != MF_F64) throw std::runtime_error("the learned smoother works on the FP64 fine operator");
   if (!op.on_device) throw std::runtime_error("operator was created host_only");
   auto ls = std::make_unique<LearnedSmoother>();

@@ -148,6 +148,13 @@ struct DGPlan {
   // max(|F|/|K-|, |F|/|K+|) = penalty_factor max(len, len |det J-|/|det J+|)
   // with len = |J^-T nhat_f| (|F|/|K| = 3 len), penalty_factor = 3 (p+1)(p+2)
   int penalty_kind = 0;
+  // curved cells (opt.manifold != 0): the P2 geometry map of each cell as the
+  // 27 coefficients of J(xi) (27 x n_cells); tau (geo row 10) from the
+  // curved volumes and face areas by the operator's rules; the tables carry
+  // the rule points: cell_tab ends with the cell points [q][3], face_tab with
+  // the face points in cell coordinates [f][q][3]
+  bool curved = false;
+  std::vector<double> jgeo;
 };
 
 #ifdef __HIPCC__
@@ -174,7 +181,7 @@ struct Operator {
   DevBuf d_cell_start, d_own_start, d_ghost_start, d_ghost_idx, d_ccol, d_fix_ptr,
       d_pslot, d_gfix, d_lidx, d_lidx12, d_metric, d_shape, d_partial;
   DevBuf d_bnd_rows, d_bnd_ptr, d_bnd_col, d_bnd_val, d_bnd_dense, d_bcell, d_bmat;
-  DevBuf d_dg_geo, d_dg_nbr, d_dg_nface, d_dg_cell_tab, d_dg_face_tab;
+  DevBuf d_dg_geo, d_dg_nbr, d_dg_nface, d_dg_cell_tab, d_dg_face_tab, d_dg_jgeo;
   bool on_device = false;
   int64_t m() const { return is_dg() ? dg.n_dofs : cg.n_dofs * ncomp; }
   size_t wbytes() const { return precision == MF_F64 ? 8 : 4; }
@@ -249,6 +256,14 @@ void launch_dg_epilogue(const Operator& op, const T* x, RowEpilogue<T> ep, hipSt
 void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan);
 // curved-cell geometry (opt.manifold != 0): qmetric and the mapped support points
 void build_curved_geometry(const Mesh& mesh, const mf_options& opt, CGPlan& plan);
+// P2 geometry nodes of a cell: its vertices xv[0..3], then per edge (kEdgeVerts
+// order) the manifold's midpoint (manifold 1 spherical around center, 2 straight)
+void p2_geometry_nodes(const double* const xv[4], int manifold, const double* center, double X[10][3]);
+// the 27 coefficients of J(xi) = C + sum_f xi_f D_f of the P2 map with nodes X,
+// in the kernels' layout: C (3 columns), D_e column e, the shared columns of D_e, D_f
+void p2_jacobian_coeffs(const double X[10][3], double jc[27]);
+// J(xi) from those coefficients
+void p2_jacobian_at(const double jc[27], const double* xi, double J[3][3]);
 // boundary-face matrix of the Nitsche CG operator (plan.nitsche)
 void build_nitsche_boundary(const Mesh& mesh, CGPlan& plan, int penalty_kind);
 // SIPG penalty tau_K = (sum_bdry |F| + 1/2 sum_int |F|) / |K| per input cell

@@ -138,6 +138,7 @@ def lib():
         "mf_solve_fcg_learned": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, i32, C.POINTER(FcgOptions),
                                            C.POINTER(FcgStats), vp]),
         "mf_get_quadrature": (C.c_int, [vp, vp, vp]),
+        "mf_get_face_quadrature": (C.c_int, [vp, C.POINTER(C.c_int32), vp, vp]),
         "mf_mesh_read_vtk": (C.c_int, [C.c_char_p, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), vp, vp]),
         "mf_get_system_matrix": (C.c_int, [vp, C.POINTER(vp)]),
         "mf_csr_vmult": (C.c_int, [vp, vp, vp, vp]),
@@ -239,7 +240,7 @@ _SIGNATURE_NAMES = [
     "mf_ls_default_train_options", "mf_ls_default_fcg_options", "mf_ls_create", "mf_ls_destroy",
     "mf_ls_set_params", "mf_ls_get_params", "mf_ls_save", "mf_ls_load", "mf_ls_apply", "mf_ls_forward",
     "mf_ls_features", "mf_ls_gradient", "mf_ls_train_step", "mf_ls_train", "mf_solve_fcg_learned",
-    "mf_get_quadrature", "mf_mesh_read_vtk", "mf_get_system_matrix", "mf_csr_vmult", "mf_csr_get_info", "mf_csr_copy_to_host", "mf_csr_destroy",
+    "mf_get_quadrature", "mf_get_face_quadrature", "mf_mesh_read_vtk", "mf_get_system_matrix", "mf_csr_vmult", "mf_csr_get_info", "mf_csr_copy_to_host", "mf_csr_destroy",
 ]
 
 
@@ -534,10 +535,20 @@ class Operator:
         return out
 
     def quadrature(self):
-        """(points (n_q, 3), weights (n_q,)) of the cell rule (CG operators)"""
+        """(points (n_q, 3), weights (n_q,)) of the cell rule"""
         nq = self.info["n_q_points"]
         pts, w = np.empty((nq, 3)), np.empty(nq)
         check(lib().mf_get_quadrature(self.handle, pts.ctypes.data, w.ctypes.data))
+        return pts, w
+
+    def face_quadrature(self):
+        """(points (n, 2) as (s, t) on the unit triangle, weights (n,)) of the
+        face rule (DG operators); on a face with local vertices a < b < c the
+        point is x_a + s (x_b - x_a) + t (x_c - x_a)"""
+        n = C.c_int32()
+        check(lib().mf_get_face_quadrature(self.handle, C.byref(n), None, None))
+        pts, w = np.empty((n.value, 2)), np.empty(n.value)
+        check(lib().mf_get_face_quadrature(self.handle, C.byref(n), pts.ctypes.data, w.ctypes.data))
         return pts, w
 
     def host_apply(self, src=None, mode=0):
