@@ -107,8 +107,7 @@ def test_curved_symmetry_and_matrix(smf):
 
 def test_curved_options_rejected(smf):
     mesh = smf.Mesh.kuhn_cube(2)
-    with pytest.raises(smf.MFError, match="manifold"):
-        smf.Operator(mesh, 2, kind="dg", host_only=True, manifold="spherical")
+    # DG operators take curved cells (tests/test_host_dg_curved.py), the Nitsche CG boundary does not
     with pytest.raises(smf.MFError, match="manifold"):
         smf.Operator(mesh, 2, boundary="nitsche", host_only=True, manifold="spherical")
     with pytest.raises(smf.MFError, match="manifold"):
