@@ -55,7 +55,7 @@ def parse():
                     help="matrix-free vs assembled CSR comparison, P2 on an n^3 cube, P1 on (2n)^3, "
                          "P3 on (n/2)^3 (0 skips)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
-                    help="MiB copied device-to-device once before timing (scale for rocprof counters)")
+                    help="MiB copied once by a full-chip copy kernel before timing (scale for rocprof counters)")
     ap.add_argument("--precision", default="f64")
     ap.add_argument("--no-reorder", action="store_true")
     ap.add_argument("--manifold", default="none", choices=["none", "spherical", "straight"],
@@ -404,7 +404,8 @@ def main():
         nb = args.calibrate_copy << 20
         a, b = smf.DeviceVector(nb // 8), smf.DeviceVector(nb // 8)
         a.zero()
-        smf.check(smf.lib().mf_memcpy_dtod(b.ptr, a.ptr, nb, None))
+        # full-chip copy kernel (convert_kernel<double, double>): nb bytes read, nb written
+        smf.check(smf.lib().mf_copy_kernel(b.ptr, a.ptr, nb // 8, None))
         smf.synchronize()
         del a, b
     head = measure(smf, args.kind, args.n, args.degree, args.precision, args.steps, args.warmup,

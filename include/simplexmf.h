@@ -54,7 +54,8 @@ typedef struct mf_options {
   int32_t penalty_kind;     /* 0: (p+1)(p+3)/3 max(tau-,tau+) (reference); 1: (p+1)(p+2)/h_face with
                                h_face = |K|/|F| per side, the larger penalty of the two sides */
   int32_t host_only;        /* 1: build the host plan only (no device memory); device entry points fail */
-  int32_t accumulate;       /* CG LDS accumulation: 0 auto, 1 LDS atomics, 2 read-add-write per colour */
+  int32_t accumulate;       /* CG LDS accumulation: 0 auto, 1 LDS atomics per colour, 2 read-add-write per
+                               colour, 3 LDS atomics per wave (wave order fixed per round) */
   int32_t dg_kernel;        /* DG kernel: 0 auto, 1 LDS tables, 2 compile-time tables + neighbour in face-3 frame,
                                3 four lanes per cell (one face each) */
   int32_t boundary;         /* CG Poisson: 0 Dirichlet rows = identity (default); 1 Nitsche: no constrained
@@ -372,6 +373,11 @@ int mf_memcpy_htod(void* dst, const void* src, uint64_t bytes);
 int mf_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
 int mf_memcpy_dtod(void* dst, const void* src, uint64_t bytes, void* stream);
 int mf_memset(void* ptr, int32_t value, uint64_t bytes, void* stream);
+/* dst[i] = src[i] for n doubles with one thread per element over the whole
+ * chip (unlike hipMemcpy's copy kernel, which runs on a few workgroups): the
+ * plain copy of known traffic (8 n bytes read, 8 n written) against which
+ * hardware byte counters are scaled */
+int mf_copy_kernel(void* dst, const void* src, uint64_t n, void* stream);
 int mf_synchronize(void);
 int mf_stream_create(void** stream);
 int mf_stream_destroy(void* stream);

@@ -9,9 +9,10 @@
 //      (interpolation with the reference shape-gradient matrix), the affine
 //      quadrature-point operator w_q |det J| J^-1 J^-T and the transposed
 //      interpolation (integration), in registers with FP64/FP32 FMA;
-//   3. the round's cell vectors are added into an LDS accumulator colour by
-//      colour, with a workgroup barrier after each colour (cells of one
-//      colour share no DoF), so every DoF is summed in cell order;
+//   3. the round's cell vectors are added into an LDS accumulator: FP64 with
+//      ds_add_f64 one wave after the other, in a wave order fixed per round
+//      (W - 1 barriers per round); FP32 colour by colour with read-add-write
+//      and a barrier after each colour (cells of one colour share no DoF);
 //   4. owned DoFs are stored; ghost DoFs (owned by another patch) are written
 //      as partial sums that a fix-up kernel adds to the owner's value in patch
 //      order.  No floating-point atomics reach global memory: the result is
@@ -146,15 +147,24 @@ __device__ __forceinline__ T sum_partials(const T* __restrict__ part, int32_t k0
 
 // MODE 0: y = A_K u;  MODE 1: y_i = (A_K)_ii
 // shape: G[(q*NL + i)*3 + d] = d phi_i / d xhat_d at point q, then w[q].
-// ATOMIC: accumulate with ds_add (fast for f64 on gfx950) or with a plain
-// LDS read-add-write per colour (ds_add_f32 is slow on gfx950).
+// ACC: how a round's cell vectors reach the LDS accumulator --
+//   0: plain LDS read-add-write, one pass per colour (ds_add_f32 is slow on
+//      gfx950);
+//   1: ds_add (fast for f64 on gfx950), one pass per colour;
+//   2: ds_add, one pass per wave: the waves of the workgroup add their whole
+//      round in turn, in an order fixed per round (0..W-1 on even rounds,
+//      W-1..0 on odd ones, so one wave adds twice in a row across the round
+//      boundary and a round needs W-1 barriers).  Same-address lanes of one
+//      ds_add instruction are applied by the LDS unit in its own fixed order,
+//      so every DoF's sum has one order for given tables: bit-reproducible,
+//      but not the cell order of ACC 0/1.
 // NC components (components-batched vector Laplacian,
 // CG/components_batched/application.cc:1024-1216) share the index tables and
 // the geometry of a cell; component c of scalar DoF d is row c*cstride + d.
 // FEAT: bit 0 = pending interface staging (ChebyRun steps), bit 1 = per-cell
 // Nitsche boundary terms; both compile out of the plain operator kernels
 // (they cost registers: P2 FP32 64 -> 120 VGPRs with both)
-template <int P, typename T, int MODE, bool ATOMIC, int NC, int FEAT>
+template <int P, typename T, int MODE, int ACC, int NC, int FEAT>
 __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ own_start,
     const int32_t* __restrict__ ghost_start, const int32_t* __restrict__ ghost_idx,
@@ -503,14 +513,27 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
         }
       }
     }
-    // 3. colour-ordered accumulation (colours ascend with the cell index);
-    //    the colour range of the round is uniform over the workgroup
+    // 3. accumulation.  ACC 2: wave-ordered passes (see above)
+    if constexpr (ACC == 2) {
+      const int nw = nt / kWave, wave = lane / kWave;
+      const bool fwd = ((r0 / nt) & 1) == 0;
+      for (int k = 0; k < nw; ++k) {
+        if (wave == (fwd ? k : nw - 1 - k) && active) {
+#pragma unroll
+          for (int i = 0; i < NL; ++i) atomicAdd(&sy[li[i]], y[i]);
+        }
+        if (k + 1 < nw) __syncthreads();
+      }
+      continue;
+    }
+    // ACC 0/1: colour-ordered accumulation (colours ascend with the cell
+    // index); the colour range of the round is uniform over the workgroup
     const int clo = ccol[c0 + r0];
     const int last = min(r0 + nt, nc) - 1;
     const int chi = ccol[c0 + last];
     for (int c = clo; c <= chi; ++c) {
       if (active && mycol == c) {
-        if (ATOMIC) {
+        if (ACC == 1) {
 #pragma unroll
           for (int i = 0; i < NL; ++i) atomicAdd(&sy[li[i]], y[i]);
         } else {
@@ -583,8 +606,14 @@ void launch_cg_k(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpi
                  T* partial_out, bool fixup, bool cell_boundary) {
   const CGPlan& pl = op.cg;
   const size_t shmem = NC * 2 * ((size_t)pl.max_local + 1) * sizeof(T);
-  const bool atomic = op.opt.accumulate == 1 || (op.opt.accumulate == 0 && sizeof(T) == 8);
-  auto kern = atomic ? cg_wave_kernel<P, T, MODE, true, NC, FEAT> : cg_wave_kernel<P, T, MODE, false, NC, FEAT>;
+  // accumulate option: 1 atomics per colour, 2 read-add-write per colour,
+  // 3 atomics per wave; 0: FP64 atomics per wave (P2 FP64 128^3 0.348 ->
+  // 0.319 ms, vector P2 64^3 0.168 -> 0.122 ms against atomics per colour),
+  // FP32 read-add-write per colour (FP32 atomics are 3x slower)
+  const int acc = op.opt.accumulate == 3 || (op.opt.accumulate == 0 && sizeof(T) == 8) ? 2
+                  : op.opt.accumulate == 1 ? 1 : 0;
+  auto kern = acc == 2 ? cg_wave_kernel<P, T, MODE, 2, NC, FEAT>
+              : acc == 1 ? cg_wave_kernel<P, T, MODE, 1, NC, FEAT> : cg_wave_kernel<P, T, MODE, 0, NC, FEAT>;
   if (shmem > 160 * 1024)
     throw std::runtime_error("CG patch needs " + std::to_string(shmem) + " bytes of LDS (max 160 KiB): use smaller patch_cells");
   if (shmem > 65536)

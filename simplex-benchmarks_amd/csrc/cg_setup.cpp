@@ -38,18 +38,19 @@ void build_cg_plan(const Mesh& mesh, int p, const mf_options& opt, CGPlan& plan)
 
   // cells per wave by degree (P1 1024, P2 512, P3 128); a patch is one
   // workgroup of up to 4 waves.  The default patch is 4 waves' worth of
-  // cells (2 for FP64 P2, whose patches need twice the LDS), halved (down to
-  // 64 cells) until there are >= 1024 patches, so that small operators
-  // (multigrid levels) still fill the 256 CUs.  The 3-component operator,
-  // whose LDS is 3x larger, takes patches of the scalar wave size split over
-  // two waves (P2 FP64 64^3: 0.181 -> 0.167 ms against one wave of 512)
+  // cells, halved (down to 64 cells) until there are >= 1024 patches, so
+  // that small operators (multigrid levels) still fill the 256 CUs.  (FP64
+  // P2 with per-wave accumulation: 4 x 512 0.312 ms, 2 x 512 0.322 ms at
+  // 128^3.)  The 3-component operator, whose LDS is 3x larger, takes patches
+  // of the scalar wave size split over two waves (P2 FP64 64^3: 0.181 ->
+  // 0.167 ms against one wave of 512)
   const int per_wave = opt.cells_per_wave > 0
                            ? opt.cells_per_wave
                            : (p == 1 ? 1024 : (p == 2 ? 512 : 128)) / (plan.ncomp == 3 ? 2 : 1);
   int patch = opt.patch_cells;
   int waves_default = 0;
   if (patch <= 0) {
-    waves_default = plan.ncomp == 3 ? 2 : (p == 2 && plan.value_bytes == 8 ? 2 : 4);
+    waves_default = plan.ncomp == 3 ? 2 : 4;
     patch = waves_default * per_wave;
     while (patch > 64 && mesh.nc / patch < 1024) patch /= 2;
   }

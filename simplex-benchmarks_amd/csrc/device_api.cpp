@@ -6,6 +6,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "blas.h"
 #include "op.h"
 
 using namespace smf;
@@ -59,6 +60,11 @@ int mf_memcpy_dtoh(void* dst, const void* src, uint64_t bytes) {
 }
 int mf_memcpy_dtod(void* dst, const void* src, uint64_t bytes, void* stream) {
   if (bytes) MF_HIP_CALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return 0;
+}
+int mf_copy_kernel(void* dst, const void* src, uint64_t n, void* stream) {
+  blas_convert<double, double>((int64_t)n, (const double*)src, (double*)dst, (hipStream_t)stream);
+  MF_HIP_CALL(hipGetLastError());
   return 0;
 }
 int mf_memset(void* ptr, int32_t value, uint64_t bytes, void* stream) {

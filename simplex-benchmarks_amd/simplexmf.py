@@ -101,6 +101,7 @@ def lib():
         "mf_memcpy_htod": (C.c_int, [vp, vp, u64]),
         "mf_memcpy_dtoh": (C.c_int, [vp, vp, u64]),
         "mf_memcpy_dtod": (C.c_int, [vp, vp, u64, vp]),
+        "mf_copy_kernel": (C.c_int, [vp, vp, u64, vp]),
         "mf_memset": (C.c_int, [vp, i32, u64, vp]),
         "mf_synchronize": (C.c_int, []),
         "mf_stream_create": (C.c_int, [C.POINTER(vp)]),
@@ -232,7 +233,7 @@ _SIGNATURE_NAMES = [
     "mf_tvmult", "mf_compute_diagonal", "mf_compute_inverse_diagonal", "mf_m", "mf_get_info",
     "mf_get_cell_order", "mf_get_cell_dofs", "mf_get_cell_vertices", "mf_get_dof_points", "mf_get_constrained",
     "mf_debug_host_apply", "mf_last_error", "mf_device_count", "mf_device_name", "mf_malloc",
-    "mf_free", "mf_memcpy_htod", "mf_memcpy_dtoh", "mf_memcpy_dtod", "mf_memset",
+    "mf_free", "mf_memcpy_htod", "mf_memcpy_dtoh", "mf_memcpy_dtod", "mf_copy_kernel", "mf_memset",
     "mf_synchronize", "mf_stream_create", "mf_stream_destroy", "mf_stream_synchronize",
     "mf_event_create", "mf_event_destroy", "mf_event_record", "mf_event_elapsed_ms",
     "mf_mg_default_options", "mf_mg_create", "mf_mg_vmult", "mf_mg_get_info", "mf_mg_destroy",

@@ -88,10 +88,11 @@ def test_deterministic_and_tvmult(gpu):
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-@pytest.mark.parametrize("accumulate", [1, 2])
+@pytest.mark.parametrize("accumulate", [1, 2, 3])
 @pytest.mark.parametrize("patch_cells", [64, 1024])
 def test_accumulation_variants(gpu, orc, precision, accumulate, patch_cells):
-    """Both LDS accumulation schemes and extreme patch sizes give the oracle's result."""
+    """The LDS accumulation schemes (atomics or read-add-write per colour, atomics per wave) and extreme
+    patch sizes give the oracle's result, bitwise repeatably."""
     smf = gpu
     print(f"\ncase: CG p=2 {precision} accumulate={accumulate} patch={patch_cells}", flush=True)
     mesh = smf.Mesh.kuhn_cube(10, 0.2, 3)
