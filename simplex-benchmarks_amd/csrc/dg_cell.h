@@ -232,6 +232,19 @@ constexpr DGFaceMass<P> make_face_mass() {
   return m;
 }
 
+// acc += c x for an entry c of a compile-time table (c folds to a constant
+// after unrolling): zero entries vanish and +-1 entries add or subtract.
+// x * 0.0 may not be dropped under IEEE rules (x could be Inf or NaN), so
+// written as a plain product every zero of a table would cost an FMA (the
+// P3 face-gradient table has 118 nonzeros of 360).
+template <typename T>
+SMF_HDI void tmac(T& acc, double c, T x) {
+  if (c == 0.0) return;
+  if (c == 1.0) acc += x;
+  else if (c == -1.0) acc -= x;
+  else acc += T(c) * x;
+}
+
 // outward reference normals of the faces (face f opposite vertex f), unnormalised
 constexpr double kNhat[4][3] = {{1, 1, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
 
@@ -280,10 +293,10 @@ SMF_HDI void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, co
         T gx = T(0), gy = T(0), gz = T(0), uv = T(0);
 #pragma unroll
         for (int i = 0; i < NL; ++i) {
-          gx += T(CT.g[q][i][0]) * u[i];
-          gy += T(CT.g[q][i][1]) * u[i];
-          gz += T(CT.g[q][i][2]) * u[i];
-          if (MASS) uv += T(CT.v[q][i]) * u[i];
+          tmac(gx, CT.g[q][i][0], u[i]);
+          tmac(gy, CT.g[q][i][1], u[i]);
+          tmac(gz, CT.g[q][i][2], u[i]);
+          if (MASS) tmac(uv, CT.v[q][i], u[i]);
         }
         const T wl = w * lap;
         const T tx = wl * (M0 * gx + M1 * gy + M2 * gz);
@@ -292,8 +305,10 @@ SMF_HDI void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, co
         const T tv = MASS ? w * mass * adet * uv : T(0);
 #pragma unroll
         for (int i = 0; i < NL; ++i) {
-          y[i] += T(CT.g[q][i][0]) * tx + T(CT.g[q][i][1]) * ty + T(CT.g[q][i][2]) * tz;
-          if (MASS) y[i] += T(CT.v[q][i]) * tv;
+          tmac(y[i], CT.g[q][i][0], tx);
+          tmac(y[i], CT.g[q][i][1], ty);
+          tmac(y[i], CT.g[q][i][2], tz);
+          if (MASS) tmac(y[i], CT.v[q][i], tv);
         }
       } else {
 #pragma unroll
@@ -343,9 +358,9 @@ SMF_HDI void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, co
           T sx = T(0), sy = T(0), sz = T(0);
 #pragma unroll
           for (int i = 0; i < NL; ++i) {
-            sx += T(FT.dg[f][j][i][0]) * u[i];
-            sy += T(FT.dg[f][j][i][1]) * u[i];
-            sz += T(FT.dg[f][j][i][2]) * u[i];
+            tmac(sx, FT.dg[f][j][i][0], u[i]);
+            tmac(sy, FT.dg[f][j][i][1], u[i]);
+            tmac(sz, FT.dg[f][j][i][2], u[i]);
           }
           gn[j] = sx * nJ[0] + sy * nJ[1] + sz * nJ[2];
         }
@@ -380,9 +395,9 @@ SMF_HDI void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, co
             T sx = T(0), sy = T(0), sz = T(0);
 #pragma unroll
             for (int i = 0; i < NL; ++i) {
-              sx += T(FT.dg[3][j][i][0]) * ub[i];
-              sy += T(FT.dg[3][j][i][1]) * ub[i];
-              sz += T(FT.dg[3][j][i][2]) * ub[i];
+              tmac(sx, FT.dg[3][j][i][0], ub[i]);
+              tmac(sy, FT.dg[3][j][i][1], ub[i]);
+              tmac(sz, FT.dg[3][j][i][2], ub[i]);
             }
             gb[j] = sx * nb[0] + sy * nb[1] + sz * nb[2];
           }
@@ -401,9 +416,9 @@ SMF_HDI void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, co
         for (int k = 0; k < NFN; ++k) {
           T sj = T(0), sa = T(0);
 #pragma unroll
-          for (int l = 0; l < NFN; ++l) sj += T(FM.vv[k][l]) * J[l];
+          for (int l = 0; l < NFN; ++l) tmac(sj, FM.vv[k][l], J[l]);
 #pragma unroll
-          for (int j = 0; j < NFG; ++j) sa += T(FM.vg[k][j]) * A[j];
+          for (int j = 0; j < NFG; ++j) tmac(sa, FM.vg[k][j], A[j]);
           y[FT.fn[f][k]] += area2 * (sigma * sj - sa);
         }
         T tgj[NFG];
@@ -411,7 +426,7 @@ SMF_HDI void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, co
         for (int j = 0; j < NFG; ++j) {
           T sj = T(0);
 #pragma unroll
-          for (int k = 0; k < NFN; ++k) sj += T(FM.vg[k][j]) * J[k];
+          for (int k = 0; k < NFN; ++k) tmac(sj, FM.vg[k][j], J[k]);
           tgj[j] = T(-0.5) * area2 * sj;
         }
         // gradient test: y_i += sum_j tgj[j] * grad phi_i(y_j) . nJ
@@ -419,8 +434,11 @@ SMF_HDI void apply_cell(int64_t k, int64_t ncells, const T* __restrict__ geo, co
         for (int j = 0; j < NFG; ++j) {
           const T ax = tgj[j] * nJ[0], ay = tgj[j] * nJ[1], az = tgj[j] * nJ[2];
 #pragma unroll
-          for (int i = 0; i < NL; ++i)
-            y[i] += T(FT.dg[f][j][i][0]) * ax + T(FT.dg[f][j][i][1]) * ay + T(FT.dg[f][j][i][2]) * az;
+          for (int i = 0; i < NL; ++i) {
+            tmac(y[i], FT.dg[f][j][i][0], ax);
+            tmac(y[i], FT.dg[f][j][i][1], ay);
+            tmac(y[i], FT.dg[f][j][i][2], az);
+          }
         }
       } else {
         // diagonal: u = phi_i on this cell only (u+ = 0 inside, mirror on the boundary)
@@ -624,10 +642,10 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T fact
       T gx = T(0), gy = T(0), gz = T(0), uv = T(0);
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
-        gx += T(CT.g[q][i][0]) * u[i];
-        gy += T(CT.g[q][i][1]) * u[i];
-        gz += T(CT.g[q][i][2]) * u[i];
-        if (MASS) uv += T(CT.v[q][i]) * u[i];
+        tmac(gx, CT.g[q][i][0], u[i]);
+        tmac(gy, CT.g[q][i][1], u[i]);
+        tmac(gz, CT.g[q][i][2], u[i]);
+        if (MASS) tmac(uv, CT.v[q][i], u[i]);
       }
       const T wl = w * lap;
       const T tx = wl * (M0 * gx + M1 * gy + M2 * gz);
@@ -636,8 +654,10 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T fact
       const T tv = MASS ? w * mass * adet * uv : T(0);
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
-        y[i] += T(CT.g[q][i][0]) * tx + T(CT.g[q][i][1]) * ty + T(CT.g[q][i][2]) * tz;
-        if (MASS) y[i] += T(CT.v[q][i]) * tv;
+        tmac(y[i], CT.g[q][i][0], tx);
+        tmac(y[i], CT.g[q][i][1], ty);
+        tmac(y[i], CT.g[q][i][2], tz);
+        if (MASS) tmac(y[i], CT.v[q][i], tv);
       }
     } else {
 #pragma unroll
@@ -664,9 +684,9 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T fact
       T sx = T(0), sy = T(0), sz = T(0);
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
-        sx += T(FT.dg[3][j][i][0]) * u[i];
-        sy += T(FT.dg[3][j][i][1]) * u[i];
-        sz += T(FT.dg[3][j][i][2]) * u[i];
+        tmac(sx, FT.dg[3][j][i][0], u[i]);
+        tmac(sy, FT.dg[3][j][i][1], u[i]);
+        tmac(sz, FT.dg[3][j][i][2], u[i]);
       }
       gn[j] = sx * nJ[0] + sy * nJ[1] + sz * nJ[2];
     }
@@ -690,9 +710,9 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T fact
         T sx = T(0), sy = T(0), sz = T(0);
 #pragma unroll
         for (int i = 0; i < NL; ++i) {
-          sx += T(FT.dg[3][j][i][0]) * ub[i];
-          sy += T(FT.dg[3][j][i][1]) * ub[i];
-          sz += T(FT.dg[3][j][i][2]) * ub[i];
+          tmac(sx, FT.dg[3][j][i][0], ub[i]);
+          tmac(sy, FT.dg[3][j][i][1], ub[i]);
+          tmac(sz, FT.dg[3][j][i][2], ub[i]);
         }
         gb[j] = sx * nb[0] + sy * nb[1] + sz * nb[2];
       }
@@ -711,9 +731,9 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T fact
     for (int kk = 0; kk < NFN; ++kk) {
       T sj = T(0), sa = T(0);
 #pragma unroll
-      for (int l = 0; l < NFN; ++l) sj += T(FM.vv[kk][l]) * J[l];
+      for (int l = 0; l < NFN; ++l) tmac(sj, FM.vv[kk][l], J[l]);
 #pragma unroll
-      for (int j = 0; j < NFG; ++j) sa += T(FM.vg[kk][j]) * A[j];
+      for (int j = 0; j < NFG; ++j) tmac(sa, FM.vg[kk][j], A[j]);
       y[FT.fn[3][kk]] += area2 * (sigma * sj - sa);
     }
     T tgj[NFG];
@@ -721,15 +741,18 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T fact
     for (int j = 0; j < NFG; ++j) {
       T sj = T(0);
 #pragma unroll
-      for (int kk = 0; kk < NFN; ++kk) sj += T(FM.vg[kk][j]) * J[kk];
+      for (int kk = 0; kk < NFN; ++kk) tmac(sj, FM.vg[kk][j], J[kk]);
       tgj[j] = T(-0.5) * area2 * sj;
     }
 #pragma unroll
     for (int j = 0; j < NFG; ++j) {
       const T ax = tgj[j] * nJ[0], ay = tgj[j] * nJ[1], az = tgj[j] * nJ[2];
 #pragma unroll
-      for (int i = 0; i < NL; ++i)
-        y[i] += T(FT.dg[3][j][i][0]) * ax + T(FT.dg[3][j][i][1]) * ay + T(FT.dg[3][j][i][2]) * az;
+      for (int i = 0; i < NL; ++i) {
+        tmac(y[i], FT.dg[3][j][i][0], ax);
+        tmac(y[i], FT.dg[3][j][i][1], ay);
+        tmac(y[i], FT.dg[3][j][i][2], az);
+      }
     }
   } else {
     const T s = interior ? T(1) : T(2);
