@@ -1,0 +1,8 @@
+// DG SIPG kernels of degree 3 in FP64 (every operator kind, mode and kernel
+// variant of dg_kernels.h).
+#include "dg_kernels.h"
+
+namespace smf {
+template void launch_dg_degree<3, double>(const Operator&, int, double*, const double*, hipStream_t);
+template void launch_dg_ep_degree<3, double>(const Operator&, const double*, RowEpilogue<double>, hipStream_t);
+}  // namespace smf
