@@ -248,9 +248,13 @@ SMF_HDI void tmac(T& acc, double c, T x) {
 // outward reference normals of the faces (face f opposite vertex f), unnormalised
 constexpr double kNhat[4][3] = {{1, 1, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
 
+// a balanced tree of two-way selects on the bits of f: three v_cndmask per
+// 32-bit half, no branches (f differs between the lanes of a wave)
 template <typename T>
 SMF_HD inline T select4(int f, T a0, T a1, T a2, T a3) {
-  return f == 0 ? a0 : (f == 1 ? a1 : (f == 2 ? a2 : a3));
+  const bool odd = (f & 1) != 0;
+  const T lo = odd ? a1 : a0, hi = odd ? a3 : a2;
+  return (f & 2) ? hi : lo;
 }
 
 // One cell k: MODE 0 dst rows of k = (A src) rows; MODE 1 = diagonal of A.
@@ -529,16 +533,26 @@ SMF_HDI int lane_slot4(int g, int t, int f) {
 // section, so their loads stay in flight across the cell term.
 template <typename T>
 struct LaneGeo {
-  T nJ[3], M[6], area2, adet, tau, len, n[3];
-  T jb[9], adetb, taub;
+  T nJ[3], M[6], area2, adet;
+  T nb[3], sigma;  // the lane's face record
   int64_t kn;
   int fb;
   bool interior;
 };
 
+// Per-face record of the four-lane kernel, [cell][local face][4], built once
+// at set-up from the geometry of both cells of the face: the neighbour's
+// J^-1 n (its barycentric gradients in the frame of its face fb, dotted with
+// this cell's outward unit normal n; zero on the boundary) and the face's
+// penalty sigma (penalty factor included).  Lane f of cell k reads the 4
+// values at (4 k + f) 4: one contiguous 32 B (FP64) read per lane instead of
+// eleven gathers of the neighbour's geometry arrays.
+constexpr int kFaceRec = 4;
+
 template <int P, typename T>
 SMF_HDI LaneGeo<T> lane_geometry(int64_t k, int f, int64_t ncells, const T* __restrict__ geo,
-                                 const int32_t* __restrict__ nbr, const uint32_t* __restrict__ nface, T lap) {
+                                 const T* __restrict__ fgeo, const int32_t* __restrict__ nbr,
+                                 const uint32_t* __restrict__ nface, T lap) {
   constexpr DGFaceTab<P> FT = make_face_tab<P>();
   using std::sqrt;
   LaneGeo<T> g;
@@ -550,28 +564,23 @@ SMF_HDI LaneGeo<T> lane_geometry(int64_t k, int f, int64_t ncells, const T* __re
     gl[3][d] = geo[(3 * d + 2) * ncells + k];
     gl[0][d] = -(gl[1][d] + gl[2][d] + gl[3][d]);
   }
-  const T adet = geo[9 * ncells + k], tau = geo[10 * ncells + k];
+  const T adet = geo[9 * ncells + k];
   g.adet = adet;
-  // neighbour geometry issued early so that its latency overlaps the rest
   const int32_t kb = nbr[f * ncells + k];
   g.interior = kb >= 0;
   g.kn = g.interior ? kb : k;
   g.fb = g.interior ? (int)((nface[k] >> (8 * f)) & 3) : 3;
-#pragma unroll
-  for (int t = 0; t < 9; ++t) g.jb[t] = geo[t * ncells + g.kn];
-  g.adetb = geo[9 * ncells + g.kn];
-  g.taub = geo[10 * ncells + g.kn];
-  g.tau = tau;
+  const T* fr = fgeo + (k * 4 + f) * kFaceRec;
+  g.nb[0] = fr[0];
+  g.nb[1] = fr[1];
+  g.nb[2] = fr[2];
+  g.sigma = fr[3];
   // own face f: J^-T nhat_f = -grad lambda_f; outward unit normal, area
   T gf[3];
   for (int d = 0; d < 3; ++d) gf[d] = select4<T>(f, gl[0][d], gl[1][d], gl[2][d], gl[3][d]);
   const T len = sqrt(gf[0] * gf[0] + gf[1] * gf[1] + gf[2] * gf[2]);
   const T il = T(1) / len;
   const T n0 = -gf[0] * il, n1 = -gf[1] * il, n2 = -gf[2] * il;
-  g.n[0] = n0;
-  g.n[1] = n1;
-  g.n[2] = n2;
-  g.len = len;
   g.area2 = adet * len * lap;
   T bo[4];
   for (int a = 0; a < 4; ++a) bo[a] = gl[a][0] * n0 + gl[a][1] * n1 + gl[a][2] * n2;
@@ -603,8 +612,8 @@ SMF_HDI LaneGeo<T> lane_geometry(int64_t k, int f, int64_t ncells, const T* __re
 // four lanes' rows add up to apply_cell's result.  Storing in the lane frame
 // keeps the permutation out of the registers live across the lane's work.
 template <int P, typename T, int NC, bool MASS, int MODE>
-SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T factor, T mass, T lap,
-                            const T* __restrict__ src, T* __restrict__ yrow, int pkind) {
+SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass, T lap,
+                            const T* __restrict__ src, T* __restrict__ yrow) {
   constexpr DGCellTab<P, MASS> CT = make_cell_tab<P, MASS>();
   constexpr DGCellShare<P, MASS> CS = make_cell_share<P, MASS>();
   constexpr DGFaceTab<P> FT = make_face_tab<P>();
@@ -670,13 +679,7 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T fact
     }
   }
   // ---- face f (= face 3 of the lane frame) ----
-  T sigma;
-  if (pkind == 1) {  // (p+1)(p+2) |F| / min |K|: factor * max(len, len |det J| / |det J+|)
-    const T lb = g.len * adet / g.adetb;
-    sigma = factor * (g.len > lb ? g.len : lb);
-  } else {
-    sigma = factor * (interior ? (g.tau > g.taub ? g.tau : g.taub) : g.tau);
-  }
+  const T sigma = g.sigma;
   if (MODE == 0) {
     T gn[NFG];
 #pragma unroll
@@ -694,17 +697,8 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T fact
 #pragma unroll
     for (int j = 0; j < NFG; ++j) gb[j] = T(0);
     if (interior) {
-      // the neighbour's J^-1 n in the frame of its face fb
-      T bb[4];
-      bb[1] = g.jb[0] * g.n[0] + g.jb[3] * g.n[1] + g.jb[6] * g.n[2];
-      bb[2] = g.jb[1] * g.n[0] + g.jb[4] * g.n[1] + g.jb[7] * g.n[2];
-      bb[3] = g.jb[2] * g.n[0] + g.jb[5] * g.n[1] + g.jb[8] * g.n[2];
-      bb[0] = -(bb[1] + bb[2] + bb[3]);
-      T nb[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        nb[d] = select4<T>(g.fb, bb[FT.sigma[0][d + 1]], bb[FT.sigma[1][d + 1]], bb[FT.sigma[2][d + 1]],
-                           bb[FT.sigma[3][d + 1]]);
+      // the neighbour's J^-1 n in the frame of its face fb (face record)
+      const T* nb = g.nb;
 #pragma unroll
       for (int j = 0; j < NFG; ++j) {
         T sx = T(0), sy = T(0), sz = T(0);
@@ -786,10 +780,11 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T fact
 // Lane f of cell k, component c (geometry and component together)
 template <int P, typename T, int NC, bool MASS, int MODE>
 SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __restrict__ geo,
-                             const int32_t* __restrict__ nbr, const uint32_t* __restrict__ nface, T factor, T mass,
-                             T lap, const T* __restrict__ src, T* __restrict__ yrow, int pkind = 0) {
-  const LaneGeo<T> g = lane_geometry<P, T>(k, f, ncells, geo, nbr, nface, lap);
-  lane_component<P, T, NC, MASS, MODE>(g, k, f, c, factor, mass, lap, src, yrow, pkind);
+                             const T* __restrict__ fgeo, const int32_t* __restrict__ nbr,
+                             const uint32_t* __restrict__ nface, T mass, T lap, const T* __restrict__ src,
+                             T* __restrict__ yrow) {
+  const LaneGeo<T> g = lane_geometry<P, T>(k, f, ncells, geo, fgeo, nbr, nface, lap);
+  lane_component<P, T, NC, MASS, MODE>(g, k, f, c, mass, lap, src, yrow);
 }
 
 }  // namespace dgt

@@ -455,10 +455,10 @@ __global__ __launch_bounds__(kThreads) void dg_cell_kernel_v2(int64_t ncells, co
 constexpr int kQuadCells = 64;  // cells per 256-thread block
 template <int P, typename T, int NC, bool MASS, int MODE>
 __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells, const T* __restrict__ geo,
+                                                                  const T* __restrict__ fgeo,
                                                                   const int32_t* __restrict__ nbr,
-                                                                  const uint32_t* __restrict__ nface, T factor,
-                                                                  T mass, T lap, const T* __restrict__ src,
-                                                                  T* __restrict__ dst, int pkind) {
+                                                                  const uint32_t* __restrict__ nface, T mass, T lap,
+                                                                  const T* __restrict__ src, T* __restrict__ dst) {
   constexpr int NL = dgt::nloc(P);
   __shared__ T ybuf[kQuadCells][4][NL + 1];
   const int cl = threadIdx.x >> 2, f = threadIdx.x & 3;
@@ -467,10 +467,10 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells,
   const bool active = k < ncells;
   const int64_t kc = active ? k : ncells - 1;  // every lane reaches the barriers
   // the lane's geometry once for all components
-  const dgt::LaneGeo<T> g = dgt::lane_geometry<P, T>(kc, f, ncells, geo, nbr, nface, lap);
+  const dgt::LaneGeo<T> g = dgt::lane_geometry<P, T>(kc, f, ncells, geo, fgeo, nbr, nface, lap);
 #pragma unroll 1
   for (int c = 0; c < NC; ++c) {
-    dgt::lane_component<P, T, NC, MASS, MODE>(g, kc, f, c, factor, mass, lap, src, ybuf[cl][f], pkind);
+    dgt::lane_component<P, T, NC, MASS, MODE>(g, kc, f, c, mass, lap, src, ybuf[cl][f]);
     __syncthreads();
     if (active)
 #pragma unroll
@@ -489,18 +489,17 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells,
 // level).
 template <int P, typename T>
 __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_ep_kernel(int64_t ncells, const T* __restrict__ geo,
+                                                                     const T* __restrict__ fgeo,
                                                                      const int32_t* __restrict__ nbr,
-                                                                     const uint32_t* __restrict__ nface, T factor,
-                                                                     const T* __restrict__ src, int pkind,
-                                                                     RowEpilogue<T> ep) {
+                                                                     const uint32_t* __restrict__ nface,
+                                                                     const T* __restrict__ src, RowEpilogue<T> ep) {
   constexpr int NL = dgt::nloc(P);
   __shared__ T ybuf[kQuadCells][4][NL + 1];
   const int cl = threadIdx.x >> 2, f = threadIdx.x & 3;
   const int64_t k = xcd_block(blockIdx.x, gridDim.x) * kQuadCells + cl;
   const bool active = k < ncells;
   const int64_t kc = active ? k : ncells - 1;
-  dgt::apply_cell_lane<P, T, 1, false, 0>(kc, f, 0, ncells, geo, nbr, nface, factor, T(0), T(1), src, ybuf[cl][f],
-                                          pkind);
+  dgt::apply_cell_lane<P, T, 1, false, 0>(kc, f, 0, ncells, geo, fgeo, nbr, nface, T(0), T(1), src, ybuf[cl][f]);
   __syncthreads();
   if (active)
 #pragma unroll
@@ -521,8 +520,8 @@ void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
     if (pl.n_cells > 0)
       hipLaunchKernelGGL((dg_quad_kernel<P, T, NC, MASS, MODE>),
                          dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells)), dim3(4 * kQuadCells), 0, s,
-                         pl.n_cells, op.d_dg_geo.as<T>(), op.d_dg_nbr.as<int32_t>(), op.d_dg_nface.as<uint32_t>(),
-                         (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, dst, pl.penalty_kind);
+                         pl.n_cells, op.d_dg_geo.as<T>(), op.d_dg_fgeo.as<T>(), op.d_dg_nbr.as<int32_t>(),
+                         op.d_dg_nface.as<uint32_t>(), (T)pl.mass, (T)pl.laplace, src, dst);
     SMF_HIP(hipGetLastError());
     return;
   }
@@ -564,8 +563,8 @@ void launch_dg_ep_t(const Operator& op, const T* x, RowEpilogue<T> ep, hipStream
   const DGPlan& pl = op.dg;
   if (pl.n_cells > 0)
     hipLaunchKernelGGL((dg_quad_ep_kernel<P, T>), dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells)),
-                       dim3(4 * kQuadCells), 0, s, pl.n_cells, op.d_dg_geo.as<T>(), op.d_dg_nbr.as<int32_t>(),
-                       op.d_dg_nface.as<uint32_t>(), (T)pl.penalty_factor, x, pl.penalty_kind, ep);
+                       dim3(4 * kQuadCells), 0, s, pl.n_cells, op.d_dg_geo.as<T>(), op.d_dg_fgeo.as<T>(),
+                       op.d_dg_nbr.as<int32_t>(), op.d_dg_nface.as<uint32_t>(), x, ep);
   SMF_HIP(hipGetLastError());
 }
 

@@ -107,6 +107,54 @@ void build_dg_curved(const Mesh& mesh, const mf_options& opt, const Quadrature& 
   }
   if (bad) throw std::runtime_error("curved cells: the Jacobian determinant changes sign inside a cell");
 }
+// Face records of the four-lane kernel (dgt::kFaceRec values per cell and
+// local face, [cell][face][4]): the neighbour's barycentric gradients of the
+// reference directions of the frame of its face fb (frame vertices kFV[fb]
+// ascending, then fb; direction d is frame vertex d + 1), dotted with
+// this cell's outward unit normal, and the face penalty sigma with the
+// penalty factor: kind 0 factor max(tau-, tau+) (tau on the boundary), kind 1
+// factor max(len, len |det J-| / |det J+|), len = |grad lambda_f|.  Formed in
+// FP64 from the cell geometry (geo rows: J^-T row-major, |det J|, tau).
+void build_dg_face_records(DGPlan& plan) {
+  const int64_t nc = plan.n_cells;
+  plan.fgeo.assign((size_t)nc * 4 * dgt::kFaceRec, 0.0);
+  auto grad = [&](int64_t k, int a, double* g) {  // grad lambda_a of cell k
+    for (int d = 0; d < 3; ++d) {
+      if (a == 0) {
+        g[d] = 0;
+        for (int b = 0; b < 3; ++b) g[d] -= plan.geo[(size_t)(3 * d + b) * nc + k];
+      } else {
+        g[d] = plan.geo[(size_t)(3 * d + a - 1) * nc + k];
+      }
+    }
+  };
+  for (int64_t k = 0; k < nc; ++k) {
+    const double adet = plan.geo[(size_t)9 * nc + k], tau = plan.geo[(size_t)10 * nc + k];
+    for (int f = 0; f < 4; ++f) {
+      double* r = &plan.fgeo[((size_t)k * 4 + f) * dgt::kFaceRec];
+      double gf[3];
+      grad(k, f, gf);
+      const double len = std::sqrt(gf[0] * gf[0] + gf[1] * gf[1] + gf[2] * gf[2]);
+      const double n[3] = {-gf[0] / len, -gf[1] / len, -gf[2] / len};
+      const int32_t kb = plan.nbr[(size_t)f * nc + k];
+      if (kb >= 0) {
+        const int fb = (int)((plan.nface[k] >> (8 * f)) & 3);
+        // reference direction d of the frame is its vertex d + 1: (kFV[fb][0], kFV[fb][1], kFV[fb][2], fb)
+        const int frame[3] = {tab::kFV[fb][1], tab::kFV[fb][2], fb};
+        for (int d = 0; d < 3; ++d) {
+          double gb[3];
+          grad(kb, frame[d], gb);
+          r[d] = gb[0] * n[0] + gb[1] * n[1] + gb[2] * n[2];
+        }
+      }
+      const double adetb = kb >= 0 ? plan.geo[(size_t)9 * nc + kb] : adet;
+      const double taub = kb >= 0 ? plan.geo[(size_t)10 * nc + kb] : tau;
+      r[3] = plan.penalty_kind == 1 ? plan.penalty_factor * std::max(len, len * adet / adetb)
+                                    : plan.penalty_factor * std::max(tau, taub);
+    }
+  }
+}
+
 }  // namespace
 
 int cg_quadrature_degree(int p);
@@ -264,6 +312,7 @@ void build_dg_plan(const Mesh& mesh, int p, int ncomp, const mf_options& opt, bo
       for (int d = 0; d < 3; ++d) plan.face_tab.push_back(x[d]);
     }
   if (curved) build_dg_curved(mesh, opt, qc, qf, is_bdry, plan);
+  else build_dg_face_records(plan);
 }
 
 namespace {
@@ -273,6 +322,7 @@ void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
   const DGPlan& pl = op.dg;
   std::vector<T> geo(pl.geo.begin(), pl.geo.end());
   if (op.opt.dg_kernel == 3) {
+    const std::vector<T> fgeo(pl.fgeo.begin(), pl.fgeo.end());
     // four lanes per cell (one face each), summed in lane order
     constexpr int NL = dgt::nloc(P);
     T yr[4][NL];
@@ -280,13 +330,11 @@ void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
       for (int c = 0; c < NC; ++c) {
         for (int f = 0; f < 4; ++f) {
           if (mode == 0)
-            dgt::apply_cell_lane<P, T, NC, MASS, 0>(k, f, c, pl.n_cells, geo.data(), pl.nbr.data(), pl.nface.data(),
-                                                    (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, yr[f],
-                                                    pl.penalty_kind);
+            dgt::apply_cell_lane<P, T, NC, MASS, 0>(k, f, c, pl.n_cells, geo.data(), fgeo.data(), pl.nbr.data(),
+                                                    pl.nface.data(), (T)pl.mass, (T)pl.laplace, src, yr[f]);
           else
-            dgt::apply_cell_lane<P, T, NC, MASS, 1>(k, f, c, pl.n_cells, geo.data(), pl.nbr.data(), pl.nface.data(),
-                                                    (T)pl.penalty_factor, (T)pl.mass, (T)pl.laplace, src, yr[f],
-                                                    pl.penalty_kind);
+            dgt::apply_cell_lane<P, T, NC, MASS, 1>(k, f, c, pl.n_cells, geo.data(), fgeo.data(), pl.nbr.data(),
+                                                    pl.nface.data(), (T)pl.mass, (T)pl.laplace, src, yr[f]);
         }
         constexpr dgt::LaneInv<P> LI = dgt::make_lane_inv<P>();
         for (int i = 0; i < NL; ++i)
