@@ -93,12 +93,14 @@ void upload_dg(Operator& op) {
   if (op.precision == MF_F64) {
     op.d_dg_geo.upload(pl.geo);
     if (!pl.fgeo.empty()) op.d_dg_fgeo.upload(pl.fgeo);
+    if (!pl.bgeo.empty()) op.d_dg_bgeo.upload(pl.bgeo);
     op.d_dg_cell_tab.upload(pl.cell_tab);
     op.d_dg_face_tab.upload(pl.face_tab);
     if (pl.curved) op.d_dg_jgeo.upload(pl.jgeo);
   } else {
     op.d_dg_geo.upload(convert<float>(pl.geo));
     if (!pl.fgeo.empty()) op.d_dg_fgeo.upload(convert<float>(pl.fgeo));
+    if (!pl.bgeo.empty()) op.d_dg_bgeo.upload(convert<float>(pl.bgeo));
     op.d_dg_cell_tab.upload(convert<float>(pl.cell_tab));
     op.d_dg_face_tab.upload(convert<float>(pl.face_tab));
     if (pl.curved) op.d_dg_jgeo.upload(convert<float>(pl.jgeo));

@@ -549,22 +549,41 @@ struct LaneGeo {
 // eleven gathers of the neighbour's geometry arrays.
 constexpr int kFaceRec = 4;
 
+// Barycentric metric of a cell, B_ab = |det J| grad lambda_a . grad lambda_b
+// (a, b = 0..3): symmetric, rows summing to zero, so the six off-diagonal
+// entries determine it.  kBPair[a][b] is the slot of (a, b) in the cell's
+// [7][ncells] record (b01 b02 b03 b12 b13 b23 |det J|).
+constexpr int kBPair[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
+constexpr int kBGeo = 7;
+// vertex of frame role r of face f: 0 origin kFV[f][0], 1 and 2 the other
+// face vertices kFV[f][1], kFV[f][2], 3 the opposite vertex f
+constexpr int frame_vertex(int f, int r) { return r == 3 ? f : tab::kFV[f][r]; }
+template <int RX, int RY, typename T>
+SMF_HDI T bsel(int f, const T* b) {
+  return select4<T>(f, b[kBPair[frame_vertex(0, RX)][frame_vertex(0, RY)]],
+                    b[kBPair[frame_vertex(1, RX)][frame_vertex(1, RY)]],
+                    b[kBPair[frame_vertex(2, RX)][frame_vertex(2, RY)]],
+                    b[kBPair[frame_vertex(3, RX)][frame_vertex(3, RY)]]);
+}
+
+// Geometry of lane f of cell k (shared by all components of the cell).  The
+// lane frame has origin vertex o = kFV[f][0] and reference directions
+// (a, b, f) = (kFV[f][1], kFV[f][2], f).  Its metric is B restricted to
+// {a, b, f} (diagonal from the zero row sums), and because face f is
+// opposite vertex f, the outward unit normal is n = -grad lambda_f / |grad
+// lambda_f|: J^-1 n in the frame is -B[., f] / s and the face Jacobian
+// factor |det J| |grad lambda_f| is s, with s = sqrt(|det J| B_ff).  The
+// neighbour's J^-1 n and the penalty come from the face record.
 template <int P, typename T>
-SMF_HDI LaneGeo<T> lane_geometry(int64_t k, int f, int64_t ncells, const T* __restrict__ geo,
+SMF_HDI LaneGeo<T> lane_geometry(int64_t k, int f, int64_t ncells, const T* __restrict__ bgeo,
                                  const T* __restrict__ fgeo, const int32_t* __restrict__ nbr,
                                  const uint32_t* __restrict__ nface, T lap) {
-  constexpr DGFaceTab<P> FT = make_face_tab<P>();
   using std::sqrt;
   LaneGeo<T> g;
-  // barycentric gradients grad lambda_a: a = 1..3 are the columns of J^-T
-  T gl[4][3];
-  for (int d = 0; d < 3; ++d) {
-    gl[1][d] = geo[(3 * d + 0) * ncells + k];
-    gl[2][d] = geo[(3 * d + 1) * ncells + k];
-    gl[3][d] = geo[(3 * d + 2) * ncells + k];
-    gl[0][d] = -(gl[1][d] + gl[2][d] + gl[3][d]);
-  }
-  const T adet = geo[9 * ncells + k];
+  T b[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) b[t] = bgeo[t * ncells + k];
+  const T adet = bgeo[6 * ncells + k];
   g.adet = adet;
   const int32_t kb = nbr[f * ncells + k];
   g.interior = kb >= 0;
@@ -575,31 +594,22 @@ SMF_HDI LaneGeo<T> lane_geometry(int64_t k, int f, int64_t ncells, const T* __re
   g.nb[1] = fr[1];
   g.nb[2] = fr[2];
   g.sigma = fr[3];
-  // own face f: J^-T nhat_f = -grad lambda_f; outward unit normal, area
-  T gf[3];
-  for (int d = 0; d < 3; ++d) gf[d] = select4<T>(f, gl[0][d], gl[1][d], gl[2][d], gl[3][d]);
-  const T len = sqrt(gf[0] * gf[0] + gf[1] * gf[1] + gf[2] * gf[2]);
-  const T il = T(1) / len;
-  const T n0 = -gf[0] * il, n1 = -gf[1] * il, n2 = -gf[2] * il;
-  g.area2 = adet * len * lap;
-  T bo[4];
-  for (int a = 0; a < 4; ++a) bo[a] = gl[a][0] * n0 + gl[a][1] * n1 + gl[a][2] * n2;
-  // lane frame: new vertex j <- old vertex sigma[f][j]; reference direction
-  // d (1..3) is new vertex d
-  T G[3][3];
-  for (int d = 0; d < 3; ++d) {
-    g.nJ[d] = select4<T>(f, bo[FT.sigma[0][d + 1]], bo[FT.sigma[1][d + 1]], bo[FT.sigma[2][d + 1]],
-                         bo[FT.sigma[3][d + 1]]);
-    for (int e = 0; e < 3; ++e)
-      G[d][e] = select4<T>(f, gl[FT.sigma[0][d + 1]][e], gl[FT.sigma[1][d + 1]][e], gl[FT.sigma[2][d + 1]][e],
-                           gl[FT.sigma[3][d + 1]][e]);
-  }
-  g.M[0] = adet * (G[0][0] * G[0][0] + G[0][1] * G[0][1] + G[0][2] * G[0][2]);
-  g.M[1] = adet * (G[0][0] * G[1][0] + G[0][1] * G[1][1] + G[0][2] * G[1][2]);
-  g.M[2] = adet * (G[0][0] * G[2][0] + G[0][1] * G[2][1] + G[0][2] * G[2][2]);
-  g.M[3] = adet * (G[1][0] * G[1][0] + G[1][1] * G[1][1] + G[1][2] * G[1][2]);
-  g.M[4] = adet * (G[1][0] * G[2][0] + G[1][1] * G[2][1] + G[1][2] * G[2][2]);
-  g.M[5] = adet * (G[2][0] * G[2][0] + G[2][1] * G[2][1] + G[2][2] * G[2][2]);
+  // B[x][y] for the frame roles x, y of lane f: one select over the four faces
+  const T bab = bsel<1, 2>(f, b), baf = bsel<1, 3>(f, b), bbf = bsel<2, 3>(f, b);
+  const T bao = bsel<1, 0>(f, b), bbo = bsel<2, 0>(f, b), bfo = bsel<3, 0>(f, b);
+  const T bff = -((bfo + baf) + bbf);
+  g.M[0] = -((bao + bab) + baf);
+  g.M[1] = bab;
+  g.M[2] = baf;
+  g.M[3] = -((bbo + bab) + bbf);
+  g.M[4] = bbf;
+  g.M[5] = bff;
+  const T sf = sqrt(adet * bff);
+  const T is = T(-1) / sf;
+  g.nJ[0] = baf * is;
+  g.nJ[1] = bbf * is;
+  g.nJ[2] = bff * is;
+  g.area2 = sf * lap;
   return g;
 }
 
@@ -779,11 +789,11 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass
 
 // Lane f of cell k, component c (geometry and component together)
 template <int P, typename T, int NC, bool MASS, int MODE>
-SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __restrict__ geo,
+SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __restrict__ bgeo,
                              const T* __restrict__ fgeo, const int32_t* __restrict__ nbr,
                              const uint32_t* __restrict__ nface, T mass, T lap, const T* __restrict__ src,
                              T* __restrict__ yrow) {
-  const LaneGeo<T> g = lane_geometry<P, T>(k, f, ncells, geo, fgeo, nbr, nface, lap);
+  const LaneGeo<T> g = lane_geometry<P, T>(k, f, ncells, bgeo, fgeo, nbr, nface, lap);
   lane_component<P, T, NC, MASS, MODE>(g, k, f, c, mass, lap, src, yrow);
 }
 

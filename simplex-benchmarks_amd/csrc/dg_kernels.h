@@ -454,7 +454,7 @@ __global__ __launch_bounds__(kThreads) void dg_cell_kernel_v2(int64_t ncells, co
 // in lane order (deterministic); each lane then stores every fourth row.
 constexpr int kQuadCells = 64;  // cells per 256-thread block
 template <int P, typename T, int NC, bool MASS, int MODE>
-__global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells, const T* __restrict__ geo,
+__global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells, const T* __restrict__ bgeo,
                                                                   const T* __restrict__ fgeo,
                                                                   const int32_t* __restrict__ nbr,
                                                                   const uint32_t* __restrict__ nface, T mass, T lap,
@@ -462,33 +462,34 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells,
   constexpr int NL = dgt::nloc(P);
   __shared__ T ybuf[kQuadCells][4][NL + 1];
   const int cl = threadIdx.x >> 2, f = threadIdx.x & 3;
-  const int64_t k0 = xcd_block(blockIdx.x, gridDim.x) * kQuadCells;
+  // one component per block, the components of a cell group in consecutive
+  // logical blocks (same XCD, so the second and third read the cells'
+  // geometry and neighbours from L2); no loop over components, whose
+  // invariant table constants the compiler would keep in registers
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int c = NC == 1 ? 0 : (int)(blk % NC);
+  const int64_t k0 = (NC == 1 ? blk : blk / NC) * kQuadCells;
   const int64_t k = k0 + cl;
   const bool active = k < ncells;
-  const int64_t kc = active ? k : ncells - 1;  // every lane reaches the barriers
-  // the lane's geometry once for all components
-  const dgt::LaneGeo<T> g = dgt::lane_geometry<P, T>(kc, f, ncells, geo, fgeo, nbr, nface, lap);
-#pragma unroll 1
-  for (int c = 0; c < NC; ++c) {
-    dgt::lane_component<P, T, NC, MASS, MODE>(g, kc, f, c, mass, lap, src, ybuf[cl][f]);
-    __syncthreads();
-    if (active)
+  const int64_t kc = active ? k : ncells - 1;  // every lane reaches the barrier
+  const dgt::LaneGeo<T> g = dgt::lane_geometry<P, T>(kc, f, ncells, bgeo, fgeo, nbr, nface, lap);
+  dgt::lane_component<P, T, NC, MASS, MODE>(g, kc, f, c, mass, lap, src, ybuf[cl][f]);
+  __syncthreads();
+  if (active)
 #pragma unroll
-      for (int t = 0; t < (NL + 3) / 4; ++t)
-        if (f + 4 * t < NL)
-          dst[(k * NC + c) * NL + f + 4 * t] =
-              ((ybuf[cl][0][dgt::lane_slot4<P>(0, t, f)] + ybuf[cl][1][dgt::lane_slot4<P>(1, t, f)]) +
-               ybuf[cl][2][dgt::lane_slot4<P>(2, t, f)]) +
-              ybuf[cl][3][dgt::lane_slot4<P>(3, t, f)];
-    __syncthreads();
-  }
+    for (int t = 0; t < (NL + 3) / 4; ++t)
+      if (f + 4 * t < NL)
+        dst[(k * NC + c) * NL + f + 4 * t] =
+            ((ybuf[cl][0][dgt::lane_slot4<P>(0, t, f)] + ybuf[cl][1][dgt::lane_slot4<P>(1, t, f)]) +
+             ybuf[cl][2][dgt::lane_slot4<P>(2, t, f)]) +
+            ybuf[cl][3][dgt::lane_slot4<P>(3, t, f)];
 }
 
 // The four-lane kernel for scalar SIPG with a RowEpilogue on every row
 // instead of the store (the Chebyshev step or the residual of a multigrid
 // level).
 template <int P, typename T>
-__global__ __launch_bounds__(4 * kQuadCells) void dg_quad_ep_kernel(int64_t ncells, const T* __restrict__ geo,
+__global__ __launch_bounds__(4 * kQuadCells) void dg_quad_ep_kernel(int64_t ncells, const T* __restrict__ bgeo,
                                                                      const T* __restrict__ fgeo,
                                                                      const int32_t* __restrict__ nbr,
                                                                      const uint32_t* __restrict__ nface,
@@ -499,7 +500,7 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_ep_kernel(int64_t ncel
   const int64_t k = xcd_block(blockIdx.x, gridDim.x) * kQuadCells + cl;
   const bool active = k < ncells;
   const int64_t kc = active ? k : ncells - 1;
-  dgt::apply_cell_lane<P, T, 1, false, 0>(kc, f, 0, ncells, geo, fgeo, nbr, nface, T(0), T(1), src, ybuf[cl][f]);
+  dgt::apply_cell_lane<P, T, 1, false, 0>(kc, f, 0, ncells, bgeo, fgeo, nbr, nface, T(0), T(1), src, ybuf[cl][f]);
   __syncthreads();
   if (active)
 #pragma unroll
@@ -519,8 +520,8 @@ void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
   if (op.opt.dg_kernel == 3 || (op.opt.dg_kernel == 0 && P >= 2)) {
     if (pl.n_cells > 0)
       hipLaunchKernelGGL((dg_quad_kernel<P, T, NC, MASS, MODE>),
-                         dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells)), dim3(4 * kQuadCells), 0, s,
-                         pl.n_cells, op.d_dg_geo.as<T>(), op.d_dg_fgeo.as<T>(), op.d_dg_nbr.as<int32_t>(),
+                         dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells * NC)), dim3(4 * kQuadCells), 0, s,
+                         pl.n_cells, op.d_dg_bgeo.as<T>(), op.d_dg_fgeo.as<T>(), op.d_dg_nbr.as<int32_t>(),
                          op.d_dg_nface.as<uint32_t>(), (T)pl.mass, (T)pl.laplace, src, dst);
     SMF_HIP(hipGetLastError());
     return;
@@ -563,7 +564,7 @@ void launch_dg_ep_t(const Operator& op, const T* x, RowEpilogue<T> ep, hipStream
   const DGPlan& pl = op.dg;
   if (pl.n_cells > 0)
     hipLaunchKernelGGL((dg_quad_ep_kernel<P, T>), dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells)),
-                       dim3(4 * kQuadCells), 0, s, pl.n_cells, op.d_dg_geo.as<T>(), op.d_dg_fgeo.as<T>(),
+                       dim3(4 * kQuadCells), 0, s, pl.n_cells, op.d_dg_bgeo.as<T>(), op.d_dg_fgeo.as<T>(),
                        op.d_dg_nbr.as<int32_t>(), op.d_dg_nface.as<uint32_t>(), x, ep);
   SMF_HIP(hipGetLastError());
 }

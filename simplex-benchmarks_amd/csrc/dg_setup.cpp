@@ -107,7 +107,8 @@ void build_dg_curved(const Mesh& mesh, const mf_options& opt, const Quadrature& 
   }
   if (bad) throw std::runtime_error("curved cells: the Jacobian determinant changes sign inside a cell");
 }
-// Face records of the four-lane kernel (dgt::kFaceRec values per cell and
+// Data of the four-lane kernel: the cell's barycentric metric (dgt::kBGeo
+// values per cell, [7][cell]) and face records (dgt::kFaceRec values per cell and
 // local face, [cell][face][4]): the neighbour's barycentric gradients of the
 // reference directions of the frame of its face fb (frame vertices kFV[fb]
 // ascending, then fb; direction d is frame vertex d + 1), dotted with
@@ -115,7 +116,7 @@ void build_dg_curved(const Mesh& mesh, const mf_options& opt, const Quadrature& 
 // penalty factor: kind 0 factor max(tau-, tau+) (tau on the boundary), kind 1
 // factor max(len, len |det J-| / |det J+|), len = |grad lambda_f|.  Formed in
 // FP64 from the cell geometry (geo rows: J^-T row-major, |det J|, tau).
-void build_dg_face_records(DGPlan& plan) {
+void build_dg_lane_data(DGPlan& plan) {
   const int64_t nc = plan.n_cells;
   plan.fgeo.assign((size_t)nc * 4 * dgt::kFaceRec, 0.0);
   auto grad = [&](int64_t k, int a, double* g) {  // grad lambda_a of cell k
@@ -128,6 +129,18 @@ void build_dg_face_records(DGPlan& plan) {
       }
     }
   };
+  // barycentric metric B_ab = |det J| grad lambda_a . grad lambda_b, a < b, and |det J|
+  plan.bgeo.assign((size_t)dgt::kBGeo * nc, 0.0);
+  for (int64_t k = 0; k < nc; ++k) {
+    const double adet = plan.geo[(size_t)9 * nc + k];
+    double gl[4][3];
+    for (int a = 0; a < 4; ++a) grad(k, a, gl[a]);
+    for (int a = 0; a < 4; ++a)
+      for (int b = a + 1; b < 4; ++b)
+        plan.bgeo[(size_t)dgt::kBPair[a][b] * nc + k] =
+            adet * (gl[a][0] * gl[b][0] + gl[a][1] * gl[b][1] + gl[a][2] * gl[b][2]);
+    plan.bgeo[(size_t)6 * nc + k] = adet;
+  }
   for (int64_t k = 0; k < nc; ++k) {
     const double adet = plan.geo[(size_t)9 * nc + k], tau = plan.geo[(size_t)10 * nc + k];
     for (int f = 0; f < 4; ++f) {
@@ -312,7 +325,7 @@ void build_dg_plan(const Mesh& mesh, int p, int ncomp, const mf_options& opt, bo
       for (int d = 0; d < 3; ++d) plan.face_tab.push_back(x[d]);
     }
   if (curved) build_dg_curved(mesh, opt, qc, qf, is_bdry, plan);
-  else build_dg_face_records(plan);
+  else build_dg_lane_data(plan);
 }
 
 namespace {
@@ -322,7 +335,7 @@ void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
   const DGPlan& pl = op.dg;
   std::vector<T> geo(pl.geo.begin(), pl.geo.end());
   if (op.opt.dg_kernel == 3) {
-    const std::vector<T> fgeo(pl.fgeo.begin(), pl.fgeo.end());
+    const std::vector<T> fgeo(pl.fgeo.begin(), pl.fgeo.end()), bgeo(pl.bgeo.begin(), pl.bgeo.end());
     // four lanes per cell (one face each), summed in lane order
     constexpr int NL = dgt::nloc(P);
     T yr[4][NL];
@@ -330,10 +343,10 @@ void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
       for (int c = 0; c < NC; ++c) {
         for (int f = 0; f < 4; ++f) {
           if (mode == 0)
-            dgt::apply_cell_lane<P, T, NC, MASS, 0>(k, f, c, pl.n_cells, geo.data(), fgeo.data(), pl.nbr.data(),
+            dgt::apply_cell_lane<P, T, NC, MASS, 0>(k, f, c, pl.n_cells, bgeo.data(), fgeo.data(), pl.nbr.data(),
                                                     pl.nface.data(), (T)pl.mass, (T)pl.laplace, src, yr[f]);
           else
-            dgt::apply_cell_lane<P, T, NC, MASS, 1>(k, f, c, pl.n_cells, geo.data(), fgeo.data(), pl.nbr.data(),
+            dgt::apply_cell_lane<P, T, NC, MASS, 1>(k, f, c, pl.n_cells, bgeo.data(), fgeo.data(), pl.nbr.data(),
                                                     pl.nface.data(), (T)pl.mass, (T)pl.laplace, src, yr[f]);
         }
         constexpr dgt::LaneInv<P> LI = dgt::make_lane_inv<P>();
