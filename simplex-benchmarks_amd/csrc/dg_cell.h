@@ -621,13 +621,42 @@ SMF_HDI LaneGeo<T> lane_geometry(int64_t k, int f, int64_t ncells, const T* __re
 // make_lane_inv inverts it) (MODE 0) or to that diagonal entry (MODE 1); the
 // four lanes' rows add up to apply_cell's result.  Storing in the lane frame
 // keeps the permutation out of the registers live across the lane's work.
+// Reference mass matrix of the degree-p Lagrange basis (exact: the cell rule
+// of degree 2p) and each node's share for the four lanes: lane f computes the
+// mass rows of its frame nodes in proportion alpha_3 / p, the node's
+// barycentric index on the frame vertex that is cell vertex f, so the four
+// lanes' shares of every row sum to one.  The mass matrix of an affine cell is
+// |det J| times the reference one, and the reference one is the same in every
+// lane frame (its entries depend only on the nodes' barycentric indices).
+template <int P>
+struct DGMassTab {
+  double m[nloc(P)][nloc(P)];
+  double share[nloc(P)];
+};
+template <int P>
+constexpr DGMassTab<P> make_mass_tab() {
+  DGMassTab<P> t{};
+  constexpr DGCellTab<P, true> CT = make_cell_tab<P, true>();
+  for (int i = 0; i < nloc(P); ++i) {
+    t.share[i] = (double)alpha(P, i).a[3] / P;
+    for (int j = 0; j < nloc(P); ++j) {
+      double a = 0;
+      for (int q = 0; q < DGCellTab<P, true>::NQ; ++q) a += CT.w[q] * CT.v[q][i] * CT.v[q][j];
+      t.m[i][j] = a;
+    }
+  }
+  return t;
+}
+
 template <int P, typename T, int NC, bool MASS, int MODE>
 SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass, T lap,
                             const T* __restrict__ src, T* __restrict__ yrow) {
-  constexpr DGCellTab<P, MASS> CT = make_cell_tab<P, MASS>();
-  constexpr DGCellShare<P, MASS> CS = make_cell_share<P, MASS>();
+  // the stiffness at the stiffness rule; a mass term through the reference
+  // mass matrix (below)
+  constexpr DGCellTab<P, false> CT = make_cell_tab<P, false>();
+  constexpr DGCellShare<P, false> CS = make_cell_share<P, false>();
   constexpr DGFaceTab<P> FT = make_face_tab<P>();
-  constexpr int NL = nloc(P), NQ = DGCellTab<P, MASS>::NQ, NQF = DGFaceTab<P>::NQF;
+  constexpr int NL = nloc(P), NQ = DGCellTab<P, false>::NQ, NQF = DGFaceTab<P>::NQF;
   constexpr int NFN = DGFaceTab<P>::NFN, NFG = DGFaceTab<P>::NFG;
   const T M0 = g.M[0], M1 = g.M[1], M2 = g.M[2], M3 = g.M[3], M4 = g.M[4], M5 = g.M[5];
   const T* nJ = g.nJ;
@@ -643,40 +672,41 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass
     u[i] = MODE == 0 ? src[base + pf[i]] : T(0);
     y[i] = T(0);
   }
-  // neighbour values issued before the cell term so that their latency
-  // overlaps it (a boundary face reads its own cell; those values are not used)
+  // neighbour values (a boundary face reads its own cell; those values are
+  // not used): up to P2 issued before the cell term so that their latency
+  // overlaps it; from P3 on after the cell term and the own normal
+  // derivatives, where three live arrays of 20 values would cost a wave per
+  // SIMD
   T ub[NL];
-  if (MODE == 0) {
+  auto load_ub = [&]() {
     const int64_t nbase = (g.kn * NC + c) * NL;
 #pragma unroll
     for (int i = 0; i < NL; ++i)
       ub[i] = src[nbase + select4<int>(g.fb, FT.perm[0][i], FT.perm[1][i], FT.perm[2][i], FT.perm[3][i])];
-  }
+  };
+  if (MODE == 0 && P <= 2) load_ub();
   // ---- the lane's share of the cell integral ----
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     if (CS.s[q] == 0.0) continue;
     const T w = T(CT.w[q] * CS.s[q]);
     if (MODE == 0) {
-      T gx = T(0), gy = T(0), gz = T(0), uv = T(0);
+      T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
         tmac(gx, CT.g[q][i][0], u[i]);
         tmac(gy, CT.g[q][i][1], u[i]);
         tmac(gz, CT.g[q][i][2], u[i]);
-        if (MASS) tmac(uv, CT.v[q][i], u[i]);
       }
       const T wl = w * lap;
       const T tx = wl * (M0 * gx + M1 * gy + M2 * gz);
       const T ty = wl * (M1 * gx + M3 * gy + M4 * gz);
       const T tz = wl * (M2 * gx + M4 * gy + M5 * gz);
-      const T tv = MASS ? w * mass * adet * uv : T(0);
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
         tmac(y[i], CT.g[q][i][0], tx);
         tmac(y[i], CT.g[q][i][1], ty);
         tmac(y[i], CT.g[q][i][2], tz);
-        if (MASS) tmac(y[i], CT.v[q][i], tv);
       }
     } else {
 #pragma unroll
@@ -684,8 +714,24 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass
         const T gx = T(CT.g[q][i][0]), gy = T(CT.g[q][i][1]), gz = T(CT.g[q][i][2]);
         y[i] += w * lap * (gx * (M0 * gx + M1 * gy + M2 * gz) + gy * (M1 * gx + M3 * gy + M4 * gz) +
                            gz * (M2 * gx + M4 * gy + M5 * gz));
-        if (MASS) y[i] += w * mass * adet * T(CT.v[q][i]) * T(CT.v[q][i]);
       }
+    }
+  }
+  if (MASS) {
+    // this lane's share of |det J| mass (M_ref u)
+    constexpr DGMassTab<P> MT = make_mass_tab<P>();
+    const T ma = mass * adet;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      if (MT.share[i] == 0.0) continue;
+      T acc = T(0);
+      if (MODE == 0) {
+#pragma unroll
+        for (int j = 0; j < NL; ++j) tmac(acc, MT.m[i][j], u[j]);
+      } else {
+        acc = T(MT.m[i][i]);
+      }
+      y[i] += (T(MT.share[i]) * ma) * acc;
     }
   }
   // ---- face f (= face 3 of the lane frame) ----
@@ -702,6 +748,16 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass
         tmac(sz, FT.dg[3][j][i][2], u[i]);
       }
       gn[j] = sx * nJ[0] + sy * nJ[1] + sz * nJ[2];
+    }
+    // own values at the face nodes (the rest of u is no longer needed)
+    T uf[NFN];
+#pragma unroll
+    for (int kk = 0; kk < NFN; ++kk) uf[kk] = u[FT.fn[3][kk]];
+    if (P > 2) {
+#ifdef __HIP_DEVICE_COMPILE__
+      __builtin_amdgcn_sched_barrier(0);  // keep the loads after the cell term and gn
+#endif
+      load_ub();
     }
     T gb[NFG];
 #pragma unroll
@@ -728,7 +784,7 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass
     constexpr DGFaceMass<P> FM = make_face_mass<P>();
     T J[NFN], A[NFG];
 #pragma unroll
-    for (int kk = 0; kk < NFN; ++kk) J[kk] = interior ? u[FT.fn[3][kk]] - ub[FT.fn[3][kk]] : T(2) * u[FT.fn[3][kk]];
+    for (int kk = 0; kk < NFN; ++kk) J[kk] = interior ? uf[kk] - ub[FT.fn[3][kk]] : T(2) * uf[kk];
 #pragma unroll
     for (int j = 0; j < NFG; ++j) A[j] = interior ? T(0.5) * (gn[j] + gb[j]) : gn[j];
 #pragma unroll
