@@ -1,5 +1,6 @@
 """Parity at the benchmark workloads' full sizes, through the C ABI:
-CG P2 16^3, DG SIPG 64^3 (P1, P2; P3 by size-independent properties),
+CG P2 16^3, DG SIPG 64^3 (P1, P2, P3; FP64 and FP32, reordering on and off;
+and size-independent properties),
 Helmholtz-like vector DG P2 32^3, MG-preconditioned CG on CG P2 48^3.  The oracle runs with OpenMP on the host (seconds to tens
 of seconds per case)."""
 import numpy as np
@@ -26,15 +27,44 @@ def test_cg_p2_16cube(gpu, orc, precision, tol):
     assert rel_err(gpu_vmult(smf, op, x), want) < tol
 
 
-@pytest.mark.parametrize("p", [1, 2])
+def _cell_order(op):
+    """internal cell -> rank of its (ascending) vertex tuple: matches the
+    cells of two operators on one mesh whatever their internal orders"""
+    cv = np.sort(op.cell_vertices(), axis=1)
+    return np.lexsort(cv.T[::-1])
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
 def test_dg_64cube_vs_oracle(gpu, orc, p):
+    """DG SIPG at 64^3 x 6 (the DG benchmark mesh) against the OpenMP oracle,
+    for FP64 and FP32 with the hierarchical reordering and FP64 without it.
+    The input is FP32-representable, so one oracle product serves both
+    precisions; the reordering-off operator numbers the same cells in another
+    order (cell-contiguous DoFs, local nodes by ascending vertex ids), so its
+    result is the oracle's with the cell blocks permuted."""
     smf = gpu
-    print(f"\ncase: dg 64^3 p={p}", flush=True)
-    op = smf.Operator(smf.Mesh.kuhn_cube(64, 0.2, 7), p, kind="dg", precision="f64")
-    x = np.random.default_rng(p).uniform(-1, 1, op.m())
-    y = gpu_vmult(smf, op, x)
+    mesh = smf.Mesh.kuhn_cube(64, 0.2, 7)
+    op = smf.Operator(mesh, p, kind="dg", precision="f64")
+    x = np.random.default_rng(p).uniform(-1, 1, op.m()).astype(np.float32).astype(np.float64)
     want = orc.DGOracleProblem.from_operator(op).vmult_omp(x)
-    assert rel_err(y, want) < 1e-12
+    print(f"\ncase: dg 64^3 p={p} f64 reorder on", flush=True)
+    assert rel_err(gpu_vmult(smf, op, x), want) < 1e-12
+    nl = op.info["dofs_per_cell"]
+    o_on = _cell_order(op)
+    del op
+    print(f"case: dg 64^3 p={p} f32 reorder on", flush=True)
+    op32 = smf.Operator(mesh, p, kind="dg", precision="f32")
+    assert rel_err(gpu_vmult(smf, op32, x), want) < 1e-5
+    del op32
+    print(f"case: dg 64^3 p={p} f64 reorder off", flush=True)
+    off = smf.Operator(mesh, p, kind="dg", precision="f64", reorder=False)
+    o_off = _cell_order(off)
+    # cell o_off[r] of `off` is cell o_on[r] of `op`
+    to_on = np.empty_like(o_off)
+    to_on[o_off] = o_on
+    xo = x.reshape(-1, nl)[to_on].ravel()
+    wo = want.reshape(-1, nl)[to_on].ravel()
+    assert rel_err(gpu_vmult(smf, off, xo), wo) < 1e-12
 
 
 def test_dg_64cube_properties(gpu):
