@@ -90,6 +90,7 @@ void upload_dg(Operator& op) {
   DGPlan& pl = op.dg;
   op.d_dg_nbr.upload(pl.nbr);
   op.d_dg_nface.upload(pl.nface);
+  if (!pl.nbrf.empty()) op.d_dg_nbrf.upload(pl.nbrf);
   if (op.precision == MF_F64) {
     op.d_dg_geo.upload(pl.geo);
     if (!pl.fgeo.empty()) op.d_dg_fgeo.upload(pl.fgeo);

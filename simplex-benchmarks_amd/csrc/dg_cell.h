@@ -548,6 +548,41 @@ struct LaneGeo {
 // values at (4 k + f) 4: one contiguous 32 B (FP64) read per lane instead of
 // eleven gathers of the neighbour's geometry arrays.
 constexpr int kFaceRec = 4;
+// Neighbour word of the four-lane kernel, [cell][local face] (one coalesced
+// 4 B read per lane): -1 on the boundary, else the neighbour's cell index in
+// the low kNbrShift bits and its local face fb above them.
+constexpr int kNbrShift = 29;
+constexpr int32_t kNbrMask = (1 << kNbrShift) - 1;
+
+// The lane-frame node permutations perm[f] packed into 32-bit words, B bits
+// per entry (4 up to 16 nodes, 5 up to 32), so that a run-time f costs one
+// select per word and one bit-field extract per node instead of a select per
+// node.
+template <int P>
+struct PermWords {
+  static constexpr int B = nloc(P) <= 16 ? 4 : 5, E = 32 / B, W = (nloc(P) + E - 1) / E;
+  uint32_t w[4][W];
+};
+template <int P>
+constexpr PermWords<P> make_perm_words() {
+  constexpr DGFaceTab<P> FT = make_face_tab<P>();
+  PermWords<P> r{};
+  for (int f = 0; f < 4; ++f)
+    for (int i = 0; i < nloc(P); ++i)
+      r.w[f][i / PermWords<P>::E] |= (uint32_t)FT.perm[f][i] << (PermWords<P>::B * (i % PermWords<P>::E));
+  return r;
+}
+// perm[f][i] for i = 0..NL-1 into out[]
+template <int P>
+SMF_HDI void lane_perm(int f, int* out) {
+  constexpr PermWords<P> PW = make_perm_words<P>();
+  constexpr int B = PermWords<P>::B, E = PermWords<P>::E, W = PermWords<P>::W;
+  uint32_t w[W];
+#pragma unroll
+  for (int t = 0; t < W; ++t) w[t] = select4<uint32_t>(f, PW.w[0][t], PW.w[1][t], PW.w[2][t], PW.w[3][t]);
+#pragma unroll
+  for (int i = 0; i < nloc(P); ++i) out[i] = (int)((w[i / E] >> (B * (i % E))) & ((1u << B) - 1));
+}
 
 // Barycentric metric of a cell, B_ab = |det J| grad lambda_a . grad lambda_b
 // (a, b = 0..3): symmetric, rows summing to zero, so the six off-diagonal
@@ -576,8 +611,7 @@ SMF_HDI T bsel(int f, const T* b) {
 // neighbour's J^-1 n and the penalty come from the face record.
 template <int P, typename T>
 SMF_HDI LaneGeo<T> lane_geometry(int64_t k, int f, int64_t ncells, const T* __restrict__ bgeo,
-                                 const T* __restrict__ fgeo, const int32_t* __restrict__ nbr,
-                                 const uint32_t* __restrict__ nface, T lap) {
+                                 const T* __restrict__ fgeo, const int32_t* __restrict__ nbrf, T lap) {
   using std::sqrt;
   LaneGeo<T> g;
   T b[6];
@@ -585,10 +619,10 @@ SMF_HDI LaneGeo<T> lane_geometry(int64_t k, int f, int64_t ncells, const T* __re
   for (int t = 0; t < 6; ++t) b[t] = bgeo[t * ncells + k];
   const T adet = bgeo[6 * ncells + k];
   g.adet = adet;
-  const int32_t kb = nbr[f * ncells + k];
+  const int32_t kb = nbrf[k * 4 + f];
   g.interior = kb >= 0;
-  g.kn = g.interior ? kb : k;
-  g.fb = g.interior ? (int)((nface[k] >> (8 * f)) & 3) : 3;
+  g.kn = g.interior ? (kb & kNbrMask) : k;
+  g.fb = (kb >> kNbrShift) & 3;  // unused on the boundary
   const T* fr = fgeo + (k * 4 + f) * kFaceRec;
   g.nb[0] = fr[0];
   g.nb[1] = fr[1];
@@ -663,7 +697,7 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass
   const T adet = g.adet, area2 = g.area2;
   const bool interior = g.interior;
   int pf[NL];
-  for (int i = 0; i < NL; ++i) pf[i] = select4<int>(f, FT.perm[0][i], FT.perm[1][i], FT.perm[2][i], FT.perm[3][i]);
+  lane_perm<P>(f, pf);
 
   const int64_t base = (k * NC + c) * NL;
   T u[NL], y[NL];
@@ -680,9 +714,10 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass
   T ub[NL];
   auto load_ub = [&]() {
     const int64_t nbase = (g.kn * NC + c) * NL;
+    int pb[NL];
+    lane_perm<P>(g.fb, pb);
 #pragma unroll
-    for (int i = 0; i < NL; ++i)
-      ub[i] = src[nbase + select4<int>(g.fb, FT.perm[0][i], FT.perm[1][i], FT.perm[2][i], FT.perm[3][i])];
+    for (int i = 0; i < NL; ++i) ub[i] = src[nbase + pb[i]];
   };
   if (MODE == 0 && P <= 2) load_ub();
   // ---- the lane's share of the cell integral ----
@@ -846,10 +881,10 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass
 // Lane f of cell k, component c (geometry and component together)
 template <int P, typename T, int NC, bool MASS, int MODE>
 SMF_HDI void apply_cell_lane(int64_t k, int f, int c, int64_t ncells, const T* __restrict__ bgeo,
-                             const T* __restrict__ fgeo, const int32_t* __restrict__ nbr,
-                             const uint32_t* __restrict__ nface, T mass, T lap, const T* __restrict__ src,
+                             const T* __restrict__ fgeo, const int32_t* __restrict__ nbrf, T mass, T lap,
+                             const T* __restrict__ src,
                              T* __restrict__ yrow) {
-  const LaneGeo<T> g = lane_geometry<P, T>(k, f, ncells, bgeo, fgeo, nbr, nface, lap);
+  const LaneGeo<T> g = lane_geometry<P, T>(k, f, ncells, bgeo, fgeo, nbrf, lap);
   lane_component<P, T, NC, MASS, MODE>(g, k, f, c, mass, lap, src, yrow);
 }
 

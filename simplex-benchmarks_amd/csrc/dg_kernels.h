@@ -456,8 +456,7 @@ constexpr int kQuadCells = 64;  // cells per 256-thread block
 template <int P, typename T, int NC, bool MASS, int MODE>
 __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells, const T* __restrict__ bgeo,
                                                                   const T* __restrict__ fgeo,
-                                                                  const int32_t* __restrict__ nbr,
-                                                                  const uint32_t* __restrict__ nface, T mass, T lap,
+                                                                  const int32_t* __restrict__ nbrf, T mass, T lap,
                                                                   const T* __restrict__ src, T* __restrict__ dst) {
   constexpr int NL = dgt::nloc(P);
   __shared__ T ybuf[kQuadCells][4][NL + 1];
@@ -472,7 +471,7 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells,
   const int64_t k = k0 + cl;
   const bool active = k < ncells;
   const int64_t kc = active ? k : ncells - 1;  // every lane reaches the barrier
-  const dgt::LaneGeo<T> g = dgt::lane_geometry<P, T>(kc, f, ncells, bgeo, fgeo, nbr, nface, lap);
+  const dgt::LaneGeo<T> g = dgt::lane_geometry<P, T>(kc, f, ncells, bgeo, fgeo, nbrf, lap);
   dgt::lane_component<P, T, NC, MASS, MODE>(g, kc, f, c, mass, lap, src, ybuf[cl][f]);
   __syncthreads();
   if (active)
@@ -491,8 +490,7 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells,
 template <int P, typename T>
 __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_ep_kernel(int64_t ncells, const T* __restrict__ bgeo,
                                                                      const T* __restrict__ fgeo,
-                                                                     const int32_t* __restrict__ nbr,
-                                                                     const uint32_t* __restrict__ nface,
+                                                                     const int32_t* __restrict__ nbrf,
                                                                      const T* __restrict__ src, RowEpilogue<T> ep) {
   constexpr int NL = dgt::nloc(P);
   __shared__ T ybuf[kQuadCells][4][NL + 1];
@@ -500,7 +498,7 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_ep_kernel(int64_t ncel
   const int64_t k = xcd_block(blockIdx.x, gridDim.x) * kQuadCells + cl;
   const bool active = k < ncells;
   const int64_t kc = active ? k : ncells - 1;
-  dgt::apply_cell_lane<P, T, 1, false, 0>(kc, f, 0, ncells, bgeo, fgeo, nbr, nface, T(0), T(1), src, ybuf[cl][f]);
+  dgt::apply_cell_lane<P, T, 1, false, 0>(kc, f, 0, ncells, bgeo, fgeo, nbrf, T(0), T(1), src, ybuf[cl][f]);
   __syncthreads();
   if (active)
 #pragma unroll
@@ -521,8 +519,8 @@ void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
     if (pl.n_cells > 0)
       hipLaunchKernelGGL((dg_quad_kernel<P, T, NC, MASS, MODE>),
                          dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells * NC)), dim3(4 * kQuadCells), 0, s,
-                         pl.n_cells, op.d_dg_bgeo.as<T>(), op.d_dg_fgeo.as<T>(), op.d_dg_nbr.as<int32_t>(),
-                         op.d_dg_nface.as<uint32_t>(), (T)pl.mass, (T)pl.laplace, src, dst);
+                         pl.n_cells, op.d_dg_bgeo.as<T>(), op.d_dg_fgeo.as<T>(), op.d_dg_nbrf.as<int32_t>(),
+                         (T)pl.mass, (T)pl.laplace, src, dst);
     SMF_HIP(hipGetLastError());
     return;
   }
@@ -565,7 +563,7 @@ void launch_dg_ep_t(const Operator& op, const T* x, RowEpilogue<T> ep, hipStream
   if (pl.n_cells > 0)
     hipLaunchKernelGGL((dg_quad_ep_kernel<P, T>), dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells)),
                        dim3(4 * kQuadCells), 0, s, pl.n_cells, op.d_dg_bgeo.as<T>(), op.d_dg_fgeo.as<T>(),
-                       op.d_dg_nbr.as<int32_t>(), op.d_dg_nface.as<uint32_t>(), x, ep);
+                       op.d_dg_nbrf.as<int32_t>(), x, ep);
   SMF_HIP(hipGetLastError());
 }
 

@@ -107,7 +107,8 @@ void build_dg_curved(const Mesh& mesh, const mf_options& opt, const Quadrature& 
   }
   if (bad) throw std::runtime_error("curved cells: the Jacobian determinant changes sign inside a cell");
 }
-// Data of the four-lane kernel: the cell's barycentric metric (dgt::kBGeo
+// Data of the four-lane kernel: the neighbour words (dgt::kNbrShift), the
+// cell's barycentric metric (dgt::kBGeo
 // values per cell, [7][cell]) and face records (dgt::kFaceRec values per cell and
 // local face, [cell][face][4]): the neighbour's barycentric gradients of the
 // reference directions of the frame of its face fb (frame vertices kFV[fb]
@@ -129,6 +130,15 @@ void build_dg_lane_data(DGPlan& plan) {
       }
     }
   };
+  // neighbour words: cell index and local face fb packed, -1 on the boundary
+  if (nc > (int64_t)dgt::kNbrMask) throw std::runtime_error("DG: too many cells for the neighbour words");
+  plan.nbrf.assign((size_t)nc * 4, -1);
+  for (int64_t k = 0; k < nc; ++k)
+    for (int f = 0; f < 4; ++f) {
+      const int32_t kb = plan.nbr[(size_t)f * nc + k];
+      if (kb >= 0)
+        plan.nbrf[(size_t)k * 4 + f] = kb | (int32_t)(((plan.nface[k] >> (8 * f)) & 3) << dgt::kNbrShift);
+    }
   // barycentric metric B_ab = |det J| grad lambda_a . grad lambda_b, a < b, and |det J|
   plan.bgeo.assign((size_t)dgt::kBGeo * nc, 0.0);
   for (int64_t k = 0; k < nc; ++k) {
@@ -336,6 +346,7 @@ void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
   std::vector<T> geo(pl.geo.begin(), pl.geo.end());
   if (op.opt.dg_kernel == 3) {
     const std::vector<T> fgeo(pl.fgeo.begin(), pl.fgeo.end()), bgeo(pl.bgeo.begin(), pl.bgeo.end());
+    const int32_t* nbrf = pl.nbrf.data();
     // four lanes per cell (one face each), summed in lane order
     constexpr int NL = dgt::nloc(P);
     T yr[4][NL];
@@ -343,11 +354,11 @@ void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
       for (int c = 0; c < NC; ++c) {
         for (int f = 0; f < 4; ++f) {
           if (mode == 0)
-            dgt::apply_cell_lane<P, T, NC, MASS, 0>(k, f, c, pl.n_cells, bgeo.data(), fgeo.data(), pl.nbr.data(),
-                                                    pl.nface.data(), (T)pl.mass, (T)pl.laplace, src, yr[f]);
+            dgt::apply_cell_lane<P, T, NC, MASS, 0>(k, f, c, pl.n_cells, bgeo.data(), fgeo.data(), nbrf,
+                                                    (T)pl.mass, (T)pl.laplace, src, yr[f]);
           else
-            dgt::apply_cell_lane<P, T, NC, MASS, 1>(k, f, c, pl.n_cells, bgeo.data(), fgeo.data(), pl.nbr.data(),
-                                                    pl.nface.data(), (T)pl.mass, (T)pl.laplace, src, yr[f]);
+            dgt::apply_cell_lane<P, T, NC, MASS, 1>(k, f, c, pl.n_cells, bgeo.data(), fgeo.data(), nbrf,
+                                                    (T)pl.mass, (T)pl.laplace, src, yr[f]);
         }
         constexpr dgt::LaneInv<P> LI = dgt::make_lane_inv<P>();
         for (int i = 0; i < NL; ++i)

@@ -139,6 +139,7 @@ struct DGPlan {
   std::vector<double> geo;          // 11 x n_cells: J^-T (row-major 9), |det J|, tau
   std::vector<double> fgeo;         // [n_cells][4][4] face records of the four-lane kernel (dg_cell.h)
   std::vector<double> bgeo;         // [7][n_cells] barycentric metric of the four-lane kernel (dg_cell.h)
+  std::vector<int32_t> nbrf;        // [n_cells][4] neighbour words of the four-lane kernel (dg_cell.h)
   std::vector<int32_t> nbr;         // 4 x n_cells: neighbour across local face f, -1 on the boundary
   std::vector<uint32_t> nface;      // per cell: neighbour's local face for f in bits 8f..8f+1
   std::vector<double> dof_points;   // 3 * n_cells * n_loc
@@ -183,7 +184,7 @@ struct Operator {
   DevBuf d_cell_start, d_own_start, d_ghost_start, d_ghost_idx, d_ccol, d_fix_ptr,
       d_pslot, d_gfix, d_lidx, d_lidx12, d_metric, d_shape, d_partial;
   DevBuf d_bnd_rows, d_bnd_ptr, d_bnd_col, d_bnd_val, d_bnd_dense, d_bcell, d_bmat;
-  DevBuf d_dg_geo, d_dg_fgeo, d_dg_bgeo, d_dg_nbr, d_dg_nface, d_dg_cell_tab, d_dg_face_tab, d_dg_jgeo;
+  DevBuf d_dg_geo, d_dg_fgeo, d_dg_bgeo, d_dg_nbrf, d_dg_nbr, d_dg_nface, d_dg_cell_tab, d_dg_face_tab, d_dg_jgeo;
   bool on_device = false;
   int64_t m() const { return is_dg() ? dg.n_dofs : cg.n_dofs * ncomp; }
   size_t wbytes() const { return precision == MF_F64 ? 8 : 4; }
