@@ -34,7 +34,8 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 METRIC = "vmult GDoF/s + %HBM peak; time to 1e-10 solve: learned vs plain hybrid MG"
 HBM_PEAK_GBS = 8000.0
 KERNEL = {"cg": "cg_wave_kernel + cg_fixup_kernel (one launch each per vmult)", "cg_vector": "cg_wave_kernel (NC=3) + cg_fixup_kernel",
-          "dg": "dg_quad_kernel (four lanes per cell, P >= 2; dg_cell_kernel_v2 for P1; one launch per vmult)",
+          "dg": "dg_quad_kernel (four lanes per cell, P >= 2, block values staged in LDS at P2; dg_cell_kernel_v2 "
+                "for P1; one launch per vmult)",
           "dg_vector": "dg_quad_kernel (NC=3)", "dg_helmholtz": "dg_quad_kernel (NC=3, mass term)"}
 WORKLOAD = {"cg": "CG Poisson", "cg_vector": "CG vector Laplace (3 components)", "dg": "DG SIPG Poisson", "dg_vector": "DG SIPG vector Laplace",
             "dg_helmholtz": "DG Helmholtz-like vector (mass 1 + Laplace 1)"}
@@ -63,7 +64,9 @@ def parse():
     ap.add_argument("--patch-cells", type=int, default=0)
     ap.add_argument("--cells-per-wave", type=int, default=0, help="CG: cells per wave of a patch workgroup")
     ap.add_argument("--accumulate", type=int, default=0, help="CG LDS accumulation: 0 auto, 1 atomics, 2 read-add-write")
-    ap.add_argument("--dg-kernel", type=int, default=0, help="DG kernel: 0 auto, 1 LDS tables, 2 compile-time tables")
+    ap.add_argument("--dg-kernel", type=int, default=0,
+                    help="DG kernel: 0 auto, 1 LDS tables, 2 compile-time tables, 3 four lanes per cell, "
+                         "4 four lanes per cell with the block's values staged in LDS")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="0 to skip the CPU leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--seed", type=int, default=7)

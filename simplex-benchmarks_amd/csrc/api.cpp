@@ -311,7 +311,7 @@ int mf_create(const mf_mesh_desc* desc, int32_t degree, int32_t kind, int32_t pr
     if (degree < 1 || degree > 3) throw std::runtime_error("degree must be 1, 2 or 3");
     if (opt && (opt->accumulate < 0 || opt->accumulate > 3)) throw std::runtime_error("accumulate must be 0 to 3");
     if (opt && (opt->boundary < 0 || opt->boundary > 1)) throw std::runtime_error("boundary must be 0 or 1");
-    if (opt && (opt->dg_kernel < 0 || opt->dg_kernel > 3)) throw std::runtime_error("dg_kernel must be 0..3");
+    if (opt && (opt->dg_kernel < 0 || opt->dg_kernel > 4)) throw std::runtime_error("dg_kernel must be 0..4");
     if (opt && (opt->penalty_kind < 0 || opt->penalty_kind > 1)) throw std::runtime_error("penalty_kind must be 0 or 1");
     if (opt && (opt->manifold < 0 || opt->manifold > 2)) throw std::runtime_error("manifold must be 0, 1 or 2");
     if (opt && opt->manifold != 0 && opt->boundary != 0)

@@ -682,9 +682,13 @@ constexpr DGMassTab<P> make_mass_tab() {
   return t;
 }
 
-template <int P, typename T, int NC, bool MASS, int MODE>
+// ST: the values of the workgroup's cells (component c) are staged in LDS,
+// sblk[(k - k0) NL + i] for k0 <= k < k0 + ncb; the own values and the
+// neighbour's, when the neighbour is one of those cells, are read there.
+template <int P, typename T, int NC, bool MASS, int MODE, bool ST = false>
 SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass, T lap,
-                            const T* __restrict__ src, T* __restrict__ yrow) {
+                            const T* __restrict__ src, T* __restrict__ yrow, const T* sblk = nullptr,
+                            int64_t k0 = 0, int ncb = 0) {
   // the stiffness at the stiffness rule; a mass term through the reference
   // mass matrix (below)
   constexpr DGCellTab<P, false> CT = make_cell_tab<P, false>();
@@ -703,7 +707,7 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass
   T u[NL], y[NL];
 #pragma unroll
   for (int i = 0; i < NL; ++i) {
-    u[i] = MODE == 0 ? src[base + pf[i]] : T(0);
+    u[i] = MODE == 0 ? (ST ? sblk[(k - k0) * NL + pf[i]] : src[base + pf[i]]) : T(0);
     y[i] = T(0);
   }
   // neighbour values (a boundary face reads its own cell; those values are
@@ -716,8 +720,14 @@ SMF_HDI void lane_component(const LaneGeo<T>& g, int64_t k, int f, int c, T mass
     const int64_t nbase = (g.kn * NC + c) * NL;
     int pb[NL];
     lane_perm<P>(g.fb, pb);
+    if (ST && (uint64_t)(g.kn - k0) < (uint64_t)ncb) {
+      const T* row = sblk + (g.kn - k0) * NL;
 #pragma unroll
-    for (int i = 0; i < NL; ++i) ub[i] = src[nbase + pb[i]];
+      for (int i = 0; i < NL; ++i) ub[i] = row[pb[i]];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NL; ++i) ub[i] = src[nbase + pb[i]];
+    }
   };
   if (MODE == 0 && P <= 2) load_ub();
   // ---- the lane's share of the cell integral ----

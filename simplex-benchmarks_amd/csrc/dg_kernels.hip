@@ -31,7 +31,8 @@ void launch_dg_prec(const Operator& op, int mode, T* dst, const T* src, hipStrea
 }  // namespace
 
 bool dg_has_epilogue(const Operator& op) {
-  return op.kind == MF_DG_SIPG && !op.dg.curved && (op.opt.dg_kernel == 3 || (op.opt.dg_kernel == 0 && op.dg.p >= 2));
+  return op.kind == MF_DG_SIPG && !op.dg.curved &&
+         (op.opt.dg_kernel == 3 || op.opt.dg_kernel == 4 || (op.opt.dg_kernel == 0 && op.dg.p >= 2));
 }
 
 template <typename T>

@@ -344,7 +344,7 @@ template <int P, typename T, int NC, bool MASS>
 void host_dg_run(const Operator& op, int mode, T* dst, const T* src) {
   const DGPlan& pl = op.dg;
   std::vector<T> geo(pl.geo.begin(), pl.geo.end());
-  if (op.opt.dg_kernel == 3) {
+  if (op.opt.dg_kernel == 3 || op.opt.dg_kernel == 4) {
     const std::vector<T> fgeo(pl.fgeo.begin(), pl.fgeo.end()), bgeo(pl.bgeo.begin(), pl.bgeo.end());
     const int32_t* nbrf = pl.nbrf.data();
     // four lanes per cell (one face each), summed in lane order

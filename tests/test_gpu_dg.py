@@ -43,7 +43,7 @@ def test_dg_reorder_off_and_determinism(gpu, orc, p):
     assert rel_err(a.numpy(), pr.vmult(x)) < 1e-12
 
 
-@pytest.mark.parametrize("dg_kernel", [1, 2, 3])
+@pytest.mark.parametrize("dg_kernel", [1, 2, 3, 4])
 @pytest.mark.parametrize("p", [1, 2, 3])
 def test_dg_kernel_variants(gpu, orc, p, dg_kernel):
     smf = gpu

@@ -9,7 +9,7 @@ from conftest import rel_err
 TOL = {"f64": 1e-13, "f32": 3e-6}
 
 
-@pytest.mark.parametrize("dg_kernel", [2, 3])
+@pytest.mark.parametrize("dg_kernel", [2, 3, 4])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 @pytest.mark.parametrize("kind", ["dg", "dg_vector", "dg_helmholtz"])
 @pytest.mark.parametrize("p", [1, 2, 3])
@@ -33,7 +33,7 @@ def test_dg_sizes(smf):
     assert op.info["n_constrained"] == 0
 
 
-@pytest.mark.parametrize("dg_kernel", [2, 3])
+@pytest.mark.parametrize("dg_kernel", [2, 3, 4])
 @pytest.mark.parametrize("p", [1, 2, 3])
 def test_host_dg_penalty_variant(smf, orc, p, dg_kernel):
     """penalty_kind 1: sigma = (p+1)(p+2)/h_F with h_F = |K|/|F| per side (the
