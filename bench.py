@@ -34,8 +34,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 METRIC = "vmult GDoF/s + %HBM peak; time to 1e-10 solve: learned vs plain hybrid MG"
 HBM_PEAK_GBS = 8000.0
 KERNEL = {"cg": "cg_wave_kernel + cg_fixup_kernel (one launch each per vmult)", "cg_vector": "cg_wave_kernel (NC=3) + cg_fixup_kernel",
-          "dg": "dg_quad_kernel (four lanes per cell, P >= 2, block values staged in LDS at P2; dg_cell_kernel_v2 "
-                "for P1; one launch per vmult)",
+          "dg": "dg_quad_kernel (four lanes per cell, block values staged in LDS; one launch per vmult)",
           "dg_vector": "dg_quad_kernel (NC=3)", "dg_helmholtz": "dg_quad_kernel (NC=3, mass term)"}
 WORKLOAD = {"cg": "CG Poisson", "cg_vector": "CG vector Laplace (3 components)", "dg": "DG SIPG Poisson", "dg_vector": "DG SIPG vector Laplace",
             "dg_helmholtz": "DG Helmholtz-like vector (mass 1 + Laplace 1)"}

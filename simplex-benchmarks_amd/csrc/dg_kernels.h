@@ -455,11 +455,13 @@ __global__ __launch_bounds__(kThreads) void dg_cell_kernel_v2(int64_t ncells, co
 constexpr int kQuadCells = 64;  // cells per 256-thread block
 // Stage the block's own values in LDS (coalesced loads; the neighbour's
 // values too when it is one of the block's cells: 69% of the interior faces
-// in the hierarchical order).  Measured at 64^3: P2 FP64 0.190 -> 0.177 ms,
-// FP32 0.108 -> 0.099 ms, Helmholtz P2 32^3 0.068 -> 0.066 ms; P3 FP64
-// 0.380 -> 0.395 ms (the extra 10 KB of LDS per block), so not for P3.
-template <int P>
-constexpr bool kQuadStage = P == 2;
+// in the hierarchical order).  Measured at 64^3 (same run, staged against
+// not): P2 FP64 0.177 / 0.190 ms, P2 FP32 0.099 / 0.108, Helmholtz P2 32^3
+// 0.066 / 0.068, P1 FP64 0.098 / 0.100, P3 FP32 0.227 / 0.247; slower for
+// P3 FP64 (0.395 / 0.380, the extra 10 KB of LDS per block) and P1 FP32
+// (0.046 / 0.044).
+template <int P, typename T>
+constexpr bool kQuadStage = P == 2 || (P == 1 && sizeof(T) == 8) || (P == 3 && sizeof(T) == 4);
 template <int P, typename T, int NC, bool MASS, int MODE, bool ST = false>
 __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells, const T* __restrict__ bgeo,
                                                                   const T* __restrict__ fgeo,
@@ -514,7 +516,7 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_ep_kernel(int64_t ncel
   const int64_t k = k0 + cl;
   const bool active = k < ncells;
   const int64_t kc = active ? k : ncells - 1;
-  constexpr bool ST = kQuadStage<P>;
+  constexpr bool ST = kQuadStage<P, T>;
   __shared__ T sblk[ST ? kQuadCells * NL : 1];
   const int ncb = (int)(ncells - k0 < kQuadCells ? ncells - k0 : kQuadCells);
   if (ST) {
@@ -538,8 +540,10 @@ template <int P, typename T, int NC, bool MASS, int MODE>
 void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
   const DGPlan& pl = op.dg;
   if (pl.curved) return launch_dg_curved<P, T, NC, MASS, MODE>(op, dst, src, s);
-  // auto: four lanes per cell from P2 up (registers, not bandwidth, limit those)
-  if (MODE == 0 && (op.opt.dg_kernel == 4 || (op.opt.dg_kernel == 0 && kQuadStage<P>))) {
+  // auto: four lanes per cell (the products at every degree, the diagonal
+  // from P2 up; at P1 64^3 FP64 0.098-0.100 against 0.115 ms for one lane per
+  // cell, FP32 0.044 against 0.072 ms)
+  if (MODE == 0 && (op.opt.dg_kernel == 4 || (op.opt.dg_kernel == 0 && kQuadStage<P, T>))) {
     if (pl.n_cells > 0)
       hipLaunchKernelGGL((dg_quad_kernel<P, T, NC, MASS, MODE, true>),
                          dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells * NC)), dim3(4 * kQuadCells), 0, s,
@@ -548,7 +552,7 @@ void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
     SMF_HIP(hipGetLastError());
     return;
   }
-  if (op.opt.dg_kernel == 3 || op.opt.dg_kernel == 4 || (op.opt.dg_kernel == 0 && P >= 2)) {
+  if (op.opt.dg_kernel == 3 || op.opt.dg_kernel == 4 || (op.opt.dg_kernel == 0 && (P >= 2 || MODE == 0))) {
     if (pl.n_cells > 0)
       hipLaunchKernelGGL((dg_quad_kernel<P, T, NC, MASS, MODE>),
                          dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells * NC)), dim3(4 * kQuadCells), 0, s,
