@@ -58,7 +58,9 @@ typedef struct mf_options {
                                colour, 3 LDS atomics per wave (wave order fixed per round) */
   int32_t dg_kernel;        /* DG kernel: 0 auto, 1 LDS tables, 2 compile-time tables + neighbour in face-3 frame,
                                3 four lanes per cell (one face each), 4 the same with the workgroup's cell
-                               values staged in LDS (the default for P2) */
+                               values staged in LDS. Auto: four lanes per cell for the product at every
+                               degree, staged for P2 and for P1 FP64 and P3 FP32 (measured faster there);
+                               the one-lane form for the P1 diagonal */
   int32_t boundary;         /* CG Poisson: 0 Dirichlet rows = identity (default); 1 Nitsche: no constrained
                                DoFs, boundary faces add int(2 sigma u v - du/dn v - u dv/dn) with
                                sigma = (p+1)(p+3)/3 tau_K (the CG level under a DG operator in the hybrid
