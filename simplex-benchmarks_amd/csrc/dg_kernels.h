@@ -460,6 +460,33 @@ constexpr int kQuadCells = 64;  // cells per 256-thread block
 // 0.066 / 0.068, P1 FP64 0.098 / 0.100, P3 FP32 0.227 / 0.247; slower for
 // P3 FP64 (0.395 / 0.380, the extra 10 KB of LDS per block) and P1 FP32
 // (0.046 / 0.044).
+// The four lanes' rows of each cell (ybuf[cell][lane][slot], written by
+// lane_component) summed in lane order, then written back into the same
+// buffer as the block's result rows, cell after cell ([cell][node], the
+// order of dst), so that the caller's stores are contiguous runs instead of
+// 32 B pieces 8 NL B apart.  Every thread of the block calls it (barriers).
+// Same runs at 64^3 against the direct stores: DG SIPG P2 FP64 0.1748 /
+// 0.1771 ms; DG P2 MG-CG at 48^3 (the FP32 epilogue kernel) 15.4 / 15.7 ms.
+template <int P, typename T, int W>
+__device__ inline T* quad_rows_to_block(T (&ybuf)[kQuadCells][4][W], int cl, int f) {
+  constexpr int NL = dgt::nloc(P);
+  static_assert(W >= NL, "row buffer too short");
+  __syncthreads();
+  T sum[(NL + 3) / 4];
+#pragma unroll
+  for (int t = 0; t < (NL + 3) / 4; ++t)
+    sum[t] = f + 4 * t < NL ? ((ybuf[cl][0][dgt::lane_slot4<P>(0, t, f)] + ybuf[cl][1][dgt::lane_slot4<P>(1, t, f)]) +
+                               ybuf[cl][2][dgt::lane_slot4<P>(2, t, f)]) +
+                                  ybuf[cl][3][dgt::lane_slot4<P>(3, t, f)]
+                            : T(0);
+  __syncthreads();
+  T* const yblk = &ybuf[0][0][0];
+#pragma unroll
+  for (int t = 0; t < (NL + 3) / 4; ++t)
+    if (f + 4 * t < NL) yblk[cl * NL + f + 4 * t] = sum[t];
+  __syncthreads();
+  return yblk;
+}
 template <int P, typename T>
 constexpr bool kQuadStage = P == 2 || (P == 1 && sizeof(T) == 8) || (P == 3 && sizeof(T) == 4);
 template <int P, typename T, int NC, bool MASS, int MODE, bool ST = false>
@@ -490,6 +517,14 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells,
   }
   const dgt::LaneGeo<T> g = dgt::lane_geometry<P, T>(kc, f, ncells, bgeo, fgeo, nbrf, lap);
   dgt::lane_component<P, T, NC, MASS, MODE, ST>(g, kc, f, c, mass, lap, src, ybuf[cl][f], sblk, k0, ncb);
+  if (sizeof(T) == 8) {
+    T* const yblk = quad_rows_to_block<P>(ybuf, cl, f);
+    for (int j = threadIdx.x; j < ncb * NL; j += 4 * kQuadCells)
+      dst[NC == 1 ? k0 * NL + j : ((k0 + j / NL) * NC + c) * NL + j % NL] = yblk[j];
+    return;
+  }
+  // FP32: the rows stored straight from the lane sums (the block-ordered
+  // store measured slower in FP32: Helmholtz P2 32^3 0.0383 against 0.0360 ms)
   __syncthreads();
   if (active)
 #pragma unroll
@@ -525,15 +560,8 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_ep_kernel(int64_t ncel
   }
   const dgt::LaneGeo<T> g = dgt::lane_geometry<P, T>(kc, f, ncells, bgeo, fgeo, nbrf, T(1));
   dgt::lane_component<P, T, 1, false, 0, ST>(g, kc, f, 0, T(0), T(1), src, ybuf[cl][f], sblk, k0, ncb);
-  __syncthreads();
-  if (active)
-#pragma unroll
-    for (int t = 0; t < (NL + 3) / 4; ++t)
-      if (f + 4 * t < NL)
-        apply_epilogue(ep, k * NL + f + 4 * t,
-                       ((ybuf[cl][0][dgt::lane_slot4<P>(0, t, f)] + ybuf[cl][1][dgt::lane_slot4<P>(1, t, f)]) +
-                        ybuf[cl][2][dgt::lane_slot4<P>(2, t, f)]) +
-                           ybuf[cl][3][dgt::lane_slot4<P>(3, t, f)]);
+  T* const yblk = quad_rows_to_block<P>(ybuf, cl, f);
+  for (int j = threadIdx.x; j < ncb * NL; j += 4 * kQuadCells) apply_epilogue(ep, k0 * NL + j, yblk[j]);
 }
 
 template <int P, typename T, int NC, bool MASS, int MODE>
