@@ -587,14 +587,12 @@ int mf_mg_transfer(mf_mg_t mg, int32_t level, int32_t direction, void* dst, cons
     auto& L = mg->mg->levels;
     if (level < 1 || level >= (int)L.size()) throw std::runtime_error("level out of range");
     if (direction != 0 && direction != 1) throw std::runtime_error("direction must be 0 or 1");
-    const Csr& A = direction == 0 ? L[level].P : L[level].R;
     const hipStream_t s = static_cast<hipStream_t>(stream);
     if (mg->mg->opt.level_precision == MF_F64)
-      blas_csr<double>(A.nrows, A.d_ptr.as<int64_t>(), A.d_col.as<int32_t>(), A.d_val.as<double>(),
-                       static_cast<const double*>(src), static_cast<double*>(dst), false, s);
+      level_transfer<double>(L[level], direction, static_cast<double*>(dst), static_cast<const double*>(src), false,
+                             s);
     else
-      blas_csr<float>(A.nrows, A.d_ptr.as<int64_t>(), A.d_col.as<int32_t>(), A.d_val.as<float>(),
-                      static_cast<const float*>(src), static_cast<float*>(dst), false, s);
+      level_transfer<float>(L[level], direction, static_cast<float*>(dst), static_cast<const float*>(src), false, s);
   });
 }
 

@@ -189,6 +189,18 @@ bool make_row_blocks(SysMatrix& A) {
   return true;
 }
 
+bool upload_blocked(const std::vector<int64_t>& ptr, const std::vector<int32_t>& col, const std::vector<double>& val,
+                    int precision, SysMatrix& A) {
+  A.precision = precision;
+  A.n_rows = ptr.empty() ? 0 : (int64_t)ptr.size() - 1;
+  A.ptr = ptr;
+  A.col = col;
+  A.val = val;
+  if (A.n_rows == 0 || !make_row_blocks(A)) return false;
+  upload_matrix(A);
+  return true;
+}
+
 void upload_matrix(SysMatrix& A) {
   A.d_ptr.upload(A.ptr);
   A.d_col.upload(A.col);

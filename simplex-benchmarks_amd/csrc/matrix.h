@@ -36,7 +36,11 @@ void upload_matrix(SysMatrix& A);
 // row blocks of the product kernel from A.ptr; false (and no blocks) when a
 // row has more than kSpmvBlockEntries entries
 bool make_row_blocks(SysMatrix& A);
-// dst = A src (device vectors in A.precision)
-void launch_spmv(const SysMatrix& A, void* dst, const void* src, hipStream_t s);
+// dst = A src, or dst += A src with add (device vectors in A.precision)
+void launch_spmv(const SysMatrix& A, void* dst, const void* src, hipStream_t s, bool add = false);
+// A in row blocks on the device (precision: MF_F64 or MF_F32); false when a
+// row has more than kSpmvBlockEntries entries (then A is not on the device)
+bool upload_blocked(const std::vector<int64_t>& ptr, const std::vector<int32_t>& col, const std::vector<double>& val,
+                    int precision, SysMatrix& A);
 
 }  // namespace smf

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "blas.h"
+#include "matrix.h"
 #include "op.h"
 
 namespace smf {
@@ -32,8 +33,15 @@ struct MGLevel {
   DevBuf inv_diag, x, b, r, t, d;  // level vectors
   DevBuf run_part[2], run_xi[2], run_di[2];  // ChebyRun buffers (scalar CG levels)
   Csr P, R;                        // prolongation from / restriction to the next coarser level
+  // the same in row blocks for the CSR-stream product (coalesced entry loads;
+  // rows of the restriction hold up to ~30 entries), when every row fits a block
+  SysMatrix Pm, Rm;
   double lambda_max = 0, lambda_min = 0;
 };
+
+// dst = P src (dir 0) or R src (dir 1) of a level, dst += with add
+template <typename T>
+void level_transfer(const MGLevel& L, int dir, T* dst, const T* src, bool add, hipStream_t s);
 
 struct MGOptions {
   int smoother_degree = 5;

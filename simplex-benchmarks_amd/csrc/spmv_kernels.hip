@@ -29,7 +29,8 @@ constexpr int kPerLane = kSpmvBlockEntries / kSpmvThreads;
 static_assert(kPerLane * kSpmvThreads == kSpmvBlockEntries, "row-block entries per lane");
 static_assert(kSpmvBlockRows <= kSpmvThreads, "one lane per row of a block");
 
-template <typename T>
+// ADD: y += A x (the prolongation of a multigrid level into its iterate)
+template <typename T, bool ADD>
 __global__ __launch_bounds__(kSpmvThreads) void spmv_stream_kernel(int64_t nblocks, const int32_t* __restrict__ blk,
                                                                    const int64_t* __restrict__ ptr,
                                                                    const int32_t* __restrict__ col,
@@ -73,24 +74,28 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_stream_kernel(int64_t nbloc
     for (int64_t k = k0 - e0 + sub; k < k1 - e0; k += G) s += prod[k];
   // every lane takes part (G is uniform over the workgroup)
   for (int m = 1; m < G; m <<= 1) s += __shfl_xor(s, m);
-  if (has_row && sub == 0) y[rr] = s;
+  if (has_row && sub == 0) y[rr] = ADD ? y[rr] + s : s;
 }
 
 }  // namespace
 
-void launch_spmv(const SysMatrix& A, void* dst, const void* src, hipStream_t s) {
+template <typename T>
+void launch_spmv_t(const SysMatrix& A, T* dst, const T* src, bool add, hipStream_t s) {
+  const int64_t nb = A.n_blocks();
+  auto kern = add ? spmv_stream_kernel<T, true> : spmv_stream_kernel<T, false>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(kSpmvThreads), 0, s, nb, A.d_blk.as<int32_t>(),
+                     A.d_ptr.as<int64_t>(), A.d_col.as<int32_t>(), A.d_val.as<T>(), src, dst);
+}
+
+void launch_spmv(const SysMatrix& A, void* dst, const void* src, hipStream_t s, bool add) {
   if (!A.on_device) throw std::runtime_error("system matrix has no device data (host_only operator)");
   const int64_t nb = A.n_blocks();
   if (nb == 0) return;
   if (nb >= (int64_t(1) << 31)) throw std::runtime_error("system matrix: too many row blocks");
   if (A.precision == MF_F64)
-    hipLaunchKernelGGL(spmv_stream_kernel<double>, dim3((unsigned)nb), dim3(kSpmvThreads), 0, s, nb,
-                       A.d_blk.as<int32_t>(), A.d_ptr.as<int64_t>(), A.d_col.as<int32_t>(), A.d_val.as<double>(),
-                       static_cast<const double*>(src), static_cast<double*>(dst));
+    launch_spmv_t<double>(A, static_cast<double*>(dst), static_cast<const double*>(src), add, s);
   else
-    hipLaunchKernelGGL(spmv_stream_kernel<float>, dim3((unsigned)nb), dim3(kSpmvThreads), 0, s, nb,
-                       A.d_blk.as<int32_t>(), A.d_ptr.as<int64_t>(), A.d_col.as<int32_t>(), A.d_val.as<float>(),
-                       static_cast<const float*>(src), static_cast<float*>(dst));
+    launch_spmv_t<float>(A, static_cast<float*>(dst), static_cast<const float*>(src), add, s);
   SMF_HIP(hipGetLastError());
 }
 
