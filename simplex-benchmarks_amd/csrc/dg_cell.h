@@ -609,25 +609,32 @@ SMF_HDI T bsel(int f, const T* b) {
 // lambda_f|: J^-1 n in the frame is -B[., f] / s and the face Jacobian
 // factor |det J| |grad lambda_f| is s, with s = sqrt(|det J| B_ff).  The
 // neighbour's J^-1 n and the penalty come from the face record.
-template <int P, typename T>
+// NT: non-temporal loads (data read once per product, so that the caches
+// keep the gathered cell values instead)
+template <bool NT, typename T>
+SMF_HDI T load_once(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <int P, typename T, bool NT = false>
 SMF_HDI LaneGeo<T> lane_geometry(int64_t k, int f, int64_t ncells, const T* __restrict__ bgeo,
                                  const T* __restrict__ fgeo, const int32_t* __restrict__ nbrf, T lap) {
   using std::sqrt;
   LaneGeo<T> g;
   T b[6];
 #pragma unroll
-  for (int t = 0; t < 6; ++t) b[t] = bgeo[t * ncells + k];
-  const T adet = bgeo[6 * ncells + k];
+  for (int t = 0; t < 6; ++t) b[t] = load_once<NT>(bgeo + t * ncells + k);
+  const T adet = load_once<NT>(bgeo + 6 * ncells + k);
   g.adet = adet;
-  const int32_t kb = nbrf[k * 4 + f];
+  const int32_t kb = load_once<NT>(nbrf + k * 4 + f);
   g.interior = kb >= 0;
   g.kn = g.interior ? (kb & kNbrMask) : k;
   g.fb = (kb >> kNbrShift) & 3;  // unused on the boundary
   const T* fr = fgeo + (k * 4 + f) * kFaceRec;
-  g.nb[0] = fr[0];
-  g.nb[1] = fr[1];
-  g.nb[2] = fr[2];
-  g.sigma = fr[3];
+  g.nb[0] = load_once<NT>(fr);
+  g.nb[1] = load_once<NT>(fr + 1);
+  g.nb[2] = load_once<NT>(fr + 2);
+  g.sigma = load_once<NT>(fr + 3);
   // B[x][y] for the frame roles x, y of lane f: one select over the four faces
   const T bab = bsel<1, 2>(f, b), baf = bsel<1, 3>(f, b), bbf = bsel<2, 3>(f, b);
   const T bao = bsel<1, 0>(f, b), bbo = bsel<2, 0>(f, b), bfo = bsel<3, 0>(f, b);

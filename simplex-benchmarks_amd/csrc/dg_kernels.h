@@ -489,7 +489,7 @@ __device__ inline T* quad_rows_to_block(T (&ybuf)[kQuadCells][4][W], int cl, int
 }
 template <int P, typename T>
 constexpr bool kQuadStage = P == 2 || (P == 1 && sizeof(T) == 8) || (P == 3 && sizeof(T) == 4);
-template <int P, typename T, int NC, bool MASS, int MODE, bool ST = false>
+template <int P, typename T, int NC, bool MASS, int MODE, bool ST = false, bool NT = false>
 __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells, const T* __restrict__ bgeo,
                                                                   const T* __restrict__ fgeo,
                                                                   const int32_t* __restrict__ nbrf, T mass, T lap,
@@ -515,12 +515,19 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_kernel(int64_t ncells,
       sblk[j] = src[((k0 + j / NL) * NC + c) * NL + j % NL];
     __syncthreads();
   }
-  const dgt::LaneGeo<T> g = dgt::lane_geometry<P, T>(kc, f, ncells, bgeo, fgeo, nbrf, lap);
+  // NT (scalar FP64 products whose data exceeds the Infinity Cache, chosen
+  // at launch): the geometry, face records and neighbour words are loaded and
+  // dst stored non-temporally, as they are touched once per product, and the
+  // caches are left to the gathered neighbour values
+  static_assert(!NT || (NC == 1 && sizeof(T) == 8), "non-temporal variant: scalar FP64 products");
+  const dgt::LaneGeo<T> g = dgt::lane_geometry<P, T, NT>(kc, f, ncells, bgeo, fgeo, nbrf, lap);
   dgt::lane_component<P, T, NC, MASS, MODE, ST>(g, kc, f, c, mass, lap, src, ybuf[cl][f], sblk, k0, ncb);
   if (sizeof(T) == 8) {
     T* const yblk = quad_rows_to_block<P>(ybuf, cl, f);
-    for (int j = threadIdx.x; j < ncb * NL; j += 4 * kQuadCells)
-      dst[NC == 1 ? k0 * NL + j : ((k0 + j / NL) * NC + c) * NL + j % NL] = yblk[j];
+    for (int j = threadIdx.x; j < ncb * NL; j += 4 * kQuadCells) {
+      if (NT) __builtin_nontemporal_store(yblk[j], dst + k0 * NL + j);
+      else dst[NC == 1 ? k0 * NL + j : ((k0 + j / NL) * NC + c) * NL + j % NL] = yblk[j];
+    }
     return;
   }
   // FP32: the rows stored straight from the lane sums (the block-ordered
@@ -564,6 +571,33 @@ __global__ __launch_bounds__(4 * kQuadCells) void dg_quad_ep_kernel(int64_t ncel
   for (int j = threadIdx.x; j < ncb * NL; j += 4 * kQuadCells) apply_epilogue(ep, k0 * NL + j, yblk[j]);
 }
 
+// The non-temporal variant of the four-lane kernel for scalar FP64 products
+// whose geometry, face records, neighbour words, src and dst exceed the
+// 256 MB Infinity Cache (at 64^3 P2 566 MB; at 48^3 239 MB, whose data the
+// cache keeps between the products of a solve).  Same runs at 64^3 against
+// normal loads and stores: P1 0.0890 / 0.0900 against 0.0928 / 0.0949 ms, P2
+// 0.1684 / 0.1702 against 0.1738 / 0.1749 ms, P3 0.394 / 0.390 against
+// 0.408 / 0.406 ms.  FP32 products keep normal loads: P1 FP32 at 64^3 (239
+// MB) stays in the cache from one product to the next and measured 0.051
+// against 0.039 ms with the hints.
+template <int P, typename T, int NC, bool MASS, int MODE, bool ST>
+void launch_quad(const Operator& op, T* dst, const T* src, hipStream_t s) {
+  const DGPlan& pl = op.dg;
+  const dim3 grid((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells * NC)), block(4 * kQuadCells);
+  if constexpr (NC == 1 && sizeof(T) == 8 && MODE == 0) {
+    const double bytes = (double)pl.n_cells * (7 * sizeof(T) + 4 * (dgt::kFaceRec * sizeof(T) + 4)) +
+                         2.0 * sizeof(T) * op.m();
+    if (bytes > 256.0 * (1 << 20)) {
+      hipLaunchKernelGGL((dg_quad_kernel<P, T, NC, MASS, MODE, ST, true>), grid, block, 0, s, pl.n_cells,
+                         op.d_dg_bgeo.as<T>(), op.d_dg_fgeo.as<T>(), op.d_dg_nbrf.as<int32_t>(), (T)pl.mass,
+                         (T)pl.laplace, src, dst);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((dg_quad_kernel<P, T, NC, MASS, MODE, ST>), grid, block, 0, s, pl.n_cells, op.d_dg_bgeo.as<T>(),
+                     op.d_dg_fgeo.as<T>(), op.d_dg_nbrf.as<int32_t>(), (T)pl.mass, (T)pl.laplace, src, dst);
+}
+
 template <int P, typename T, int NC, bool MASS, int MODE>
 void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
   const DGPlan& pl = op.dg;
@@ -572,20 +606,12 @@ void launch_dg_t(const Operator& op, T* dst, const T* src, hipStream_t s) {
   // from P2 up; at P1 64^3 FP64 0.098-0.100 against 0.115 ms for one lane per
   // cell, FP32 0.044 against 0.072 ms)
   if (MODE == 0 && (op.opt.dg_kernel == 4 || (op.opt.dg_kernel == 0 && kQuadStage<P, T>))) {
-    if (pl.n_cells > 0)
-      hipLaunchKernelGGL((dg_quad_kernel<P, T, NC, MASS, MODE, true>),
-                         dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells * NC)), dim3(4 * kQuadCells), 0, s,
-                         pl.n_cells, op.d_dg_bgeo.as<T>(), op.d_dg_fgeo.as<T>(), op.d_dg_nbrf.as<int32_t>(),
-                         (T)pl.mass, (T)pl.laplace, src, dst);
+    if (pl.n_cells > 0) launch_quad<P, T, NC, MASS, MODE, true>(op, dst, src, s);
     SMF_HIP(hipGetLastError());
     return;
   }
   if (op.opt.dg_kernel == 3 || op.opt.dg_kernel == 4 || (op.opt.dg_kernel == 0 && (P >= 2 || MODE == 0))) {
-    if (pl.n_cells > 0)
-      hipLaunchKernelGGL((dg_quad_kernel<P, T, NC, MASS, MODE>),
-                         dim3((unsigned)((pl.n_cells + kQuadCells - 1) / kQuadCells * NC)), dim3(4 * kQuadCells), 0, s,
-                         pl.n_cells, op.d_dg_bgeo.as<T>(), op.d_dg_fgeo.as<T>(), op.d_dg_nbrf.as<int32_t>(),
-                         (T)pl.mass, (T)pl.laplace, src, dst);
+    if (pl.n_cells > 0) launch_quad<P, T, NC, MASS, MODE, false>(op, dst, src, s);
     SMF_HIP(hipGetLastError());
     return;
   }
