@@ -163,7 +163,9 @@ __device__ __forceinline__ T sum_partials(const T* __restrict__ part, int32_t k0
 // the geometry of a cell; component c of scalar DoF d is row c*cstride + d.
 // FEAT: bit 0 = pending interface staging (ChebyRun steps), bit 1 = per-cell
 // Nitsche boundary terms; both compile out of the plain operator kernels
-// (they cost registers: P2 FP32 64 -> 120 VGPRs with both)
+// (they cost registers: P2 FP32 64 -> 120 VGPRs with both).  Bit 16:
+// the packed indices and the metric, read once per product, are loaded
+// non-temporally (FP64 products whose data exceeds the Infinity Cache)
 template <int P, typename T, int MODE, int ACC, int NC, int FEAT>
 __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ own_start,
@@ -278,14 +280,20 @@ __global__ __launch_bounds__(kMaxThreads) void cg_wave_kernel(
     const int64_t cell = c0 + min(r0 + lane, nc - 1);
     if constexpr ((FEAT & 8) != 0) {
 #pragma unroll
-      for (int w = 0; w < NW; ++w) lw_n[w] = lidx12[(int64_t)w * ncells_total + cell];
+      for (int w = 0; w < NW; ++w) {
+        if constexpr ((FEAT & 16) != 0) lw_n[w] = __builtin_nontemporal_load(lidx12 + (int64_t)w * ncells_total + cell);
+        else lw_n[w] = lidx12[(int64_t)w * ncells_total + cell];
+      }
     } else {
 #pragma unroll
       for (int i = 0; i < NL; ++i) li_n[i] = lidx[(int64_t)i * ncells_total + cell];
     }
     if constexpr ((FEAT & 4) == 0) {
 #pragma unroll
-      for (int t = 0; t < 6; ++t) m_n[t] = metric[t * ncells_total + cell];
+      for (int t = 0; t < 6; ++t) {
+        if constexpr ((FEAT & 16) != 0) m_n[t] = __builtin_nontemporal_load(metric + t * ncells_total + cell);
+        else m_n[t] = metric[t * ncells_total + cell];
+      }
     }
     if constexpr ((FEAT & 8) == 0) col_n = ccol[cell];  // packed: decoded at the round start
     if ((FEAT & 2) && bcell) bc_n = bcell[cell];
@@ -658,6 +666,14 @@ void launch_cg_t(const Operator& op, T* dst, const T* src, hipStream_t s, RowEpi
         if (partial_out) return launch_cg_k<P, T, MODE, NC, 9>(op, dst, src, s, ep, pg, partial_out, fixup, false);
       }
       if (pg.on || cell_boundary) throw std::runtime_error("CG kernel: pending staging / cell boundary need a Chebyshev run");
+      if constexpr (MODE == 0 && NC == 1 && sizeof(T) == 8) {
+        // indices, metric, src and dst beyond the 256 MB Infinity Cache
+        const CGPlan& pl = op.cg;
+        const double bytes = (double)pl.n_cells * (4.0 * op.cg.lidx12_words + 6.0 * sizeof(T)) +
+                             2.0 * sizeof(T) * pl.n_dofs;
+        if (ep.kind == 0 && bytes > 256.0 * (1 << 20))
+          return launch_cg_k<P, T, MODE, NC, 8 | 16>(op, dst, src, s, ep, pg, partial_out, fixup, false);
+      }
       return launch_cg_k<P, T, MODE, NC, 8>(op, dst, src, s, ep, pg, partial_out, fixup, false);
     }
   }
